@@ -1,0 +1,155 @@
+/*
+ * Extensions to the smoke-step C ABI (MI355X implementation).
+ *
+ * None of these entry points exists in the reference; a caller that uses only
+ * visual-simulation-of-smoke-3d.h gets the reference semantics unchanged
+ * (fixed-dt step, pressure by `pressure_iterations` plain CG iterations from a
+ * zero initial guess, backend-cuda.cu:1788-1824).  The extensions add what the
+ * reference ABI cannot express:
+ *   - a tolerance-terminated pressure solve (Jacobi PCG with FP64 reductions
+ *     and FP64 iterative refinement) and its statistics,
+ *   - a CFL-adaptive time step computed on the device,
+ *   - import/export of the full MAC state (the reference can neither set nor
+ *     read face velocities: its velocity view is cell-centred,
+ *     backend-cuda.cu:2243-2248),
+ *   - running a sub-range of the step's stages (stage-level parity checks),
+ *   - on-disk checkpoint / restore of the state,
+ *   - per-kernel device timing and a stream-copy bandwidth probe.
+ * Host-pointer entry points here are synchronous; the rest are stream-ordered.
+ */
+#ifndef VISUAL_SIMULATION_OF_SMOKE_3D_EXT_H
+#define VISUAL_SIMULATION_OF_SMOKE_3D_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "visual-simulation-of-smoke-3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum SmokeSimulationPressureMode {
+    /* Reference semantics: exactly `pressure_iterations` unpreconditioned CG
+       iterations from x = 0, no convergence test (backend-cuda.cu:1800-1823). */
+    SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS = 0,
+    /* Iterate until the true residual ||b - A x||_2 / ||b||_2, evaluated in
+       FP64, is at most `relative_tolerance`. */
+    SMOKE_SIMULATION_PRESSURE_TOLERANCE = 1,
+} SmokeSimulationPressureMode;
+
+typedef enum SmokeSimulationPreconditioner {
+    SMOKE_SIMULATION_PRECONDITIONER_NONE = 0,
+    SMOKE_SIMULATION_PRECONDITIONER_JACOBI = 1,
+} SmokeSimulationPreconditioner;
+
+typedef struct SmokeSimulationSolverOptions {
+    uint32_t pressure_mode;       /* SmokeSimulationPressureMode */
+    uint32_t preconditioner;      /* SmokeSimulationPreconditioner (tolerance mode) */
+    double relative_tolerance;    /* e.g. 1e-6 */
+    int32_t max_iterations;       /* inner PCG iterations per solve, all passes */
+    int32_t max_refinements;      /* FP64 refinement passes after the first inner solve */
+    uint32_t warm_start;          /* start from the previous step's FP64 pressure */
+    uint32_t adaptive_dt;         /* dt = min(dt_max, cfl * h / max|u_face|) */
+    float cfl;
+    float dt_max;
+    int32_t check_interval;       /* PCG iterations between host convergence polls */
+    int32_t reserved;
+} SmokeSimulationSolverOptions;
+
+typedef struct SmokeSimulationSolverStats {
+    int32_t iterations;           /* inner PCG iterations of the last solve */
+    int32_t refinements;          /* FP64 refinement passes of the last solve */
+    int32_t converged;            /* tolerance reached (tolerance mode) */
+    int32_t anchor;               /* pinned pressure cell (first fluid cell) */
+    double relative_residual;     /* true FP64 ||b - A x|| / ||b|| after the last solve */
+    double rhs_norm;              /* ||b||_2 */
+    float dt;                     /* time step used by the last step */
+    float max_face_speed;         /* max |u_face| at the start of the last step */
+    uint64_t steps;               /* steps taken by this context */
+} SmokeSimulationSolverStats;
+
+/* Host-side copy of the simulation state.  Any pointer may be NULL (skipped).
+   Face arrays are x-fastest: velocity_x is (nx+1)*ny*nz, velocity_y is
+   nx*(ny+1)*nz, velocity_z is nx*ny*(nz+1); cell arrays are nx*ny*nz. */
+typedef struct SmokeSimulationHostState {
+    float* velocity_x;
+    float* velocity_y;
+    float* velocity_z;
+    float* density;
+    float* temperature;
+    float* pressure;
+    /* intermediate (pre-projection) face velocities, export only */
+    float* velocity_star_x;
+    float* velocity_star_y;
+    float* velocity_star_z;
+} SmokeSimulationHostState;
+
+/* Stages of one step, in order. */
+typedef enum SmokeSimulationStage {
+    SMOKE_SIMULATION_STAGE_FORCES = 0,          /* dt, solid T, vorticity, buoyancy, confinement, faces += dt f */
+    SMOKE_SIMULATION_STAGE_ADVECT_VELOCITY = 1, /* u* = advect(u) */
+    SMOKE_SIMULATION_STAGE_PRESSURE = 2,        /* b = rhs(u*), solve A p = b */
+    SMOKE_SIMULATION_STAGE_PROJECT = 3,         /* u = u* - dt grad p / h */
+    SMOKE_SIMULATION_STAGE_SCALARS = 4,         /* T, rho advection, solid fill */
+} SmokeSimulationStage;
+
+typedef struct SmokeSimulationKernelTiming {
+    uint64_t pcg_iterations;      /* iterations timed */
+    double pcg_pass_a_ms;         /* summed device time of the p-update + SpMV + p.Ap pass */
+    double pcg_pass_b_ms;         /* summed device time of the x,r update + r.z pass */
+    uint64_t pcg_pass_a_launches;
+    uint64_t pcg_pass_b_launches;
+    double step_ms;               /* summed device time of whole steps */
+    uint64_t steps;
+} SmokeSimulationKernelTiming;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_default_solver_options(SmokeSimulationSolverOptions* out_options);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_solver_options(SmokeSimulationContext context, const SmokeSimulationSolverOptions* options);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_solver_options(SmokeSimulationContext context, SmokeSimulationSolverOptions* out_options);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_solver_stats(SmokeSimulationContext context, SmokeSimulationSolverStats* out_stats);
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_import_state(SmokeSimulationContext context, const SmokeSimulationHostState* state);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_state(SmokeSimulationContext context, const SmokeSimulationHostState* state);
+/* The FP64 pressure accumulator of the tolerance-mode solve (nx*ny*nz doubles). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_pressure_f64(SmokeSimulationContext context, double* values);
+
+/* Runs stages first..last (inclusive) of one step; the full step is 0..4. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_run_stages(SmokeSimulationContext context, int32_t first_stage, int32_t last_stage);
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContext context, const char* path);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContext context, const char* path);
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_kernel_timing(SmokeSimulationContext context, int32_t enable);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_kernel_timing(SmokeSimulationContext context, SmokeSimulationKernelTiming* out_timing);
+
+/* Spherical plume emitter written straight into the density and temperature
+   source buffers (the device-side form of the source set-up the reference's
+   benchmark does on the host, benchmark.cu:130-147).  Cells whose centre lies
+   within `radius` of `center` (both in cell units) get density source
+   `density_rate` and temperature source `temperature_rate + temperature_noise
+   * U`, U uniform in [-1, 1) from an integer hash of (seed, step, cell); every
+   other cell gets 0.  Stream-ordered. */
+typedef struct SmokeSimulationPlumeSource {
+    float center_x;
+    float center_y;
+    float center_z;
+    float radius;
+    float density_rate;
+    float temperature_rate;
+    float temperature_noise;
+    uint32_t seed;
+} SmokeSimulationPlumeSource;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_plume_sources(SmokeSimulationContext context, const SmokeSimulationPlumeSource* source, uint32_t step);
+
+/* Device-to-device copy of `bytes` bytes, `repetitions` times, on a private
+   stream; reports the achieved read+write bandwidth (the stream-copy ceiling
+   that roofline fractions are read against). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gb_per_s, double* out_ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISUAL_SIMULATION_OF_SMOKE_3D_EXT_H */

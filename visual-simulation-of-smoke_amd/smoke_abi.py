@@ -1,0 +1,421 @@
+"""ctypes mirror of the smoke-step C ABI (include/visual-simulation-of-smoke-3d.h)
+and of its extensions (include/visual-simulation-of-smoke-3d-ext.h).
+
+This is host plumbing for tests and the benchmark: it loads the HIP library
+built in-tree (lib/libvisual-simulation-of-smoke.so) and refuses to run
+without it -- there is no CPU fallback.  Device buffers are allocated through
+the HIP runtime (libamdhip64) so that the update_* entry points receive device
+pointers exactly as the reference's callers pass them (benchmark.cu:163-172).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libvisual-simulation-of-smoke.so"
+
+# ---------------------------------------------------------------- enums
+RESULT_OK, RESULT_OUT_OF_MEMORY, RESULT_BACKEND_FAILURE = 0, 1, 2
+NO_SLIP_WALL, FREE_SLIP_WALL, OUTFLOW, PERIODIC = 0, 1, 2, 3
+FIXED_VALUE, ZERO_FLUX, SCALAR_PERIODIC = 0, 1, 2
+ADVECTION_LINEAR, ADVECTION_MONOTONIC_CUBIC = 0, 1
+
+VIEW_KINDS = {
+    "DENSITY": 0, "DENSITY_SOURCE": 1, "TEMPERATURE": 2, "TEMPERATURE_SOURCE": 3, "FORCE": 4,
+    "SOLID_VELOCITY": 5, "SOLID_TEMPERATURE": 6, "FLOW_VELOCITY": 7, "FLOW_VELOCITY_MAGNITUDE": 8,
+    "FLOW_PRESSURE": 9, "FLOW_PRESSURE_RHS": 10, "FLOW_DIVERGENCE": 11, "FLOW_VORTICITY": 12,
+    "FLOW_VORTICITY_MAGNITUDE": 13, "OCCUPANCY": 14,
+}
+LAYOUT_F32_3D, LAYOUT_F32_3D_SOA3 = 0, 1
+
+PRESSURE_FIXED_ITERATIONS, PRESSURE_TOLERANCE = 0, 1
+PRECONDITIONER_NONE, PRECONDITIONER_JACOBI = 0, 1
+STAGE_FORCES, STAGE_ADVECT_VELOCITY, STAGE_PRESSURE, STAGE_PROJECT, STAGE_SCALARS = range(5)
+
+
+# ---------------------------------------------------------------- structs
+class FlowFace(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("velocity_x", C.c_float), ("velocity_y", C.c_float),
+                ("velocity_z", C.c_float), ("pressure", C.c_float)]
+
+
+class FlowBoundary(C.Structure):
+    _fields_ = [(n, FlowFace) for n in ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")]
+
+
+class ScalarFace(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("value", C.c_float)]
+
+
+class ScalarBoundary(C.Structure):
+    _fields_ = [(n, ScalarFace) for n in ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("cell_size", C.c_float), ("dt", C.c_float), ("pressure_iterations", C.c_int32),
+        ("ambient_temperature", C.c_float), ("buoyancy_density_factor", C.c_float),
+        ("buoyancy_temperature_factor", C.c_float), ("vorticity_confinement", C.c_float),
+        ("scalar_advection_mode", C.c_uint32),
+        ("flow_boundary", FlowBoundary), ("density_boundary", ScalarBoundary),
+        ("temperature_boundary", ScalarBoundary),
+    ]
+
+
+class CreateDesc(C.Structure):
+    _fields_ = [("config", Config), ("stream", C.c_void_p), ("initial_density", C.c_float),
+                ("initial_temperature", C.c_float)]
+
+
+class ViewRequest(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("consumer_stream", C.c_void_p)]
+
+
+class View(C.Structure):
+    _fields_ = [("layout", C.c_uint32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("row_stride_bytes", C.c_uint64), ("slice_stride_bytes", C.c_uint64),
+                ("data0", C.c_void_p), ("data1", C.c_void_p), ("data2", C.c_void_p)]
+
+
+class SolverOptions(C.Structure):
+    _fields_ = [("pressure_mode", C.c_uint32), ("preconditioner", C.c_uint32),
+                ("relative_tolerance", C.c_double), ("max_iterations", C.c_int32),
+                ("max_refinements", C.c_int32), ("warm_start", C.c_uint32), ("adaptive_dt", C.c_uint32),
+                ("cfl", C.c_float), ("dt_max", C.c_float), ("check_interval", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
+                ("anchor", C.c_int32), ("relative_residual", C.c_double), ("rhs_norm", C.c_double),
+                ("dt", C.c_float), ("max_face_speed", C.c_float), ("steps", C.c_uint64)]
+
+
+class HostState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "velocity_x", "velocity_y", "velocity_z", "density", "temperature", "pressure",
+        "velocity_star_x", "velocity_star_y", "velocity_star_z")]
+
+
+class KernelTiming(C.Structure):
+    _fields_ = [("pcg_iterations", C.c_uint64), ("pcg_pass_a_ms", C.c_double), ("pcg_pass_b_ms", C.c_double),
+                ("pcg_pass_a_launches", C.c_uint64), ("pcg_pass_b_launches", C.c_uint64),
+                ("step_ms", C.c_double), ("steps", C.c_uint64)]
+
+
+class PlumeSource(C.Structure):
+    _fields_ = [("center_x", C.c_float), ("center_y", C.c_float), ("center_z", C.c_float),
+                ("radius", C.c_float), ("density_rate", C.c_float), ("temperature_rate", C.c_float),
+                ("temperature_noise", C.c_float), ("seed", C.c_uint32)]
+
+
+# ---------------------------------------------------------------- config helpers
+def make_config(nx, ny, nz, *, cell_size=None, dt=0.01, pressure_iterations=64, ambient=0.0,
+                buoyancy_density=0.05, buoyancy_temperature=0.25, vorticity=2.0, cubic=True,
+                flow=None, density_bc=None, temperature_bc=None) -> Config:
+    """Builds a Config; `flow` is six (type, vx, vy, vz, p) tuples (x-,x+,y-,y+,z-,z+),
+    `density_bc` / `temperature_bc` six (type, value) tuples.  Defaults: closed
+    no-slip box, FIXED_VALUE scalars at 0 (density) / ambient (temperature)."""
+    cfg = Config()
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.cell_size = cell_size if cell_size is not None else 1.0 / nx
+    cfg.dt = dt
+    cfg.pressure_iterations = pressure_iterations
+    cfg.ambient_temperature = ambient
+    cfg.buoyancy_density_factor = buoyancy_density
+    cfg.buoyancy_temperature_factor = buoyancy_temperature
+    cfg.vorticity_confinement = vorticity
+    cfg.scalar_advection_mode = ADVECTION_MONOTONIC_CUBIC if cubic else ADVECTION_LINEAR
+    names = ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")
+    flow = flow or [(NO_SLIP_WALL, 0.0, 0.0, 0.0, 0.0)] * 6
+    for n, f in zip(names, flow):
+        face = getattr(cfg.flow_boundary, n)
+        face.type, face.velocity_x, face.velocity_y, face.velocity_z, face.pressure = f
+    density_bc = density_bc or [(FIXED_VALUE, 0.0)] * 6
+    temperature_bc = temperature_bc or [(FIXED_VALUE, ambient)] * 6
+    for n, d, t in zip(names, density_bc, temperature_bc):
+        getattr(cfg.density_boundary, n).type, getattr(cfg.density_boundary, n).value = d
+        getattr(cfg.temperature_boundary, n).type, getattr(cfg.temperature_boundary, n).value = t
+    return cfg
+
+
+def face_shape(cfg: Config, comp: int):
+    """numpy (z, y, x) shape of the MAC face array of component `comp`."""
+    n = [cfg.nx, cfg.ny, cfg.nz]
+    n[comp] += 1
+    return (n[2], n[1], n[0])
+
+
+def cell_shape(cfg: Config):
+    return (cfg.nz, cfg.ny, cfg.nx)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- HIP runtime plumbing
+class Hip:
+    """The handful of HIP runtime calls the harness needs (device buffers, streams, sync)."""
+
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        L = self.lib
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        L.hipDeviceSynchronize.argtypes = []
+        L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipStreamDestroy.argtypes = [C.c_void_p]
+        L.hipStreamSynchronize.argtypes = [C.c_void_p]
+        L.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipStreamCreate",
+                  "hipStreamDestroy", "hipStreamSynchronize", "hipGetDeviceCount"):
+            getattr(L, f).restype = C.c_int
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with hipError {rc}")
+
+    def device_count(self) -> int:
+        n = C.c_int(0)
+        rc = self.lib.hipGetDeviceCount(C.byref(n))
+        return n.value if rc == 0 else 0
+
+    def malloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.lib.hipMalloc(C.byref(p), max(int(nbytes), 1)), "hipMalloc")
+        return p.value
+
+    def free(self, p: int):
+        if p:
+            self.check(self.lib.hipFree(C.c_void_p(p)), "hipFree")
+
+    def upload(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.malloc(arr.nbytes)
+        self.check(self.lib.hipMemcpy(C.c_void_p(p), C.c_void_p(arr.ctypes.data), arr.nbytes, self.H2D), "hipMemcpy H2D")
+        return p
+
+    def download(self, p: int, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        self.check(self.lib.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(p), out.nbytes, self.D2H), "hipMemcpy D2H")
+        return out
+
+    def synchronize(self):
+        self.check(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+# ---------------------------------------------------------------- library
+_LIB = None
+
+
+def load_library(path: Path | str | None = None):
+    """Loads the in-tree HIP build; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python __graft_entry__.py build` (no CPU fallback exists)")
+    L = C.CDLL(str(p))
+    R = C.c_int
+    sigs = {
+        "smoke_simulation_create_context_cuda": [C.POINTER(CreateDesc), C.POINTER(C.c_void_p)],
+        "smoke_simulation_destroy_context_cuda": [C.c_void_p],
+        "smoke_simulation_update_density_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_density_source_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_temperature_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_temperature_source_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_force_cuda": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_occupancy_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_solid_velocity_cuda": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+        "smoke_simulation_update_solid_temperature_cuda": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_step_cuda": [C.c_void_p],
+        "smoke_simulation_get_view_cuda": [C.c_void_p, C.POINTER(ViewRequest), C.POINTER(View)],
+        "smoke_simulation_ext_default_solver_options": [C.POINTER(SolverOptions)],
+        "smoke_simulation_ext_set_solver_options": [C.c_void_p, C.POINTER(SolverOptions)],
+        "smoke_simulation_ext_get_solver_options": [C.c_void_p, C.POINTER(SolverOptions)],
+        "smoke_simulation_ext_get_solver_stats": [C.c_void_p, C.POINTER(SolverStats)],
+        "smoke_simulation_ext_import_state": [C.c_void_p, C.POINTER(HostState)],
+        "smoke_simulation_ext_export_state": [C.c_void_p, C.POINTER(HostState)],
+        "smoke_simulation_ext_export_pressure_f64": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_ext_run_stages": [C.c_void_p, C.c_int32, C.c_int32],
+        "smoke_simulation_ext_save_checkpoint": [C.c_void_p, C.c_char_p],
+        "smoke_simulation_ext_load_checkpoint": [C.c_void_p, C.c_char_p],
+        "smoke_simulation_ext_set_kernel_timing": [C.c_void_p, C.c_int32],
+        "smoke_simulation_ext_get_kernel_timing": [C.c_void_p, C.POINTER(KernelTiming)],
+        "smoke_simulation_ext_measure_copy_bandwidth": [C.c_uint64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "smoke_simulation_ext_plume_sources": [C.c_void_p, C.POINTER(PlumeSource), C.c_uint32],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = R
+    if path is None:
+        _LIB = L
+    return L
+
+
+def abi_symbols_from_headers(include_dir: Path) -> list[str]:
+    """Every function name declared with SMOKE_SIMULATION_API in include/*.h."""
+    import re
+    names = []
+    for h in sorted(Path(include_dir).glob("*.h")):
+        names += re.findall(r"SMOKE_SIMULATION_API\s+\w+\s+(\w+)\s*\(", h.read_text())
+    return names
+
+
+class SmokeError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str):
+    if rc != RESULT_OK:
+        raise SmokeError(f"{what} returned {rc}")
+
+
+class SmokeContext:
+    """One simulation context (owns its stream) driven through the C ABI."""
+
+    def __init__(self, cfg: Config, initial_density=0.0, initial_temperature=None, hip: Hip | None = None):
+        self.lib = load_library()
+        self.hip = hip or Hip()
+        self.cfg = cfg
+        desc = CreateDesc()
+        desc.config = cfg
+        desc.stream = None
+        desc.initial_density = initial_density
+        desc.initial_temperature = cfg.ambient_temperature if initial_temperature is None else initial_temperature
+        h = C.c_void_p()
+        check(self.lib.smoke_simulation_create_context_cuda(C.byref(desc), C.byref(h)), "create_context")
+        self.handle = h
+        self._owned = []
+
+    def close(self):
+        if self.handle:
+            check(self.lib.smoke_simulation_destroy_context_cuda(self.handle), "destroy_context")
+            self.handle = None
+        for p in self._owned:
+            self.hip.free(p)
+        self._owned = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- device uploads through update_*
+    def _dev(self, arr, dtype=np.float32):
+        if arr is None:
+            return None
+        p = self.hip.upload(np.ascontiguousarray(arr, dtype=dtype))
+        self._owned.append(p)
+        return C.c_void_p(p)
+
+    def _release(self):
+        self.hip.synchronize()
+        for p in self._owned:
+            self.hip.free(p)
+        self._owned = []
+
+    def update(self, name: str, *arrays, dtype=np.float32):
+        fn = getattr(self.lib, f"smoke_simulation_update_{name}_cuda")
+        ptrs = [self._dev(a, dtype) for a in arrays]
+        check(fn(self.handle, *ptrs), f"update_{name}")
+        self._release()
+
+    def set_occupancy(self, occ: np.ndarray | None):
+        self.update("occupancy", occ, dtype=np.uint8)
+
+    def step(self, count: int = 1):
+        for _ in range(count):
+            check(self.lib.smoke_simulation_step_cuda(self.handle), "step")
+
+    def run_stages(self, first: int, last: int):
+        check(self.lib.smoke_simulation_ext_run_stages(self.handle, first, last), "run_stages")
+
+    def synchronize(self):
+        self.hip.synchronize()
+
+    def view(self, kind: str):
+        req = ViewRequest(VIEW_KINDS[kind], None)
+        v = View()
+        check(self.lib.smoke_simulation_get_view_cuda(self.handle, C.byref(req), C.byref(v)), f"get_view {kind}")
+        self.hip.synchronize()
+        shape = (v.nz, v.ny, v.nx)
+        if v.layout == LAYOUT_F32_3D:
+            return self.hip.download(v.data0, shape)
+        return tuple(self.hip.download(p, shape) for p in (v.data0, v.data1, v.data2))
+
+    def set_solver_options(self, **kw):
+        o = SolverOptions()
+        check(self.lib.smoke_simulation_ext_get_solver_options(self.handle, C.byref(o)), "get_solver_options")
+        for k, val in kw.items():
+            setattr(o, k, val)
+        check(self.lib.smoke_simulation_ext_set_solver_options(self.handle, C.byref(o)), "set_solver_options")
+
+    def stats(self) -> SolverStats:
+        s = SolverStats()
+        check(self.lib.smoke_simulation_ext_get_solver_stats(self.handle, C.byref(s)), "get_solver_stats")
+        return s
+
+    def import_state(self, u=None, v=None, w=None, density=None, temperature=None, pressure=None):
+        keep = {k: (None if a is None else np.ascontiguousarray(a, dtype=np.float32))
+                for k, a in dict(velocity_x=u, velocity_y=v, velocity_z=w, density=density,
+                                 temperature=temperature, pressure=pressure).items()}
+        hs = HostState()
+        for k, a in keep.items():
+            setattr(hs, k, None if a is None else a.ctypes.data)
+        check(self.lib.smoke_simulation_ext_import_state(self.handle, C.byref(hs)), "import_state")
+
+    def export_state(self) -> dict:
+        cfg = self.cfg
+        out = {
+            "velocity_x": np.empty(face_shape(cfg, 0), np.float32),
+            "velocity_y": np.empty(face_shape(cfg, 1), np.float32),
+            "velocity_z": np.empty(face_shape(cfg, 2), np.float32),
+            "density": np.empty(cell_shape(cfg), np.float32),
+            "temperature": np.empty(cell_shape(cfg), np.float32),
+            "pressure": np.empty(cell_shape(cfg), np.float32),
+            "velocity_star_x": np.empty(face_shape(cfg, 0), np.float32),
+            "velocity_star_y": np.empty(face_shape(cfg, 1), np.float32),
+            "velocity_star_z": np.empty(face_shape(cfg, 2), np.float32),
+        }
+        hs = HostState()
+        for k, a in out.items():
+            setattr(hs, k, a.ctypes.data)
+        check(self.lib.smoke_simulation_ext_export_state(self.handle, C.byref(hs)), "export_state")
+        return out
+
+    def export_pressure_f64(self) -> np.ndarray:
+        out = np.empty(cell_shape(self.cfg), np.float64)
+        check(self.lib.smoke_simulation_ext_export_pressure_f64(self.handle, C.c_void_p(out.ctypes.data)), "export_pressure_f64")
+        return out
+
+    def plume_sources(self, src: PlumeSource, step: int):
+        check(self.lib.smoke_simulation_ext_plume_sources(self.handle, C.byref(src), step), "plume_sources")
+
+    def set_kernel_timing(self, enable: bool):
+        check(self.lib.smoke_simulation_ext_set_kernel_timing(self.handle, 1 if enable else 0), "set_kernel_timing")
+
+    def kernel_timing(self) -> KernelTiming:
+        t = KernelTiming()
+        check(self.lib.smoke_simulation_ext_get_kernel_timing(self.handle, C.byref(t)), "get_kernel_timing")
+        return t
+
+
+def measure_copy_bandwidth(nbytes: int, reps: int):
+    lib = load_library()
+    gbps, ms = C.c_double(), C.c_double()
+    check(lib.smoke_simulation_ext_measure_copy_bandwidth(nbytes, reps, C.byref(gbps), C.byref(ms)), "measure_copy_bandwidth")
+    return gbps.value, ms.value
