@@ -1,0 +1,158 @@
+"""Benchmark of the MI355X smoke step on the 256^3 plume workload.
+
+Workload (the headline configuration): 256^3 cells, h = 1/256, closed no-slip
+box, FIXED_VALUE 0 scalars, monotone-cubic advection, buoyancy 0.05 rho /
+0.25 T (y up), vorticity confinement 2.0, CFL-adaptive dt (0.5 h / max|u|,
+capped at 0.01), spherical source of radius 24 cells at (128, 32, 128)
+injecting density 1 and temperature 1 +- 0.05 (seeded, regenerated each
+step).  Every pressure solve is Jacobi-PCG with FP64 reductions and FP64
+iterative refinement to a true relative residual ||b - Ax|| / ||b|| <= 1e-6.
+
+One "step" is one full smoke step (forces, advection, pressure solve to
+tolerance, projection, scalar advection) with all inputs resident in HBM.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "visual-simulation-of-smoke_amd"))
+
+import smoke_abi as abi  # noqa: E402
+
+METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+PASS_A_BYTES_PER_CELL = 13   # reads r, p_old (4+4), flags (1); writes p_new (4)
+PASS_B_BYTES_PER_CELL = 21   # reads p_new, e, r (4+4+4), flags (1); writes e, r (4+4)
+
+
+def workload_config(n: int) -> abi.Config:
+    return abi.make_config(n, n, n, cell_size=1.0 / n, dt=0.01, pressure_iterations=64, ambient=0.0,
+                           buoyancy_density=0.05, buoyancy_temperature=0.25, vorticity=2.0, cubic=True)
+
+
+def plume(n: int) -> abi.PlumeSource:
+    s = n / 256.0
+    return abi.PlumeSource(128.0 * s, 32.0 * s, 128.0 * s, 24.0 * s, 1.0, 1.0, 0.05, 2)
+
+
+def solver_options(ctx: abi.SmokeContext, check_interval: int):
+    ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI,
+                           relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=1,
+                           adaptive_dt=1, cfl=0.5, dt_max=0.01, check_interval=check_interval)
+
+
+def cpu_baseline(n: int, iterations_per_step: float, threads: int):
+    """Bounded CPU sample of the same workload with the FP32 host port of the
+    step (the oracle source built with R = float, FP64 reductions, OpenMP):
+    one step's non-solve stages timed once plus 12 CG iterations timed, scaled
+    to the GPU run's mean PCG iterations per step."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as orc
+    cfg = workload_config(n)
+    opts = orc.options(mode=1, preconditioner=1, tol=1e-6, max_refinements=4, warm_start=1, adaptive_dt=1, cfl=0.5,
+                       dt_max=0.01, inner_reduction=1e-4)
+    t0 = time.time()
+    other, per_iter = orc.f32_step_sample(cfg, opts, plume(n), 12, threads)
+    wall = time.time() - t0
+    seconds_per_step = other + iterations_per_step * per_iter
+    return {
+        "value": 1.0 / seconds_per_step, "unit": "steps/s", "cores": threads, "kind": "port",
+        "sample": (f"{n}^3 step from rest on {threads} host threads: non-solve stages {other:.2f} s once + 12 CG "
+                   f"iterations at {per_iter * 1e3:.1f} ms each, scaled to {iterations_per_step:.0f} iterations/step "
+                   f"(the GPU run's mean); {wall:.1f} s of CPU work"),
+    }
+
+
+def load_traffic():
+    f = REPO / "profiles" / "pcg_pass_b_256.traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--check-interval", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
+        sys.exit(1)
+
+    n = args.size
+    hip = abi.Hip()
+    copy_gbps, copy_ms = abi.measure_copy_bandwidth(1 << 30, 20)
+    cfg = workload_config(n)
+    src = plume(n)
+    iters, resid, dts = [], [], []
+    with abi.SmokeContext(cfg, hip=hip) as ctx:
+        solver_options(ctx, args.check_interval)
+        step_index = 0
+        for _ in range(args.warmup):
+            ctx.plume_sources(src, step_index)
+            ctx.step()
+            step_index += 1
+        hip.synchronize()
+        ctx.set_kernel_timing(not args.no_timing)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.plume_sources(src, step_index)
+            ctx.step()
+            st = ctx.stats()          # stream-synchronising; reads the solve's true FP64 residual
+            iters.append(st.iterations)
+            resid.append(st.relative_residual)
+            dts.append(st.dt)
+            step_index += 1
+        hip.synchronize()
+        elapsed = time.perf_counter() - t0
+        timing = ctx.kernel_timing()
+
+    steps_per_s = args.steps / elapsed
+    cells = n ** 3
+    mean_iters = sum(iters) / len(iters)
+    roofline = None
+    if timing.pcg_pass_b_launches > 0:
+        b_ms = timing.pcg_pass_b_ms / timing.pcg_pass_b_launches
+        a_ms = timing.pcg_pass_a_ms / timing.pcg_pass_a_launches
+        achieved = PASS_B_BYTES_PER_CELL * cells / (b_ms * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm", "kernel": "pcg_pass_kernel<64,8,false,true> (PCG pass B: e += alpha p, r -= alpha Ap, r.z, r.r)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": load_traffic(), "algorithmic_bytes_per_launch": PASS_B_BYTES_PER_CELL * cells,
+            "avg_launch_ms": round(b_ms, 5), "launches_timed": timing.pcg_pass_b_launches,
+            "pass_a": {"avg_launch_ms": round(a_ms, 5), "achieved": round(PASS_A_BYTES_PER_CELL * cells / (a_ms * 1e-3) / 1e9, 1),
+                       "algorithmic_bytes_per_launch": PASS_A_BYTES_PER_CELL * cells},
+            "stream_copy_gbps": round(copy_gbps, 1),
+        }
+    out = {
+        "metric": METRIC, "value": round(steps_per_s, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n}^3 plume, plain Jacobi-PCG (FP64 reductions + refinement) to true rel-resid 1e-6, cubic advection, adaptive dt",
+                   "grid": [n, n, n], "solver": "jacobi_pcg", "relative_tolerance": 1e-6, "learned_fpcg": None},
+        "pcg_iterations_per_step": round(mean_iters, 1), "max_relative_residual": max(resid),
+        "dt_range": [min(dts), max(dts)], "roofline": roofline, "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(n, mean_iters, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,8 @@ def lib():
         L.oracle_noise_bits.restype = C.c_uint32
         L.oracle_f32_benchmark.argtypes = [cfgp, C.POINTER(Options), C.POINTER(abi.PlumeSource), C.c_int, C.c_int,
                                            C.c_int, C.POINTER(C.c_double), C.POINTER(Stats)]
+        L.oracle_f32_step_sample.argtypes = [cfgp, C.POINTER(Options), C.POINTER(abi.PlumeSource), C.c_int, C.c_int,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -238,3 +240,11 @@ def f32_benchmark(cfg, opts: Options, src: abi.PlumeSource | None, warmup: int, 
     lib().oracle_f32_benchmark(C.byref(cfg), C.byref(opts), C.byref(src) if src is not None else None, warmup, steps,
                                threads, C.byref(secs), C.byref(st))
     return secs.value, st
+
+
+def f32_step_sample(cfg, opts: Options, src: abi.PlumeSource | None, cg_iterations: int, threads: int):
+    """(seconds of one step without its pressure iterations, seconds per CG iteration)."""
+    other, it = C.c_double(), C.c_double()
+    lib().oracle_f32_step_sample(C.byref(cfg), C.byref(opts), C.byref(src) if src is not None else None, cg_iterations, threads,
+                                 C.byref(other), C.byref(it))
+    return other.value, it.value

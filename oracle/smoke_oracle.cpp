@@ -238,7 +238,9 @@ R sample(const Geo& g, FieldKind k, bool cubic, const R* f, R x, R y, R z) {
     int b[3];
     R t[3];
     for (int a = 0; a < 3; ++a) {
-        gc[a] = a == k.comp ? p[a] / h : p[a] / h - R(0.5);
+        // beyond two cells outside the domain all taps equal the same ghost
+        // value: the clamp changes no finite result and keeps indices in range
+        gc[a] = std::fmin(std::fmax(a == k.comp ? p[a] / h : p[a] / h - R(0.5), R(-4)), R(g.n[a]) + R(4));
         b[a] = int(std::floor(gc[a]));
         t[a] = gc[a] - R(b[a]);
     }
@@ -1062,48 +1064,106 @@ void oracle_nearest_fluid(const SmokeSimulationConfig* cfg, const uint8_t* occ, 
     });
 }
 
-int oracle_f32_benchmark(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, int warmup, int steps, int threads, double* seconds, OracleStats* stats) {
+namespace {
+
+// FP32 state for the CPU baseline (host port of the step, FP64 reductions).
+struct F32Run {
+    Geo g;
+    size_t n;
+    std::vector<float> u, v, w, us, vs, ws, rho, T, rs, ts, zero, tsol, rhs, fx, fy, fz, cx, cy, cz, ox, oy, oz, om, dv, vm;
+    std::vector<double> pressure, d_rs, d_ts;
+    std::vector<uint8_t> occ;
+    Fields<float> f{};
+    explicit F32Run(const SmokeSimulationConfig* cfg) : g(make_geo(cfg)), n(size_t(g.cells)) {
+        u.assign(size_t(g.faces[0]), 0.f);
+        v.assign(size_t(g.faces[1]), 0.f);
+        w.assign(size_t(g.faces[2]), 0.f);
+        us = u;
+        vs = v;
+        ws = w;
+        for (auto* a : {&rho, &rs, &ts, &zero, &rhs, &fx, &fy, &fz, &cx, &cy, &cz, &ox, &oy, &oz, &om, &dv, &vm}) a->assign(n, 0.f);
+        T.assign(n, cfg->ambient_temperature);
+        tsol.assign(n, cfg->ambient_temperature);
+        pressure.assign(n, 0.0);
+        d_rs.assign(n, 0.0);
+        d_ts.assign(n, 0.0);
+        occ.assign(n, 0);
+        f.u = u.data(); f.v = v.data(); f.w = w.data(); f.us = us.data(); f.vs = vs.data(); f.ws = ws.data();
+        f.rho = rho.data(); f.T = T.data(); f.rho_src = rs.data(); f.T_src = ts.data();
+        f.fux = zero.data(); f.fuy = zero.data(); f.fuz = zero.data();
+        f.svx = zero.data(); f.svy = zero.data(); f.svz = zero.data();
+        f.T_solid = tsol.data(); f.rhs = rhs.data();
+        f.fx = fx.data(); f.fy = fy.data(); f.fz = fz.data(); f.cx = cx.data(); f.cy = cy.data(); f.cz = cz.data();
+        f.ox = ox.data(); f.oy = oy.data(); f.oz = oz.data(); f.om = om.data(); f.div = dv.data(); f.vmag = vm.data();
+        f.pressure = pressure.data();
+        f.occ = occ.data();
+    }
+    void sources(const SmokeSimulationConfig* cfg, const OraclePlumeSource* src, uint32_t step) {
+        if (!src) return;
+        oracle_plume_sources(cfg, src, step, d_rs.data(), d_ts.data());
+        for (size_t i = 0; i < n; ++i) {
+            rs[i] = float(d_rs[i]);
+            ts[i] = float(d_ts[i]);
+        }
+    }
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void set_threads(int threads) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
-    const Geo g = make_geo(cfg);
-    const size_t n = size_t(g.cells);
-    std::vector<float> u(size_t(g.faces[0]), 0.f), v(size_t(g.faces[1]), 0.f), w(size_t(g.faces[2]), 0.f);
-    std::vector<float> us(u), vs(v), ws(w);
-    std::vector<float> rho(n, 0.f), T(n, cfg->ambient_temperature), rs(n, 0.f), ts(n, 0.f), zero(n, 0.f), tsol(n, cfg->ambient_temperature), rhs(n, 0.f);
-    std::vector<float> fx(n), fy(n), fz(n), cx(n), cy(n), cz(n), ox(n), oy(n), oz(n), om(n), dv(n), vm(n);
-    std::vector<double> pressure(n, 0.0), d_rs(n), d_ts(n);
-    std::vector<uint8_t> occ(n, 0);
-    Fields<float> f{};
-    f.u = u.data(); f.v = v.data(); f.w = w.data(); f.us = us.data(); f.vs = vs.data(); f.ws = ws.data();
-    f.rho = rho.data(); f.T = T.data(); f.rho_src = rs.data(); f.T_src = ts.data();
-    f.fux = zero.data(); f.fuy = zero.data(); f.fuz = zero.data();
-    f.svx = zero.data(); f.svy = zero.data(); f.svz = zero.data();
-    f.T_solid = tsol.data(); f.rhs = rhs.data();
-    f.fx = fx.data(); f.fy = fy.data(); f.fz = fz.data(); f.cx = cx.data(); f.cy = cy.data(); f.cz = cz.data();
-    f.ox = ox.data(); f.oy = oy.data(); f.oz = oz.data(); f.om = om.data(); f.div = dv.data(); f.vmag = vm.data();
-    f.pressure = pressure.data();
-    f.occ = occ.data();
+}
+
+}  // namespace
+
+int oracle_f32_benchmark(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, int warmup, int steps, int threads, double* seconds, OracleStats* stats) {
+    set_threads(threads);
+    F32Run run(cfg);
     OracleStats local{};
     float dt = cfg->dt;
     double elapsed = 0.0;
     for (int step = 0; step < warmup + steps; ++step) {
-        if (src) {
-            oracle_plume_sources(cfg, src, uint32_t(step), d_rs.data(), d_ts.data());
-            for (size_t i = 0; i < n; ++i) {
-                rs[i] = float(d_rs[i]);
-                ts[i] = float(d_ts[i]);
-            }
-        }
+        run.sources(cfg, src, uint32_t(step));
         const auto t0 = std::chrono::steady_clock::now();
-        run_stages(cfg, opt, f, 0, 4, &local, dt);
-        const auto t1 = std::chrono::steady_clock::now();
-        if (step >= warmup) elapsed += std::chrono::duration<double>(t1 - t0).count();
+        run_stages(cfg, opt, run.f, 0, 4, &local, dt);
+        if (step >= warmup) elapsed += seconds_since(t0);
     }
     if (seconds) *seconds = elapsed;
     if (stats) *stats = local;
+    return 0;
+}
+
+// Bounded CPU sample of one step from the initial state, timed by stage:
+// everything but the pressure iterations once, and `cg_iterations` plain CG
+// iterations for the per-iteration cost (the pressure stage time divided by
+// the iteration count).  Stage costs do not depend on the flow state.
+int oracle_f32_step_sample(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, int cg_iterations, int threads, double* t_other, double* t_iteration) {
+    set_threads(threads);
+    F32Run run(cfg);
+    OracleStats local{};
+    float dt = cfg->dt;
+    run.sources(cfg, src, 0);
+    SmokeSimulationConfig fixed = *cfg;
+    fixed.pressure_iterations = cg_iterations > 0 ? cg_iterations : 1;
+    OracleOptions fo = *opt;
+    fo.pressure_mode = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    run_stages(&fixed, &fo, run.f, 0, 1, &local, dt);
+    double other = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
+    run_stages(&fixed, &fo, run.f, 2, 2, &local, dt);
+    const double solve = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
+    run_stages(&fixed, &fo, run.f, 3, 4, &local, dt);
+    other += seconds_since(t0);
+    if (t_other) *t_other = other;
+    if (t_iteration) *t_iteration = solve / fixed.pressure_iterations;
     return 0;
 }
 

@@ -108,4 +108,6 @@ uint32_t oracle_noise_bits(uint32_t seed, uint32_t step, uint32_t cell);
 // CPU baseline: FP32 state, FP64 reductions, OpenMP over `threads`.
 int oracle_f32_benchmark(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src,
                          int warmup, int steps, int threads, double* seconds, OracleStats* stats);
+int oracle_f32_step_sample(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, int cg_iterations,
+                           int threads, double* t_other, double* t_iteration);
 }

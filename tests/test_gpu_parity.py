@@ -1,0 +1,251 @@
+"""HIP step vs the FP64 oracle, stage by stage and for whole steps, through
+the C ABI (B = backend-cuda.cu).
+
+Tolerances (relative L2 over the field unless stated):
+  * stages without interpolation (forces, RHS, projection): 1e-5 -- FP32
+    rounding against FP64;
+  * semi-Lagrangian stages: 2e-4 -- FP32 floor / limiter decisions can flip a
+    cubic tap near cell boundaries (B:378-392, 442-484);
+  * pressure: the GPU's FP64 solution checked against the ORACLE's operator:
+    ||b - A p|| / ||b|| <= 1e-6, plus 1e-3 on p
+    itself against a 1e-10 oracle solve;
+  * whole step from a shared state: 1e-3 (solve tolerance 1e-6 feeds through
+    the projection).
+"""
+import numpy as np
+import pytest
+
+import smoke_abi as abi
+import oracle as orc
+from parity import (O, P, W, load_gpu, load_oracle, make_state, one_step_parity, rel_l2, tolerance_options)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_gpu():
+    abi.load_library()
+    if abi.Hip().device_count() == 0:
+        pytest.skip("no GPU")
+
+
+def _pair(cfg, seed=1, solids=0, solid_motion=False):
+    st = make_state(cfg, seed, solids=solids, solid_motion=solid_motion)
+    ctx = abi.SmokeContext(cfg)
+    load_gpu(ctx, st)
+    s = orc.OracleState(cfg)
+    load_oracle(s, st)
+    return ctx, s, st
+
+
+CONFIGS = {
+    "closed": dict(flow=None),
+    "benchmark": dict(flow=[P, P, W, O, P, P],
+                      density_bc=[(abi.SCALAR_PERIODIC, 0.0)] * 2 + [(abi.FIXED_VALUE, 0.0)] * 2 + [(abi.SCALAR_PERIODIC, 0.0)] * 2,
+                      temperature_bc=[(abi.SCALAR_PERIODIC, 0.0)] * 2 + [(abi.FIXED_VALUE, 0.0)] * 2 + [(abi.SCALAR_PERIODIC, 0.0)] * 2),
+    "mixed": dict(flow=[(abi.FREE_SLIP_WALL, 0, 0, 0, 0), (abi.OUTFLOW, 0, 0, 0, 0.1), W, (abi.NO_SLIP_WALL, 0.2, 0, 0.1, 0), P, P],
+                  density_bc=[(abi.ZERO_FLUX, 0.0), (abi.FIXED_VALUE, 0.0), (abi.ZERO_FLUX, 0.0), (abi.FIXED_VALUE, 0.2),
+                              (abi.SCALAR_PERIODIC, 0.0), (abi.SCALAR_PERIODIC, 0.0)]),
+}
+
+
+def cfg_for(name, n=(20, 24, 18), cubic=True):
+    return abi.make_config(*n, cell_size=1.0 / n[1], dt=0.01, cubic=cubic, vorticity=2.0, **CONFIGS[name])
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_stage_forces(name):
+    cfg = cfg_for(name)
+    ctx, s, _ = _pair(cfg, seed=2, solids=2, solid_motion=True)
+    with ctx:
+        ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_FORCES)
+        g = ctx.export_state()
+    s.run_stages(0, 0, orc.options(mode=0))
+    for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w")):
+        assert rel_l2(g[a], s[b]) < 1e-5, (a, rel_l2(g[a], s[b]))
+    assert rel_l2(g["temperature"], s["temperature"]) == 0.0
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+@pytest.mark.parametrize("cubic", [False, True])
+def test_stage_velocity_advection(name, cubic):
+    cfg = cfg_for(name, cubic=cubic)
+    ctx, s, _ = _pair(cfg, seed=3, solids=2)
+    with ctx:
+        ctx.run_stages(abi.STAGE_ADVECT_VELOCITY, abi.STAGE_ADVECT_VELOCITY)
+        g = ctx.export_state()
+    s.run_stages(1, 1, orc.options(mode=0))
+    for a, b in (("velocity_star_x", "us"), ("velocity_star_y", "vs"), ("velocity_star_z", "ws")):
+        assert rel_l2(g[a], s[b]) < 2e-4, (a, rel_l2(g[a], s[b]))
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_stage_pressure_solve(name):
+    cfg = cfg_for(name)
+    ctx, s, st = _pair(cfg, seed=4, solids=2)
+    with ctx:
+        tolerance_options(ctx, tol=1e-6, warm=0)
+        ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_PRESSURE)
+        rhs = ctx.view("FLOW_PRESSURE_RHS")
+        p64 = ctx.export_pressure_f64()
+        stats = ctx.stats()
+    s.run_stages(0, 2, orc.options(mode=1, tol=1e-10, inner_reduction=1e-12))
+    assert rel_l2(rhs, s["pressure_rhs"]) < 1e-5
+    # the GPU solution against the oracle's operator and the GPU's own b
+    r = rhs.astype(np.float64) - orc.pressure_apply(cfg, st["occupancy"], p64)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(rhs.astype(np.float64))
+    assert stats.converged == 1
+    assert true_rel <= 1.0e-6 * 1.01, true_rel
+    assert abs(true_rel - stats.relative_residual) <= 1e-3 * true_rel + 1e-12
+    assert rel_l2(p64, s["pressure"]) < 1e-3
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_stage_projection(name):
+    cfg = cfg_for(name)
+    rng = np.random.default_rng(5)
+    st = make_state(cfg, 5, solids=2, solid_motion=True)
+    st["pressure"] = rng.normal(size=abi.cell_shape(cfg)).astype(np.float32) * 0.01
+    ctx = abi.SmokeContext(cfg)
+    with ctx:
+        load_gpu(ctx, st)
+        ctx.run_stages(abi.STAGE_ADVECT_VELOCITY, abi.STAGE_ADVECT_VELOCITY)
+        g0 = ctx.export_state()
+        ctx.run_stages(abi.STAGE_PROJECT, abi.STAGE_PROJECT)
+        g = ctx.export_state()
+    s = orc.OracleState(cfg)
+    load_oracle(s, st)
+    s["us"], s["vs"], s["ws"] = g0["velocity_star_x"], g0["velocity_star_y"], g0["velocity_star_z"]
+    s["pressure"] = st["pressure"]
+    s.run_stages(3, 3, orc.options())
+    for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w")):
+        assert rel_l2(g[a], s[b]) < 1e-5, (a, rel_l2(g[a], s[b]))
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+@pytest.mark.parametrize("cubic", [False, True])
+def test_stage_scalars(name, cubic):
+    cfg = cfg_for(name, cubic=cubic)
+    ctx, s, _ = _pair(cfg, seed=6, solids=3)
+    with ctx:
+        ctx.run_stages(abi.STAGE_SCALARS, abi.STAGE_SCALARS)
+        g = ctx.export_state()
+    s.run_stages(4, 4, orc.options())
+    assert rel_l2(g["temperature"], s["temperature"]) < 2e-4
+    assert rel_l2(g["density"], s["density"]) < 2e-4
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_full_step(name):
+    errs = one_step_parity(cfg_for(name), seed=7, solids=2)
+    assert errs["solver_relative_residual"] <= 1e-6
+    for k in ("u", "v", "w", "density", "temperature"):
+        assert errs[k] < 1e-3, errs
+
+
+def test_fixed_iteration_cg_matches_reference_semantics():
+    # reference mode: K plain CG iterations from p = 0 (B:1800-1823)
+    cfg = cfg_for("closed")
+    cfg.pressure_iterations = 30
+    ctx, s, st = _pair(cfg, seed=8, solids=1)
+    with ctx:
+        ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_PRESSURE)
+        p = ctx.view("FLOW_PRESSURE")
+    s.run_stages(0, 2, orc.options(mode=0))
+    assert rel_l2(p, s["pressure"]) < 1e-4
+
+
+def test_fixed_mode_graph_step_equals_staged_step():
+    cfg = cfg_for("benchmark")
+    st = make_state(cfg, 9, solids=1)
+    outs = []
+    for staged in (False, True):
+        with abi.SmokeContext(cfg) as ctx:
+            load_gpu(ctx, st)
+            if staged:
+                ctx.run_stages(0, 4)
+            else:
+                ctx.step()
+            outs.append(ctx.export_state())
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
+def test_views_and_updates():
+    cfg = cfg_for("closed", n=(12, 10, 8))
+    st = make_state(cfg, 10, solids=1)
+    with abi.SmokeContext(cfg) as ctx:
+        load_gpu(ctx, st)
+        g = ctx.export_state()
+        np.testing.assert_array_equal(ctx.view("DENSITY"), st["density"])
+        np.testing.assert_array_equal(ctx.view("OCCUPANCY"), st["occupancy"].astype(np.float32))
+        np.testing.assert_array_equal(ctx.view("SOLID_TEMPERATURE"), st["solid_temperature"])
+        cx, cy, cz = ctx.view("FLOW_VELOCITY")
+        np.testing.assert_allclose(cx, 0.5 * (g["velocity_x"][:, :, 1:] + g["velocity_x"][:, :, :-1]), rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(cy, 0.5 * (g["velocity_y"][:, 1:, :] + g["velocity_y"][:, :-1, :]), rtol=1e-6, atol=1e-7)
+        div = ctx.view("FLOW_DIVERGENCE")
+        h = np.float32(cfg.cell_size)
+        expect = (np.diff(g["velocity_x"], axis=2) + np.diff(g["velocity_y"], axis=1) + np.diff(g["velocity_z"], axis=0)) / h
+        expect[st["occupancy"] == 1] = 0
+        np.testing.assert_allclose(div, expect, rtol=1e-4, atol=1e-4)
+        mag = ctx.view("FLOW_VELOCITY_MAGNITUDE")
+        np.testing.assert_allclose(mag, np.sqrt(cx ** 2 + cy ** 2 + cz ** 2), rtol=1e-6, atol=1e-7)
+        s = orc.OracleState(cfg)
+        load_oracle(s, st)
+        s.diagnostics()
+        ox, oy, oz = ctx.view("FLOW_VORTICITY")
+        assert rel_l2(ox, s["vorticity_x"]) < 1e-5 and rel_l2(oz, s["vorticity_z"]) < 1e-5
+        assert rel_l2(ctx.view("FLOW_VORTICITY_MAGNITUDE"), s["vorticity_magnitude"]) < 1e-5
+        # NULL updates: zero, and ambient for the solid temperature (B:2085-2159)
+        ctx.update("density", None)
+        ctx.update("solid_temperature", None)
+        assert np.all(ctx.view("DENSITY") == 0)
+        assert np.all(ctx.view("SOLID_TEMPERATURE") == np.float32(cfg.ambient_temperature))
+        ctx.update("force", None, st["density"], None)
+        fx, fy, fz = ctx.view("FORCE")
+        assert np.all(fx == 0) and np.array_equal(fy, st["density"])
+
+
+def test_adaptive_time_step():
+    cfg = cfg_for("closed")
+    st = make_state(cfg, 11)
+    with abi.SmokeContext(cfg) as ctx:
+        load_gpu(ctx, st)
+        tolerance_options(ctx, adaptive=True, cfl=0.5, dt_max=0.01)
+        ctx.step()
+        stats = ctx.stats()
+    vmax = max(np.abs(st[k]).max() for k in ("u", "v", "w"))
+    assert stats.max_face_speed == pytest.approx(vmax, rel=0)
+    assert stats.dt == pytest.approx(min(0.01, 0.5 * np.float32(cfg.cell_size) / vmax), rel=1e-6)
+
+
+def test_checkpoint_roundtrip_and_determinism(tmp_path):
+    cfg = cfg_for("mixed")
+    st = make_state(cfg, 12, solids=2)
+    path = str(tmp_path / "state.ck").encode()
+    with abi.SmokeContext(cfg) as a:
+        load_gpu(a, st)
+        tolerance_options(a)
+        a.step(2)
+        assert a.lib.smoke_simulation_ext_save_checkpoint(a.handle, path) == 0
+        a.step(2)
+        ref = a.export_state()
+    with abi.SmokeContext(cfg) as b:
+        tolerance_options(b)
+        assert b.lib.smoke_simulation_ext_load_checkpoint(b.handle, path) == 0
+        b.step(2)
+        got = b.export_state()
+    for k in ref:
+        np.testing.assert_array_equal(ref[k], got[k], err_msg=k)
+
+
+def test_plume_sources_match_oracle_bits():
+    cfg = abi.make_config(20, 24, 20)
+    src = abi.PlumeSource(10.0, 3.0, 10.0, 4.0, 1.0, 1.0, 0.05, 7)
+    with abi.SmokeContext(cfg) as ctx:
+        ctx.plume_sources(src, 5)
+        rs = ctx.view("DENSITY_SOURCE")
+        ts = ctx.view("TEMPERATURE_SOURCE")
+    ors, ots = orc.plume_sources(cfg, src, 5)
+    np.testing.assert_array_equal(rs, ors.astype(np.float32))
+    np.testing.assert_array_equal(ts, ots.astype(np.float32))

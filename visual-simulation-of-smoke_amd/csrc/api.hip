@@ -1,0 +1,658 @@
+// C ABI of the smoke step (include/visual-simulation-of-smoke-3d.h) and its
+// extensions (include/visual-simulation-of-smoke-3d-ext.h).
+//
+// Mirrors the behaviour of backend-cuda.cu:1829-2284: same validation and
+// error codes, stream ownership, update/view semantics.  The step is one
+// captured hipGraph in fixed-iteration (reference) mode; the tolerance mode
+// runs its stages eagerly and polls the solver between chunks of iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "context.hpp"
+#include "visual-simulation-of-smoke-3d-ext.h"
+
+struct SmokeSimulationContext_t : smoke::Context {};
+
+namespace smoke {
+namespace {
+
+bool valid_flow_face(const SmokeSimulationFlowBoundaryFaceDesc& f) { return f.type <= 3u; }
+bool valid_scalar_face(const SmokeSimulationScalarBoundaryFaceDesc& f) { return f.type <= 2u; }
+
+// B:1835-1863
+bool valid_config(const SmokeSimulationConfig& c) {
+    if (c.nx <= 0 || c.ny <= 0 || c.nz <= 0 || !(c.cell_size > 0.0f) || !(c.dt > 0.0f) || c.pressure_iterations <= 0) return false;
+    if (uint32_t(c.scalar_advection_mode) > 1u) return false;
+    const SmokeSimulationFlowBoundaryFaceDesc* ff[6] = {&c.flow_boundary.x_minus, &c.flow_boundary.x_plus, &c.flow_boundary.y_minus,
+                                                        &c.flow_boundary.y_plus, &c.flow_boundary.z_minus, &c.flow_boundary.z_plus};
+    for (int a = 0; a < 3; ++a) {
+        if (!valid_flow_face(*ff[2 * a]) || !valid_flow_face(*ff[2 * a + 1])) return false;
+        if ((ff[2 * a]->type == kPeriodic) != (ff[2 * a + 1]->type == kPeriodic)) return false;
+    }
+    for (const SmokeSimulationScalarBoundaryConfig* s : {&c.density_boundary, &c.temperature_boundary}) {
+        const SmokeSimulationScalarBoundaryFaceDesc* sf[6] = {&s->x_minus, &s->x_plus, &s->y_minus, &s->y_plus, &s->z_minus, &s->z_plus};
+        for (int a = 0; a < 3; ++a) {
+            if (!valid_scalar_face(*sf[2 * a]) || !valid_scalar_face(*sf[2 * a + 1])) return false;
+            if ((sf[2 * a]->type == kScalarPeriodic) != (sf[2 * a + 1]->type == kScalarPeriodic)) return false;
+        }
+    }
+    // int32 cell indices in the anchor search (the reference has the same limit, B:1344-1345)
+    const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
+    return cells < (int64_t(1) << 31) - 1;
+}
+
+template <class T>
+void alloc(T*& p, size_t count) {
+    void* q = nullptr;
+    hip_check(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+    p = static_cast<T*>(q);
+}
+
+template <class T>
+void release(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+void destroy_resources(Context& c) {
+    if (c.graph_exec) hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        release(c.face[k]);
+        release(c.face_star[k]);
+        release(c.force_user[k]);
+        release(c.solid_velocity[k]);
+        release(c.cell_force[k]);
+        release(c.vorticity[k]);
+        release(c.center_velocity[k]);
+    }
+    for (float** p : {&c.density, &c.temperature, &c.density_source, &c.temperature_source, &c.density_tmp, &c.temperature_tmp, &c.solid_temperature,
+                      &c.occupancy_float, &c.vorticity_magnitude, &c.velocity_magnitude, &c.divergence, &c.pressure, &c.pressure_rhs, &c.pcg_r,
+                      &c.pcg_p[0], &c.pcg_p[1], &c.pcg_e, &c.dt_device, &c.max_speed})
+        release(*p);
+    release(c.occupancy);
+    release(c.stencil_flags);
+    release(c.anchor);
+    release(c.solid_pairs);
+    release(c.solid_count);
+    release(c.x64[0]);
+    release(c.x64[1]);
+    release(c.scalars);
+    release(c.partials);
+    release(c.max_speed_bits);
+    if (c.host_status) hipHostFree(c.host_status);
+    c.host_status = nullptr;
+    for (hipEvent_t& e : c.status_event)
+        if (e) hipEventDestroy(e), e = nullptr;
+    for (auto& row : c.timing_events)
+        for (hipEvent_t& e : row)
+            if (e) hipEventDestroy(e), e = nullptr;
+}
+
+SmokeSimulationSolverOptions default_options(const SmokeSimulationConfig& cfg) {
+    SmokeSimulationSolverOptions o{};
+    o.pressure_mode = SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS;
+    o.preconditioner = SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
+    o.relative_tolerance = 1.0e-6;
+    o.max_iterations = 20000;
+    o.max_refinements = 2;
+    o.warm_start = 1;
+    o.adaptive_dt = 0;
+    o.cfl = 0.5f;
+    o.dt_max = cfg.dt;
+    o.check_interval = 16;
+    return o;
+}
+
+// Inner solves stop once their residual has fallen by this factor (or below
+// half the outer target); FP32 Krylov vectors still track the true residual
+// well at this depth, and the FP64 refinement pass fixes the rest.
+constexpr double kInnerReduction = 1.0e-4;
+
+void upload_solver_constants(Context& c) {
+    PcgScalars s{};
+    s.tolerance = c.options.relative_tolerance;
+    s.inner_reduction = kInnerReduction;
+    s.max_iterations = c.options.max_iterations;
+    s.fixed_mode = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS;
+    s.jacobi = !s.fixed_mode && c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
+    hip_check(hipStreamSynchronize(c.stream), "sync");
+    hip_check(hipMemcpy(c.scalars, &s, sizeof(s), hipMemcpyHostToDevice), "upload solver constants");
+}
+
+void enqueue_pressure(Context& c) {
+    enqueue_pressure_rhs(c);
+    if (c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS) enqueue_fixed_pcg(c);
+    else run_tolerance_pcg(c);
+}
+
+void enqueue_stages(Context& c, int first, int last) {
+    if (first <= 0 && last >= 0) {
+        enqueue_time_step(c);
+        enqueue_forces(c);
+    }
+    if (first <= 1 && last >= 1) enqueue_advect_velocity(c);
+    if (first <= 2 && last >= 2) enqueue_pressure(c);
+    if (first <= 3 && last >= 3) enqueue_project(c);
+    if (first <= 4 && last >= 4) enqueue_scalars(c);
+    c.diagnostics_dirty = true;
+}
+
+void rebuild_graph(Context& c) {
+    if (c.graph_exec) hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph = nullptr;
+    if (c.options.pressure_mode != SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS) return;
+    hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        enqueue_stages(c, 0, 4);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c.stream, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(c.stream, &c.graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+void select_device(const Context& c) { hip_check(hipSetDevice(c.device), "hipSetDevice"); }
+
+SmokeSimulationResult guarded_call(Context* c, auto&& fn) {
+    if (!c) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    try {
+        select_device(*c);
+        fn(*c);
+        return SMOKE_SIMULATION_RESULT_OK;
+    } catch (const std::bad_alloc&) {
+        return SMOKE_SIMULATION_RESULT_OUT_OF_MEMORY;
+    } catch (...) {
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
+}
+
+void copy_or_fill(Context& c, float* dst, const float* src, float fill, int64_t n) {
+    if (src) hip_check(hipMemcpyAsync(dst, src, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, c.stream), "hipMemcpyAsync D2D");
+    else enqueue_fill(c, dst, fill, n);
+}
+
+void step(Context& c) {
+    if (c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS && c.graph_exec) {
+        if (c.timing) hip_check(hipEventRecord(c.timing_events[0][0], c.stream), "event");
+        hip_check(hipGraphLaunch(c.graph_exec, c.stream), "hipGraphLaunch");
+        if (c.timing) {
+            hip_check(hipEventRecord(c.timing_events[0][1], c.stream), "event");
+            hip_check(hipEventSynchronize(c.timing_events[0][1]), "event sync");
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c.timing_events[0][0], c.timing_events[0][1]);
+            c.timing_totals.step_ms += ms;
+            c.timing_totals.steps += 1;
+        }
+        c.stats.iterations = c.config.pressure_iterations;
+    } else {
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (c.timing) {
+            hip_check(hipEventCreate(&t0), "event");
+            hip_check(hipEventCreate(&t1), "event");
+            hip_check(hipEventRecord(t0, c.stream), "event");
+        }
+        enqueue_stages(c, 0, 4);
+        if (c.timing) {
+            hip_check(hipEventRecord(t1, c.stream), "event");
+            hip_check(hipEventSynchronize(t1), "event sync");
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, t0, t1);
+            c.timing_totals.step_ms += ms;
+            c.timing_totals.steps += 1;
+            hipEventDestroy(t0);
+            hipEventDestroy(t1);
+        }
+    }
+    c.stats.steps += 1;
+    c.diagnostics_dirty = true;
+}
+
+// Checkpoint layout: "SMOKECK1", u32 version, i32 nx, ny, nz, u32 field count,
+// then per field: char[24] name, u64 byte count, raw little-endian data.
+struct CheckpointField {
+    const char* name;
+    void* device;
+    size_t bytes;
+};
+
+std::vector<CheckpointField> checkpoint_fields(Context& c) {
+    const size_t cells = size_t(c.grid.cells);
+    const size_t cb = cells * sizeof(float);
+    return {
+        {"velocity_x", c.face[0], size_t(c.grid.faces[0]) * sizeof(float)},
+        {"velocity_y", c.face[1], size_t(c.grid.faces[1]) * sizeof(float)},
+        {"velocity_z", c.face[2], size_t(c.grid.faces[2]) * sizeof(float)},
+        {"density", c.density, cb},
+        {"temperature", c.temperature, cb},
+        {"density_source", c.density_source, cb},
+        {"temperature_source", c.temperature_source, cb},
+        {"force_x", c.force_user[0], cb},
+        {"force_y", c.force_user[1], cb},
+        {"force_z", c.force_user[2], cb},
+        {"solid_velocity_x", c.solid_velocity[0], cb},
+        {"solid_velocity_y", c.solid_velocity[1], cb},
+        {"solid_velocity_z", c.solid_velocity[2], cb},
+        {"solid_temperature", c.solid_temperature, cb},
+        {"occupancy", c.occupancy, cells},
+        {"pressure", c.pressure, cb},
+        {"pressure_f64", c.x64[c.x64_current], cells * sizeof(double)},
+    };
+}
+
+}  // namespace
+}  // namespace smoke
+
+using smoke::Context;
+using smoke::hip_check;
+
+extern "C" {
+
+SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulationContextCreateDesc* desc, SmokeSimulationContext* out_context) {
+    if (desc == nullptr || out_context == nullptr) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    if (!smoke::valid_config(desc->config)) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    *out_context = nullptr;
+    std::unique_ptr<SmokeSimulationContext_t> ctx{new (std::nothrow) SmokeSimulationContext_t{}};
+    if (!ctx) return SMOKE_SIMULATION_RESULT_OUT_OF_MEMORY;
+    Context& c = *ctx;
+    try {
+        c.config = desc->config;
+        c.grid = smoke::make_grid(c.config);
+        hip_check(hipGetDevice(&c.device), "hipGetDevice");
+        c.stream = static_cast<hipStream_t>(desc->stream);
+        if (!c.stream) {
+            hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            c.owns_stream = true;
+        }
+        const size_t cells = size_t(c.grid.cells);
+        for (int k = 0; k < 3; ++k) {
+            smoke::alloc(c.face[k], size_t(c.grid.faces[k]));
+            smoke::alloc(c.face_star[k], size_t(c.grid.faces[k]));
+            smoke::alloc(c.force_user[k], cells);
+            smoke::alloc(c.solid_velocity[k], cells);
+            smoke::alloc(c.cell_force[k], cells);
+            smoke::alloc(c.vorticity[k], cells);
+            smoke::alloc(c.center_velocity[k], cells);
+        }
+        for (float** p : {&c.density, &c.temperature, &c.density_source, &c.temperature_source, &c.density_tmp, &c.temperature_tmp,
+                          &c.solid_temperature, &c.occupancy_float, &c.vorticity_magnitude, &c.velocity_magnitude, &c.divergence, &c.pressure,
+                          &c.pressure_rhs, &c.pcg_r, &c.pcg_p[0], &c.pcg_p[1], &c.pcg_e})
+            smoke::alloc(*p, cells);
+        smoke::alloc(c.occupancy, cells);
+        smoke::alloc(c.stencil_flags, cells);
+        smoke::alloc(c.anchor, 1);
+        smoke::alloc(c.solid_pairs, 2 * cells);
+        smoke::alloc(c.solid_count, 1);
+        smoke::alloc(c.x64[0], cells);
+        smoke::alloc(c.x64[1], cells);
+        smoke::alloc(c.scalars, 1);
+        smoke::configure_pcg(c);
+        smoke::alloc(c.partials, c.partial_capacity);
+        smoke::alloc(c.dt_device, 1);
+        smoke::alloc(c.max_speed_bits, 1);
+        smoke::alloc(c.max_speed, 1);
+        void* pinned = nullptr;
+        hip_check(hipHostMalloc(&pinned, 2 * sizeof(smoke::PcgStatus), hipHostMallocDefault), "hipHostMalloc");
+        c.host_status = static_cast<smoke::PcgStatus*>(pinned);
+        for (hipEvent_t& e : c.status_event) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        for (auto& row : c.timing_events)
+            for (hipEvent_t& e : row) hip_check(hipEventCreate(&e), "event");
+
+        // B:1952-1987 initial values
+        for (int k = 0; k < 3; ++k) {
+            smoke::enqueue_fill(c, c.face[k], 0.0f, c.grid.faces[k]);
+            smoke::enqueue_fill(c, c.face_star[k], 0.0f, c.grid.faces[k]);
+            for (float* p : {c.force_user[k], c.solid_velocity[k], c.cell_force[k], c.vorticity[k], c.center_velocity[k]})
+                smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
+        }
+        for (float* p : {c.density_source, c.temperature_source, c.occupancy_float, c.vorticity_magnitude, c.velocity_magnitude, c.divergence,
+                         c.pressure, c.pressure_rhs, c.pcg_r, c.pcg_p[0], c.pcg_p[1], c.pcg_e})
+            smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
+        smoke::enqueue_fill(c, c.density, desc->initial_density, c.grid.cells);
+        smoke::enqueue_fill(c, c.density_tmp, desc->initial_density, c.grid.cells);
+        smoke::enqueue_fill(c, c.temperature, desc->initial_temperature, c.grid.cells);
+        smoke::enqueue_fill(c, c.temperature_tmp, desc->initial_temperature, c.grid.cells);
+        smoke::enqueue_fill(c, c.solid_temperature, c.config.ambient_temperature, c.grid.cells);
+        smoke::enqueue_fill_u8(c, c.occupancy, 0, c.grid.cells);
+        hip_check(hipMemsetAsync(c.x64[0], 0, cells * sizeof(double), c.stream), "zero x64");
+        hip_check(hipMemsetAsync(c.x64[1], 0, cells * sizeof(double), c.stream), "zero x64");
+        hip_check(hipMemsetAsync(c.max_speed_bits, 0, sizeof(unsigned), c.stream), "zero max");
+        smoke::enqueue_occupancy_derived(c);
+        c.options = smoke::default_options(c.config);
+        smoke::upload_solver_constants(c);
+        smoke::rebuild_graph(c);
+        hip_check(hipStreamSynchronize(c.stream), "create sync");
+        *out_context = ctx.release();
+        return SMOKE_SIMULATION_RESULT_OK;
+    } catch (const std::bad_alloc&) {
+        smoke::destroy_resources(c);
+        if (c.owns_stream && c.stream) hipStreamDestroy(c.stream);
+        return SMOKE_SIMULATION_RESULT_OUT_OF_MEMORY;
+    } catch (...) {
+        smoke::destroy_resources(c);
+        if (c.owns_stream && c.stream) hipStreamDestroy(c.stream);
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
+}
+
+SmokeSimulationResult smoke_simulation_destroy_context_cuda(SmokeSimulationContext context) {
+    if (context == nullptr) return SMOKE_SIMULATION_RESULT_OK;
+    Context& c = *context;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    smoke::destroy_resources(c);
+    if (c.owns_stream && c.stream) hipStreamDestroy(c.stream);
+    delete context;
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_update_density_cuda(SmokeSimulationContext context, const float* values) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::copy_or_fill(c, c.density, values, 0.0f, c.grid.cells);
+        c.diagnostics_dirty = true;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_update_density_source_cuda(SmokeSimulationContext context, const float* values) {
+    return smoke::guarded_call(context, [&](Context& c) { smoke::copy_or_fill(c, c.density_source, values, 0.0f, c.grid.cells); });
+}
+
+SmokeSimulationResult smoke_simulation_update_temperature_cuda(SmokeSimulationContext context, const float* values) {
+    return smoke::guarded_call(context, [&](Context& c) { smoke::copy_or_fill(c, c.temperature, values, 0.0f, c.grid.cells); });
+}
+
+SmokeSimulationResult smoke_simulation_update_temperature_source_cuda(SmokeSimulationContext context, const float* values) {
+    return smoke::guarded_call(context, [&](Context& c) { smoke::copy_or_fill(c, c.temperature_source, values, 0.0f, c.grid.cells); });
+}
+
+SmokeSimulationResult smoke_simulation_update_force_cuda(SmokeSimulationContext context, const float* vx, const float* vy, const float* vz) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::copy_or_fill(c, c.force_user[0], vx, 0.0f, c.grid.cells);
+        smoke::copy_or_fill(c, c.force_user[1], vy, 0.0f, c.grid.cells);
+        smoke::copy_or_fill(c, c.force_user[2], vz, 0.0f, c.grid.cells);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_update_occupancy_cuda(SmokeSimulationContext context, const uint8_t* values) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        if (values) hip_check(hipMemcpyAsync(c.occupancy, values, size_t(c.grid.cells), hipMemcpyDeviceToDevice, c.stream), "hipMemcpyAsync occupancy");
+        else smoke::enqueue_fill_u8(c, c.occupancy, 0, c.grid.cells);
+        smoke::enqueue_occupancy_derived(c);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_update_solid_velocity_cuda(SmokeSimulationContext context, const float* vx, const float* vy, const float* vz) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::copy_or_fill(c, c.solid_velocity[0], vx, 0.0f, c.grid.cells);
+        smoke::copy_or_fill(c, c.solid_velocity[1], vy, 0.0f, c.grid.cells);
+        smoke::copy_or_fill(c, c.solid_velocity[2], vz, 0.0f, c.grid.cells);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_update_solid_temperature_cuda(SmokeSimulationContext context, const float* values) {
+    return smoke::guarded_call(context,
+                               [&](Context& c) { smoke::copy_or_fill(c, c.solid_temperature, values, c.config.ambient_temperature, c.grid.cells); });
+}
+
+SmokeSimulationResult smoke_simulation_step_cuda(SmokeSimulationContext context) {
+    return smoke::guarded_call(context, [&](Context& c) { smoke::step(c); });
+}
+
+SmokeSimulationResult smoke_simulation_get_view_cuda(SmokeSimulationContext context, const SmokeSimulationViewRequest* request, SmokeSimulationView* out) {
+    if (context == nullptr || request == nullptr || out == nullptr) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    Context& c = *context;
+    *out = SmokeSimulationView{};
+    out->layout = SMOKE_SIMULATION_VIEW_LAYOUT_F32_3D;
+    out->nx = c.config.nx;
+    out->ny = c.config.ny;
+    out->nz = c.config.nz;
+    out->row_stride_bytes = uint64_t(c.config.nx) * sizeof(float);
+    out->slice_stride_bytes = uint64_t(c.config.nx) * uint64_t(c.config.ny) * sizeof(float);
+    auto soa = [&](float* const f[3]) {
+        out->layout = SMOKE_SIMULATION_VIEW_LAYOUT_F32_3D_SOA3;
+        out->data0 = f[0];
+        out->data1 = f[1];
+        out->data2 = f[2];
+    };
+    bool diagnostic = false;
+    switch (request->kind) {
+        case SMOKE_SIMULATION_VIEW_DENSITY: out->data0 = c.density; break;
+        case SMOKE_SIMULATION_VIEW_DENSITY_SOURCE: out->data0 = c.density_source; break;
+        case SMOKE_SIMULATION_VIEW_TEMPERATURE: out->data0 = c.temperature; break;
+        case SMOKE_SIMULATION_VIEW_TEMPERATURE_SOURCE: out->data0 = c.temperature_source; break;
+        case SMOKE_SIMULATION_VIEW_FORCE: soa(c.force_user); break;
+        case SMOKE_SIMULATION_VIEW_SOLID_VELOCITY: soa(c.solid_velocity); break;
+        case SMOKE_SIMULATION_VIEW_SOLID_TEMPERATURE: out->data0 = c.solid_temperature; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_VELOCITY: soa(c.center_velocity), diagnostic = true; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_VELOCITY_MAGNITUDE: out->data0 = c.velocity_magnitude, diagnostic = true; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_PRESSURE: out->data0 = c.pressure; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_PRESSURE_RHS: out->data0 = c.pressure_rhs; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_DIVERGENCE: out->data0 = c.divergence, diagnostic = true; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_VORTICITY: soa(c.vorticity), diagnostic = true; break;
+        case SMOKE_SIMULATION_VIEW_FLOW_VORTICITY_MAGNITUDE: out->data0 = c.vorticity_magnitude, diagnostic = true; break;
+        case SMOKE_SIMULATION_VIEW_OCCUPANCY: out->data0 = c.occupancy_float; break;
+        default: return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
+    return smoke::guarded_call(context, [&](Context& cc) {
+        // diagnostics are derived from the current state on demand (stream-ordered)
+        if (diagnostic && cc.diagnostics_dirty) smoke::enqueue_diagnostics(cc);
+        // B:2189-2205: hand the view to another stream through an event
+        hipStream_t consumer = static_cast<hipStream_t>(request->consumer_stream);
+        if (consumer && consumer != cc.stream) {
+            hipEvent_t ready = nullptr;
+            hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+            hipError_t e1 = hipEventRecord(ready, cc.stream);
+            hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(consumer, ready, 0) : e1;
+            hipEventDestroy(ready);
+            hip_check(e2, "view event");
+        }
+    });
+}
+
+// ---------------------------------------------------------------- extensions
+
+SmokeSimulationResult smoke_simulation_ext_default_solver_options(SmokeSimulationSolverOptions* out) {
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    SmokeSimulationConfig cfg{};
+    cfg.dt = 0.01f;
+    *out = smoke::default_options(cfg);
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_ext_set_solver_options(SmokeSimulationContext context, const SmokeSimulationSolverOptions* o) {
+    if (!o) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    if (o->pressure_mode > 1u || o->preconditioner > 1u || !(o->relative_tolerance > 0.0) || o->max_iterations <= 0 || o->max_refinements < 0 ||
+        (o->adaptive_dt && (!(o->cfl > 0.0f) || !(o->dt_max > 0.0f))))
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        c.options = *o;
+        if (c.options.check_interval <= 0) c.options.check_interval = 16;
+        smoke::upload_solver_constants(c);
+        smoke::rebuild_graph(c);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_solver_options(SmokeSimulationContext context, SmokeSimulationSolverOptions* out) {
+    if (!context || !out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    *out = context->options;
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_solver_stats(SmokeSimulationContext context, SmokeSimulationSolverStats* out) {
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        if (c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS) {
+            smoke::enqueue_true_residual(c);
+            smoke::PcgScalars s{};
+            hip_check(hipMemcpyAsync(&s, c.scalars, sizeof(s), hipMemcpyDeviceToHost, c.stream), "read scalars");
+            hip_check(hipStreamSynchronize(c.stream), "sync");
+            c.stats.rhs_norm = std::sqrt(s.bb);
+            c.stats.relative_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : 0.0;
+            c.stats.converged = 0;
+            c.stats.refinements = 0;
+        }
+        float dt = 0.f, speed = 0.f;
+        int anchor = 0;
+        hip_check(hipMemcpyAsync(&dt, c.dt_device, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read dt");
+        hip_check(hipMemcpyAsync(&speed, c.max_speed, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read speed");
+        hip_check(hipMemcpyAsync(&anchor, c.anchor, sizeof(int), hipMemcpyDeviceToHost, c.stream), "read anchor");
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        c.stats.dt = dt;
+        c.stats.max_face_speed = speed;
+        c.stats.anchor = anchor;
+        *out = c.stats;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_import_state(SmokeSimulationContext context, const SmokeSimulationHostState* s) {
+    if (!s) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        const size_t cb = size_t(c.grid.cells) * sizeof(float);
+        float* faces[3] = {s->velocity_x, s->velocity_y, s->velocity_z};
+        for (int k = 0; k < 3; ++k)
+            if (faces[k]) hip_check(hipMemcpy(c.face[k], faces[k], size_t(c.grid.faces[k]) * sizeof(float), hipMemcpyHostToDevice), "import face");
+        if (s->density) hip_check(hipMemcpy(c.density, s->density, cb, hipMemcpyHostToDevice), "import density");
+        if (s->temperature) hip_check(hipMemcpy(c.temperature, s->temperature, cb, hipMemcpyHostToDevice), "import temperature");
+        if (s->pressure) {
+            std::vector<double> p(size_t(c.grid.cells));
+            for (size_t i = 0; i < p.size(); ++i) p[i] = s->pressure[i];
+            hip_check(hipMemcpy(c.pressure, s->pressure, cb, hipMemcpyHostToDevice), "import pressure");
+            hip_check(hipMemcpy(c.x64[c.x64_current], p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice), "import pressure f64");
+            smoke::enqueue_occupancy_derived(c);   // re-zeroes identity rows of the FP64 pressure
+            hip_check(hipStreamSynchronize(c.stream), "sync");
+        }
+        c.diagnostics_dirty = true;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_export_state(SmokeSimulationContext context, const SmokeSimulationHostState* s) {
+    if (!s) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        const size_t cb = size_t(c.grid.cells) * sizeof(float);
+        float* faces[3] = {s->velocity_x, s->velocity_y, s->velocity_z};
+        float* stars[3] = {s->velocity_star_x, s->velocity_star_y, s->velocity_star_z};
+        for (int k = 0; k < 3; ++k) {
+            if (faces[k]) hip_check(hipMemcpy(faces[k], c.face[k], size_t(c.grid.faces[k]) * sizeof(float), hipMemcpyDeviceToHost), "export face");
+            if (stars[k]) hip_check(hipMemcpy(stars[k], c.face_star[k], size_t(c.grid.faces[k]) * sizeof(float), hipMemcpyDeviceToHost), "export star");
+        }
+        if (s->density) hip_check(hipMemcpy(s->density, c.density, cb, hipMemcpyDeviceToHost), "export density");
+        if (s->temperature) hip_check(hipMemcpy(s->temperature, c.temperature, cb, hipMemcpyDeviceToHost), "export temperature");
+        if (s->pressure) hip_check(hipMemcpy(s->pressure, c.pressure, cb, hipMemcpyDeviceToHost), "export pressure");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_export_pressure_f64(SmokeSimulationContext context, double* values) {
+    if (!values) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        hip_check(hipMemcpy(values, c.x64[c.x64_current], size_t(c.grid.cells) * sizeof(double), hipMemcpyDeviceToHost), "export pressure f64");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_run_stages(SmokeSimulationContext context, int32_t first, int32_t last) {
+    if (first < 0 || last > 4 || first > last) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) { smoke::enqueue_stages(c, first, last); });
+}
+
+SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContext context, const char* path) {
+    if (!path) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw smoke::HipError("cannot open checkpoint for writing");
+        auto fields = smoke::checkpoint_fields(c);
+        const uint32_t version = 1, count = uint32_t(fields.size());
+        const int32_t dims[3] = {c.config.nx, c.config.ny, c.config.nz};
+        bool ok = std::fwrite("SMOKECK1", 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 && std::fwrite(dims, 4, 3, f) == 3 &&
+                  std::fwrite(&count, 4, 1, f) == 1;
+        std::vector<char> host;
+        for (auto& fd : fields) {
+            if (!ok) break;
+            char name[24] = {};
+            std::strncpy(name, fd.name, sizeof(name) - 1);
+            const uint64_t bytes = fd.bytes;
+            host.resize(fd.bytes);
+            if (hipMemcpy(host.data(), fd.device, fd.bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+            ok = ok && std::fwrite(name, 1, 24, f) == 24 && std::fwrite(&bytes, 8, 1, f) == 1 && std::fwrite(host.data(), 1, fd.bytes, f) == fd.bytes;
+        }
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) throw smoke::HipError("checkpoint write failed");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContext context, const char* path) {
+    if (!path) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) throw smoke::HipError("cannot open checkpoint");
+        char magic[8];
+        uint32_t version = 0, count = 0;
+        int32_t dims[3];
+        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SMOKECK1", 8) == 0 && std::fread(&version, 4, 1, f) == 1 && version == 1 &&
+                  std::fread(dims, 4, 3, f) == 3 && dims[0] == c.config.nx && dims[1] == c.config.ny && dims[2] == c.config.nz &&
+                  std::fread(&count, 4, 1, f) == 1;
+        auto fields = smoke::checkpoint_fields(c);
+        ok = ok && count == fields.size();
+        std::vector<char> host;
+        for (auto& fd : fields) {
+            if (!ok) break;
+            char name[24];
+            uint64_t bytes = 0;
+            ok = std::fread(name, 1, 24, f) == 24 && std::fread(&bytes, 8, 1, f) == 1 && bytes == fd.bytes && std::strncmp(name, fd.name, 24) == 0;
+            if (!ok) break;
+            host.resize(fd.bytes);
+            ok = std::fread(host.data(), 1, fd.bytes, f) == fd.bytes && hipMemcpy(fd.device, host.data(), fd.bytes, hipMemcpyHostToDevice) == hipSuccess;
+        }
+        std::fclose(f);
+        if (!ok) throw smoke::HipError("checkpoint read failed");
+        smoke::enqueue_occupancy_derived(c);
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_set_kernel_timing(SmokeSimulationContext context, int32_t enable) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        c.timing = enable != 0;
+        c.timing_totals = SmokeSimulationKernelTiming{};
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_kernel_timing(SmokeSimulationContext context, SmokeSimulationKernelTiming* out) {
+    if (!context || !out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    *out = context->timing_totals;
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_ext_plume_sources(SmokeSimulationContext context, const SmokeSimulationPlumeSource* src, uint32_t step) {
+    if (!src) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) { smoke::enqueue_plume_sources(c, *src, step); });
+}
+
+SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gbps, double* out_ms) {
+    if (!out_gbps || bytes < 16 || repetitions <= 0) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    try {
+        *out_gbps = smoke::measure_copy(bytes, repetitions, out_ms);
+        return SMOKE_SIMULATION_RESULT_OK;
+    } catch (...) {
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
+}
+
+}  // extern "C"

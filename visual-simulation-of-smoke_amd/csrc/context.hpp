@@ -1,0 +1,145 @@
+// Host-side context of one simulation: device buffers (x-fastest SoA, one
+// allocation per field), PCG workspace, options and the captured step graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "device_common.hpp"
+#include "visual-simulation-of-smoke-3d-ext.h"
+
+namespace smoke {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& what) : std::runtime_error(what) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Scalars of the pressure solve, resident in device memory so a step never
+// needs a host round trip in fixed-iteration mode.
+struct PcgScalars {
+    double rz;          // r . z of the current inner residual
+    double alpha;
+    double beta;
+    double rr;          // inner r . r after the last iteration
+    double stop2;       // inner stop threshold, squared
+    double bb;          // ||b||^2
+    double rr_true;     // ||b - A x||^2 from the last FP64 residual pass
+    double tolerance;   // relative tolerance of the whole solve
+    double inner_reduction;
+    int iteration;      // inner iterations performed in this solve
+    int done;           // inner solve finished (tolerance reached or cap)
+    int max_iterations;
+    int first;          // next pass A starts a fresh Krylov space (beta = 0)
+    int fixed_mode;     // reference semantics: no early exit
+    int jacobi;
+    unsigned ticket[4]; // last-block counters of the reducing kernels
+};
+
+struct PcgStatus {      // what the host polls between chunks
+    int iteration;
+    int done;
+};
+
+struct Context {
+    SmokeSimulationConfig config{};
+    Grid grid{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+
+    // simulation state
+    float* face[3] = {};          // MAC velocity (state)
+    float* face_star[3] = {};     // u* (advected, then projected in place of state)
+    float* density = nullptr;
+    float* temperature = nullptr;
+    float* density_source = nullptr;
+    float* temperature_source = nullptr;
+    float* density_tmp = nullptr;
+    float* temperature_tmp = nullptr;
+    float* force_user[3] = {};
+    float* solid_velocity[3] = {};
+    float* solid_temperature = nullptr;
+    uint8_t* occupancy = nullptr;
+    float* occupancy_float = nullptr;
+
+    // scratch + diagnostics (views)
+    float* cell_force[3] = {};
+    float* vorticity[3] = {};
+    float* vorticity_magnitude = nullptr;
+    float* center_velocity[3] = {};
+    float* velocity_magnitude = nullptr;
+    float* divergence = nullptr;
+    float* pressure = nullptr;       // FP32 pressure used by the projection
+    float* pressure_rhs = nullptr;
+
+    // occupancy-derived data
+    uint8_t* stencil_flags = nullptr;   // bits 0-5 fluid neighbour x-,x+,y-,y+,z-,z+; bit 6 identity row
+    int* anchor = nullptr;
+    int64_t* solid_pairs = nullptr;     // (solid cell, nearest fluid cell or -1) pairs
+    unsigned* solid_count = nullptr;
+
+    // pressure solve workspace
+    float* pcg_r = nullptr;
+    float* pcg_p[2] = {};
+    float* pcg_e = nullptr;
+    double* x64[2] = {};
+    int x64_current = 0;
+    int p_current = 0;
+    PcgScalars* scalars = nullptr;
+    double* partials = nullptr;
+    size_t partial_capacity = 0;
+    PcgStatus* host_status = nullptr;   // pinned, two slots
+    hipEvent_t status_event[2] = {};
+
+    // time step
+    float* dt_device = nullptr;          // dt used by the current step
+    unsigned* max_speed_bits = nullptr;  // running max |u_face| (float bits)
+    float* max_speed = nullptr;
+
+    SmokeSimulationSolverOptions options{};
+    SmokeSimulationSolverStats stats{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool diagnostics_dirty = true;
+
+    // PCG launch shape
+    int tile_x = 64, tile_y = 8, zchunk = 16;
+    int pcg_blocks = 0;
+
+    bool timing = false;
+    SmokeSimulationKernelTiming timing_totals{};
+    hipEvent_t timing_events[2][3] = {};
+};
+
+// Launch helpers implemented in the .hip translation units.
+void enqueue_fill(Context& c, float* p, float value, int64_t n);
+void enqueue_fill_u8(Context& c, uint8_t* p, uint8_t value, int64_t n);
+void enqueue_occupancy_derived(Context& c);
+void enqueue_time_step(Context& c);
+void enqueue_forces(Context& c);
+void enqueue_advect_velocity(Context& c);
+void enqueue_pressure_rhs(Context& c);
+void enqueue_project(Context& c);
+void enqueue_scalars(Context& c);
+void enqueue_diagnostics(Context& c);
+void enqueue_plume_sources(Context& c, const SmokeSimulationPlumeSource& src, uint32_t step);
+
+// Pressure solve (pcg.hip).
+void enqueue_pcg_init(Context& c, bool use_x64);          // r = b - A x, workspace reset, reductions
+void enqueue_pcg_iteration(Context& c);                    // pass A + pass B
+void enqueue_pcg_refine(Context& c);                       // x64 += e, r = b - A x64
+void enqueue_pressure_from_x64(Context& c);
+void enqueue_fixed_pcg(Context& c);                        // reference: K plain CG iterations from 0
+void run_tolerance_pcg(Context& c);                        // host-polled tolerance solve
+void enqueue_true_residual(Context& c);                    // rr_true, bb for the FP32 pressure
+void configure_pcg(Context& c);
+double measure_copy(uint64_t bytes, int reps, double* ms_per_copy);
+
+}  // namespace smoke

@@ -1,0 +1,412 @@
+// Device-side building blocks of the smoke step: grid descriptor, MAC
+// indexing, boundary (ghost) rules, linear / monotone-cubic samplers and the
+// RK2 backtrace.  Semantics follow backend-cuda.cu ("B") line for line where
+// cited; the implementation is organised for gfx950 (flat descriptor arrays
+// indexed by axis, an all-in-range fast path for the 64-tap cubic gathers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "visual-simulation-of-smoke-3d.h"
+
+namespace smoke {
+
+// Boundary kinds (values of the ABI enums).
+enum : uint32_t { kNoSlip = 0, kFreeSlip = 1, kOutflow = 2, kPeriodic = 3 };
+enum : uint32_t { kFixedValue = 0, kZeroFlux = 1, kScalarPeriodic = 2 };
+
+struct ScalarBC {
+    uint32_t type[6];   // x-, x+, y-, y+, z-, z+
+    float value[6];
+    int periodic[3];
+};
+
+// Flat, axis-indexed copy of SmokeSimulationConfig passed by value to kernels.
+struct Grid {
+    int n[3];
+    float h;
+    uint32_t flow_type[6];
+    float flow_velocity[6][3];
+    float flow_pressure[6];
+    int flow_periodic[3];
+    ScalarBC density;
+    ScalarBC temperature;
+    int cubic;
+    int64_t cells;
+    int64_t faces[3];
+};
+
+__host__ inline Grid make_grid(const SmokeSimulationConfig& c) {
+    Grid g{};
+    g.n[0] = c.nx;
+    g.n[1] = c.ny;
+    g.n[2] = c.nz;
+    g.h = c.cell_size;
+    const SmokeSimulationFlowBoundaryFaceDesc* ff[6] = {&c.flow_boundary.x_minus, &c.flow_boundary.x_plus, &c.flow_boundary.y_minus,
+                                                        &c.flow_boundary.y_plus, &c.flow_boundary.z_minus, &c.flow_boundary.z_plus};
+    for (int f = 0; f < 6; ++f) {
+        g.flow_type[f] = ff[f]->type;
+        g.flow_velocity[f][0] = ff[f]->velocity_x;
+        g.flow_velocity[f][1] = ff[f]->velocity_y;
+        g.flow_velocity[f][2] = ff[f]->velocity_z;
+        g.flow_pressure[f] = ff[f]->pressure;
+    }
+    for (int a = 0; a < 3; ++a) g.flow_periodic[a] = g.flow_type[2 * a] == kPeriodic && g.flow_type[2 * a + 1] == kPeriodic;
+    auto scal = [](const SmokeSimulationScalarBoundaryConfig& s) {
+        ScalarBC b{};
+        const SmokeSimulationScalarBoundaryFaceDesc* sf[6] = {&s.x_minus, &s.x_plus, &s.y_minus, &s.y_plus, &s.z_minus, &s.z_plus};
+        for (int f = 0; f < 6; ++f) {
+            b.type[f] = sf[f]->type;
+            b.value[f] = sf[f]->value;
+        }
+        for (int a = 0; a < 3; ++a) b.periodic[a] = b.type[2 * a] == kScalarPeriodic && b.type[2 * a + 1] == kScalarPeriodic;
+        return b;
+    };
+    g.density = scal(c.density_boundary);
+    g.temperature = scal(c.temperature_boundary);
+    g.cubic = c.scalar_advection_mode == SMOKE_SIMULATION_SCALAR_ADVECTION_MONOTONIC_CUBIC;
+    g.cells = int64_t(c.nx) * c.ny * c.nz;
+    for (int k = 0; k < 3; ++k) g.faces[k] = int64_t(c.nx + (k == 0)) * (c.ny + (k == 1)) * (c.nz + (k == 2));
+    return g;
+}
+
+// B:121-135 -- x-fastest SoA cells and staggered faces.
+__device__ __forceinline__ int64_t cidx(const Grid& g, int x, int y, int z) { return (int64_t(z) * g.n[1] + y) * g.n[0] + x; }
+__device__ __forceinline__ int64_t fidx(const Grid& g, int c, int i, int j, int k) {
+    const int d0 = g.n[0] + (c == 0), d1 = g.n[1] + (c == 1);
+    return (int64_t(k) * d1 + j) * d0 + i;
+}
+
+// B:137-142
+__device__ __forceinline__ int wrapi(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// B:148-153: wrap flow-periodic axes, then the bounds test.
+__device__ __forceinline__ bool resolve_flow(const Grid& g, int c[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (g.flow_periodic[a]) c[a] = wrapi(c[a], g.n[a]);
+    return c[0] >= 0 && c[0] < g.n[0] && c[1] >= 0 && c[1] < g.n[1] && c[2] >= 0 && c[2] < g.n[2];
+}
+
+__device__ __forceinline__ bool resolve_scalar(const Grid& g, const ScalarBC& bc, int c[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (bc.periodic[a]) c[a] = wrapi(c[a], g.n[a]);
+    return c[0] >= 0 && c[0] < g.n[0] && c[1] >= 0 && c[1] < g.n[1] && c[2] >= 0 && c[2] < g.n[2];
+}
+
+// B:162-166 -- out of domain counts as solid.
+__device__ __forceinline__ bool occupied(const Grid& g, const uint8_t* occ, int x, int y, int z) {
+    int c[3] = {x, y, z};
+    if (!resolve_flow(g, c)) return true;
+    return occ[cidx(g, c[0], c[1], c[2])] != 0;
+}
+
+// B:168-200 -- scalar ghost value.
+__device__ inline float load_scalar(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, int x, int y, int z) {
+    int c[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] >= 0 && c[a] < g.n[a]) continue;
+        const int side = c[a] < 0 ? 0 : 1;
+        if (bc.periodic[a]) c[a] = wrapi(c[a], g.n[a]);
+        else if (bc.type[2 * a + side] == kZeroFlux) c[a] = side ? g.n[a] - 1 : 0;
+        else return bc.value[2 * a + side];
+    }
+    return f[cidx(g, c[0], c[1], c[2])];
+}
+
+// B:202-225 -- wrap or clamp.
+__device__ inline float load_flow_cell(const Grid& g, const float* __restrict__ f, int x, int y, int z) {
+    int c[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] >= 0 && c[a] < g.n[a]) continue;
+        c[a] = g.flow_periodic[a] ? wrapi(c[a], g.n[a]) : (c[a] < 0 ? 0 : g.n[a] - 1);
+    }
+    return f[cidx(g, c[0], c[1], c[2])];
+}
+
+// Cell-centred velocity component `comp` (face average, B:859-868) at an
+// in-domain cell.
+__device__ __forceinline__ float center_velocity(const Grid& g, const float* __restrict__ f, int comp, int x, int y, int z) {
+    int hi[3] = {x, y, z};
+    hi[comp] += 1;
+    return 0.5f * (f[fidx(g, comp, x, y, z)] + f[fidx(g, comp, hi[0], hi[1], hi[2])]);
+}
+
+// B:227-274 -- ghost rule for the cell-centred velocity, evaluated from the
+// face array directly.
+__device__ inline float load_center_component(const Grid& g, const float* __restrict__ f, int comp, int x, int y, int z) {
+    int c[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] >= 0 && c[a] < g.n[a]) continue;
+        const int side = c[a] < 0 ? 0 : 1;
+        if (g.flow_periodic[a]) {
+            c[a] = wrapi(c[a], g.n[a]);
+            continue;
+        }
+        int cl[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) cl[b] = b == a ? (side ? g.n[a] - 1 : 0) : clampi(c[b], 0, g.n[b] - 1);
+        const float interior = center_velocity(g, f, comp, cl[0], cl[1], cl[2]);
+        const uint32_t t = g.flow_type[2 * a + side];
+        if (t == kOutflow) return interior;
+        if (t == kFreeSlip && comp != a) return interior;
+        return 2.0f * g.flow_velocity[2 * a + side][comp] - interior;
+    }
+    return center_velocity(g, f, comp, c[0], c[1], c[2]);
+}
+
+// B:276-376 -- MAC face ghost value for component `comp`.
+__device__ inline float load_face(const Grid& g, const float* __restrict__ f, int comp, int i, int j, int k) {
+    int c[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int hi = g.n[a] - (a == comp ? 0 : 1);
+        if (c[a] >= 0 && c[a] <= hi) continue;
+        const int side = c[a] < 0 ? 0 : 1;
+        if (g.flow_periodic[a]) {
+            c[a] = wrapi(c[a], g.n[a]);
+            continue;
+        }
+        int cl[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) cl[b] = b == a ? (side ? hi : 0) : clampi(c[b], 0, g.n[b] - (b == comp ? 0 : 1));
+        const float interior = f[fidx(g, comp, cl[0], cl[1], cl[2])];
+        const uint32_t t = g.flow_type[2 * a + side];
+        if (a == comp) {
+            if (t == kOutflow) return interior;
+        } else if (t == kOutflow || t == kFreeSlip) {
+            return interior;
+        }
+        return 2.0f * g.flow_velocity[2 * a + side][comp] - interior;
+    }
+    return f[fidx(g, comp, c[0], c[1], c[2])];
+}
+
+// B:378-392
+__device__ __forceinline__ float monotonic_cubic(float p0, float p1, float p2, float p3, float t) {
+    const float delta = p2 - p1;
+    float m1 = 0.5f * (p2 - p0);
+    float m2 = 0.5f * (p3 - p1);
+    if (fabsf(delta) < 1.0e-6f) {
+        m1 = 0.0f;
+        m2 = 0.0f;
+    } else {
+        if (m1 * delta <= 0.0f) m1 = 0.0f;
+        if (m2 * delta <= 0.0f) m2 = 0.0f;
+    }
+    const float t2 = t * t;
+    const float t3 = t2 * t;
+    return (2.0f * t3 - 3.0f * t2 + 1.0f) * p1 + (t3 - 2.0f * t2 + t) * m1 + (-2.0f * t3 + 3.0f * t2) * p2 + (t3 - t2) * m2;
+}
+
+// Sampling position -> base index and fraction on each axis.  `comp` is the
+// staggered axis of a face component, -1 for cell-centred data; `periodic`
+// selects the fmod wrap (B:394-410 and the other samplers' preambles).
+struct Stencil {
+    int b[3];
+    float t[3];
+};
+
+__device__ __forceinline__ Stencil locate(const Grid& g, const int periodic[3], int comp, float x, float y, float z) {
+    float p[3] = {x, y, z};
+    Stencil s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (periodic[a]) {
+            const float extent = float(g.n[a]) * g.h;
+            p[a] = fmodf(p[a], extent);
+            if (p[a] < 0.0f) p[a] += extent;
+        }
+        // Beyond two cells outside the domain every tap is the same ghost
+        // value, so clamping there changes no result; it keeps b +- 2 (and a
+        // non-finite position) inside int range for the bounds tests below.
+        const float gc = fminf(fmaxf(a == comp ? p[a] / g.h : p[a] / g.h - 0.5f, -4.0f), float(g.n[a]) + 4.0f);
+        s.b[a] = int(floorf(gc));
+        s.t[a] = gc - float(s.b[a]);
+    }
+    return s;
+}
+
+// Generic tap loader for a face component or a scalar.
+template <int COMP>
+__device__ __forceinline__ float tap(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, int i, int j, int k) {
+    if constexpr (COMP < 0) return load_scalar(g, bc, f, i, j, k);
+    else return load_face(g, f, COMP, i, j, k);
+}
+
+template <int COMP>
+__device__ __forceinline__ bool stencil_inside(const Grid& g, const Stencil& s, int lo, int hi) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int top = g.n[a] - ((a == COMP) ? 0 : 1);
+        ok = ok && (s.b[a] + lo >= 0) && (s.b[a] + hi <= top);
+    }
+    return ok;
+}
+
+// B:394-440 / 486-530 etc. -- trilinear, lerp order x, y, z.
+template <int COMP>
+__device__ inline float sample_linear(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, const Stencil& s) {
+    float c[8];
+    if (stencil_inside<COMP>(g, s, 0, 1)) {
+        const int d0 = g.n[0] + (COMP == 0), d1 = g.n[1] + (COMP == 1);
+        const int64_t base = (int64_t(s.b[2]) * d1 + s.b[1]) * d0 + s.b[0];
+        const int64_t sy = d0, sz = int64_t(d0) * d1;
+        c[0] = f[base];
+        c[1] = f[base + 1];
+        c[2] = f[base + sy];
+        c[3] = f[base + sy + 1];
+        c[4] = f[base + sz];
+        c[5] = f[base + sz + 1];
+        c[6] = f[base + sz + sy];
+        c[7] = f[base + sz + sy + 1];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = tap<COMP>(g, bc, f, s.b[0] + (q & 1), s.b[1] + ((q >> 1) & 1), s.b[2] + (q >> 2));
+    }
+    const float c00 = c[0] + (c[1] - c[0]) * s.t[0];
+    const float c10 = c[2] + (c[3] - c[2]) * s.t[0];
+    const float c01 = c[4] + (c[5] - c[4]) * s.t[0];
+    const float c11 = c[6] + (c[7] - c[6]) * s.t[0];
+    const float c0 = c00 + (c10 - c00) * s.t[1];
+    const float c1 = c01 + (c11 - c01) * s.t[1];
+    return c0 + (c1 - c0) * s.t[2];
+}
+
+// B:442-484 / 532-573 etc. -- 4x4x4 taps, monotone cubic along x, then y, then z.
+template <int COMP>
+__device__ inline float sample_cubic(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, const Stencil& s) {
+    float zs[4];
+    if (stencil_inside<COMP>(g, s, -1, 2)) {
+        const int d0 = g.n[0] + (COMP == 0), d1 = g.n[1] + (COMP == 1);
+        const int64_t sy = d0, sz = int64_t(d0) * d1;
+        const int64_t base = (int64_t(s.b[2] - 1) * d1 + (s.b[1] - 1)) * d0 + (s.b[0] - 1);
+#pragma unroll
+        for (int dz = 0; dz < 4; ++dz) {
+            float ys[4];
+#pragma unroll
+            for (int dy = 0; dy < 4; ++dy) {
+                const float* row = f + base + dz * sz + dy * sy;
+                ys[dy] = monotonic_cubic(row[0], row[1], row[2], row[3], s.t[0]);
+            }
+            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+        }
+    } else {
+        for (int dz = 0; dz < 4; ++dz) {
+            float ys[4];
+            for (int dy = 0; dy < 4; ++dy) {
+                const int yy = s.b[1] + dy - 1, zz = s.b[2] + dz - 1;
+                const float p0 = tap<COMP>(g, bc, f, s.b[0] - 1, yy, zz);
+                const float p1 = tap<COMP>(g, bc, f, s.b[0], yy, zz);
+                const float p2 = tap<COMP>(g, bc, f, s.b[0] + 1, yy, zz);
+                const float p3 = tap<COMP>(g, bc, f, s.b[0] + 2, yy, zz);
+                ys[dy] = monotonic_cubic(p0, p1, p2, p3, s.t[0]);
+            }
+            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+        }
+    }
+    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], s.t[2]);
+}
+
+template <int COMP>
+__device__ __forceinline__ float sample_face(const Grid& g, const float* __restrict__ f, float x, float y, float z, bool cubic) {
+    const Stencil s = locate(g, g.flow_periodic, COMP, x, y, z);
+    return cubic ? sample_cubic<COMP>(g, g.density, f, s) : sample_linear<COMP>(g, g.density, f, s);
+}
+
+__device__ __forceinline__ float sample_scalar(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, float x, float y, float z, bool cubic) {
+    const Stencil s = locate(g, bc.periodic, -1, x, y, z);
+    return cubic ? sample_cubic<-1>(g, bc, f, s) : sample_linear<-1>(g, bc, f, s);
+}
+
+// Hit test of B:785-794 / 819-828.
+__device__ __forceinline__ bool hits_solid(const Grid& g, const uint8_t* __restrict__ occ, float qx, float qy, float qz) {
+    float q[3] = {qx, qy, qz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (g.flow_periodic[a]) {
+            const float extent = float(g.n[a]) * g.h;
+            q[a] = fmodf(q[a], extent);
+            if (q[a] < 0.0f) q[a] += extent;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (q[a] < 0.0f || q[a] > float(g.n[a]) * g.h) return true;
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = int(floorf(q[a] / g.h));
+        if (c[a] == g.n[a]) c[a] = g.n[a] - 1;
+    }
+    if (c[0] < 0 || c[0] >= g.n[0] || c[1] < 0 || c[1] >= g.n[1] || c[2] < 0 || c[2] >= g.n[2]) return true;
+    return occ[cidx(g, c[0], c[1], c[2])] != 0;
+}
+
+// B:756-838 -- RK2 midpoint backtrace with linear velocity samples, bisection
+// clipping against walls and solids; the result is not wrapped.
+__device__ inline float3 trace_rk2(const Grid& g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                   const uint8_t* __restrict__ occ, float dt, float sx, float sy, float sz) {
+    const float v0x = sample_face<0>(g, u, sx, sy, sz, false);
+    const float v0y = sample_face<1>(g, v, sx, sy, sz, false);
+    const float v0z = sample_face<2>(g, w, sx, sy, sz, false);
+    const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
+    const float v1x = sample_face<0>(g, u, mx, my, mz, false);
+    const float v1y = sample_face<1>(g, v, mx, my, mz, false);
+    const float v1z = sample_face<2>(g, w, mx, my, mz, false);
+    const float tx = sx - dt * v1x, ty = sy - dt * v1y, tz = sz - dt * v1z;
+    if (!hits_solid(g, occ, tx, ty, tz)) return make_float3(tx, ty, tz);
+    float lo = 0.0f, hi = 1.0f;
+    for (int it = 0; it < 10; ++it) {
+        const float mt = 0.5f * (lo + hi);
+        if (hits_solid(g, occ, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tz - sz) * mt)) hi = mt;
+        else lo = mt;
+    }
+    return make_float3(sx + (tx - sx) * lo, sy + (ty - sy) * lo, sz + (tz - sz) * lo);
+}
+
+// B:1007-1012 with the face-neighbour logic of B:282-304: mean solid velocity
+// of the solid cells beside face (i,j,k) of component `comp`.
+__device__ inline bool solid_face_override(const Grid& g, const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel, int comp, int i, int j,
+                                           int k, float& value) {
+    int left[3] = {i, j, k};
+    int right[3] = {i, j, k};
+    left[comp] -= 1;
+    const bool hl = resolve_flow(g, left);
+    const bool hr = resolve_flow(g, right);
+    const int64_t il = hl ? cidx(g, left[0], left[1], left[2]) : 0;
+    const int64_t ir = hr ? cidx(g, right[0], right[1], right[2]) : 0;
+    const bool lo = hl && occ[il] != 0;
+    const bool ro = hr && occ[ir] != 0;
+    if (!lo && !ro) return false;
+    float sum = 0.0f, weight = 0.0f;
+    if (lo) {
+        sum += solid_vel[il];
+        weight += 1.0f;
+    }
+    if (ro) {
+        sum += solid_vel[ir];
+        weight += 1.0f;
+    }
+    value = sum / weight;
+    return true;
+}
+
+// Face (i,j,k) of component comp is a domain face handled by the boundary
+// pass: a non-periodic end face, or the duplicate end face n of a periodic axis.
+__device__ __forceinline__ bool boundary_face(const Grid& g, int comp, int idx) {
+    if (idx == 0) return !g.flow_periodic[comp];
+    return idx == g.n[comp];
+}
+
+}  // namespace smoke

@@ -1,0 +1,561 @@
+// Stage kernels of the smoke step other than the pressure solve.
+//
+// Face kernels never write the domain end faces (non-periodic faces 0 and n,
+// and face n of a periodic axis); a per-line boundary pass writes them after
+// the interior is final, which gives the race-free reading of the reference's
+// enforce / project passes (B:1014-1189, B:1425-1600): OUTFLOW copies the
+// finished neighbour face, walls take their prescribed normal velocity, a
+// periodic face n copies face 0.
+#include "context.hpp"
+
+namespace smoke {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline dim3 grid3(int nx, int ny, int nz, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y, (nz + b.z - 1) / b.z); }
+inline unsigned grid1(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+const dim3 kCellBlock(64, 4, 1);
+
+__global__ void fill_kernel(float* __restrict__ p, float value, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void fill_u8_kernel(uint8_t* __restrict__ p, uint8_t value, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// ------------------------------------------------------------ time step
+__global__ void max_speed_kernel(const float* __restrict__ u, int64_t nu, const float* __restrict__ v, int64_t nv, const float* __restrict__ w,
+                                 int64_t nw, unsigned* __restrict__ max_bits) {
+    float m = 0.0f;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nu + nv + nw; i += stride) {
+        const float s = i < nu ? u[i] : (i < nu + nv ? v[i - nu] : w[i - nu - nv]);
+        m = fmaxf(m, fabsf(s));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off));
+    __shared__ float red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / 64; ++q) m = fmaxf(m, red[q]);
+        red[0] = m;
+        // non-negative floats order like their bit patterns
+        atomicMax(max_bits, __float_as_uint(red[0]));
+    }
+}
+
+// dt = min(dt_max, cfl h / max|u|) (adaptive) or the configured dt.
+__global__ void time_step_kernel(unsigned* __restrict__ max_bits, float* __restrict__ max_speed, float* __restrict__ dt, int adaptive, float cfl,
+                                 float h, float dt_max, float dt_fixed) {
+    const float m = __uint_as_float(*max_bits);
+    *max_speed = m;
+    *max_bits = 0u;
+    if (!adaptive) {
+        *dt = dt_fixed;
+        return;
+    }
+    const float cand = m > 0.0f ? cfl * h / m : dt_max;
+    *dt = fminf(dt_max, cand);
+}
+
+// ------------------------------------------------------------ forces (stage 0)
+// B:2001-2003: solid temperature, then omega and |omega| from the
+// cell-centred velocity with the flow ghost rules (B:870-900).
+__global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                 const uint8_t* __restrict__ occ, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    if (occ[i] != 0) {
+        if (temperature) temperature[i] = solid_temperature[i];
+        ox[i] = oy[i] = oz[i] = om[i] = 0.0f;
+        return;
+    }
+    const float h = g.h;
+    const float dvz_dy = 0.5f * (load_center_component(g, w, 2, x, y + 1, z) - load_center_component(g, w, 2, x, y - 1, z)) / h;
+    const float dvy_dz = 0.5f * (load_center_component(g, v, 1, x, y, z + 1) - load_center_component(g, v, 1, x, y, z - 1)) / h;
+    const float dvx_dz = 0.5f * (load_center_component(g, u, 0, x, y, z + 1) - load_center_component(g, u, 0, x, y, z - 1)) / h;
+    const float dvz_dx = 0.5f * (load_center_component(g, w, 2, x + 1, y, z) - load_center_component(g, w, 2, x - 1, y, z)) / h;
+    const float dvy_dx = 0.5f * (load_center_component(g, v, 1, x + 1, y, z) - load_center_component(g, v, 1, x - 1, y, z)) / h;
+    const float dvx_dy = 0.5f * (load_center_component(g, u, 0, x, y + 1, z) - load_center_component(g, u, 0, x, y - 1, z)) / h;
+    const float wx = dvz_dy - dvy_dz, wy = dvx_dz - dvz_dx, wz = dvy_dx - dvx_dy;
+    ox[i] = wx;
+    oy[i] = wy;
+    oz[i] = wz;
+    om[i] = sqrtf(wx * wx + wy * wy + wz * wz);
+}
+
+// B:902-948: user force, buoyancy (+y) and vorticity confinement in fluid cells.
+__global__ void cell_force_kernel(Grid g, const float* __restrict__ fux, const float* __restrict__ fuy, const float* __restrict__ fuz,
+                                  const float* __restrict__ rho, const float* __restrict__ T, const uint8_t* __restrict__ occ,
+                                  const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
+                                  const float* __restrict__ om, float a, float b, float tamb, float eps, float* __restrict__ fx,
+                                  float* __restrict__ fy, float* __restrict__ fz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    float sx = fux[i], sy = fuy[i], sz = fuz[i];
+    if (occ[i] == 0) {
+        sy += -a * rho[i] + b * (T[i] - tamb);
+        const float h = g.h;
+        const float gx = 0.5f * (load_flow_cell(g, om, x + 1, y, z) - load_flow_cell(g, om, x - 1, y, z)) / h;
+        const float gy = 0.5f * (load_flow_cell(g, om, x, y + 1, z) - load_flow_cell(g, om, x, y - 1, z)) / h;
+        const float gz = 0.5f * (load_flow_cell(g, om, x, y, z + 1) - load_flow_cell(g, om, x, y, z - 1)) / h;
+        const float gm = sqrtf(gx * gx + gy * gy + gz * gz);
+        if (!(gm < 1.0e-6f)) {
+            const float inv = 1.0f / gm;
+            const float nx = gx * inv, ny = gy * inv, nz = gz * inv;
+            const float wx = ox[i], wy = oy[i], wz = oz[i];
+            sx += eps * h * (ny * wz - nz * wy);
+            sy += eps * h * (nz * wx - nx * wz);
+            sz += eps * h * (nx * wy - ny * wx);
+        }
+    }
+    fx[i] = sx;
+    fy[i] = sy;
+    fz[i] = sz;
+}
+
+// B:950-1005 + solid part of B:1014-1168: u_face += dt * mean of the
+// in-domain neighbour cell forces (no wrap), then the solid override.
+template <int C>
+__global__ void face_force_kernel(Grid g, float* __restrict__ f, const float* __restrict__ force, const uint8_t* __restrict__ occ,
+                                  const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
+    const int c[3] = {i, j, k};
+    if (boundary_face(g, C, c[C])) return;
+    const int64_t fi = fidx(g, C, i, j, k);
+    float value;
+    if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
+        f[fi] = value;
+        return;
+    }
+    float sum = 0.0f, weight = 0.0f;
+    if (c[C] > 0) {
+        int l[3] = {i, j, k};
+        l[C] -= 1;
+        sum += force[cidx(g, l[0], l[1], l[2])];
+        weight += 1.0f;
+    }
+    if (c[C] < g.n[C]) {
+        sum += force[cidx(g, i, j, k)];
+        weight += 1.0f;
+    }
+    f[fi] += *dt_ptr * (sum / weight);
+}
+
+// Domain end faces of component C, one thread per line along axis C; both
+// ends are read before either is written.
+template <int C>
+__global__ void boundary_faces_kernel(Grid g, float* __restrict__ f) {
+    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+    const int q1 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q2 = blockIdx.y * blockDim.y + threadIdx.y;
+    if (q1 >= g.n[A1] || q2 >= g.n[A2]) return;
+    auto at = [&](int idx) {
+        int c[3];
+        c[C] = idx;
+        c[A1] = q1;
+        c[A2] = q2;
+        return fidx(g, C, c[0], c[1], c[2]);
+    };
+    const int n = g.n[C];
+    if (g.flow_periodic[C]) {
+        f[at(n)] = f[at(0)];
+        return;
+    }
+    const float inner_lo = f[at(1)];
+    const float inner_hi = f[at(n - 1)];
+    const uint32_t tl = g.flow_type[2 * C], th = g.flow_type[2 * C + 1];
+    const float lo = tl == kOutflow ? inner_lo : g.flow_velocity[2 * C][C];
+    const float hi = th == kOutflow ? inner_hi : g.flow_velocity[2 * C + 1][C];
+    f[at(0)] = lo;
+    f[at(n)] = hi;
+}
+
+// ------------------------------------------------------------ velocity advection (stage 1)
+// B:1191-1219 + the enforce that follows (B:2025-2030): every interior face is
+// traced and sampled unless a solid override replaces it.
+template <int C>
+__global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                   float* __restrict__ dst, const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
+                                   const float* __restrict__ dt_ptr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
+    const int c[3] = {i, j, k};
+    if (boundary_face(g, C, c[C])) return;
+    const int64_t fi = fidx(g, C, i, j, k);
+    float value;
+    if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
+        dst[fi] = value;
+        return;
+    }
+    const float h = g.h;
+    const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
+    const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
+    const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
+    const float3 p = trace_rk2(g, u, v, w, occ, *dt_ptr, sx, sy, sz);
+    const float* src = C == 0 ? u : (C == 1 ? v : w);
+    dst[fi] = sample_face<C>(g, src, p.x, p.y, p.z, g.cubic != 0);
+}
+
+// ------------------------------------------------------------ projection (stage 3)
+// B:1425-1600 + enforce + copy back (B:2032-2043): the projected faces are
+// written straight into the state array.
+template <int C>
+__global__ void project_kernel(Grid g, const float* __restrict__ ustar, float* __restrict__ out, const float* __restrict__ p,
+                               const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
+    const int c[3] = {i, j, k};
+    if (boundary_face(g, C, c[C])) return;
+    const int64_t fi = fidx(g, C, i, j, k);
+    float value;
+    if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
+        out[fi] = value;
+        return;
+    }
+    int l[3] = {i, j, k};
+    int r[3] = {i, j, k};
+    l[C] -= 1;
+    const bool hl = resolve_flow(g, l), hr = resolve_flow(g, r);
+    float result = ustar[fi];
+    if (hl && hr) result -= *dt_ptr * (p[cidx(g, r[0], r[1], r[2])] - p[cidx(g, l[0], l[1], l[2])]) / g.h;
+    out[fi] = result;
+}
+
+// ------------------------------------------------------------ scalars (stage 4)
+// B:852-857: tmp = cur + dt * src for both scalars.
+__global__ void scalar_sources_kernel(const float* __restrict__ T, const float* __restrict__ ST, const float* __restrict__ rho,
+                                      const float* __restrict__ Srho, float* __restrict__ Ttmp, float* __restrict__ rtmp,
+                                      const float* __restrict__ dt_ptr, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dt = *dt_ptr;
+    Ttmp[i] = T[i] + dt * ST[i];
+    rtmp[i] = rho[i] + dt * Srho[i];
+}
+
+// B:1221-1241: one trace per fluid cell shared by temperature and density;
+// solid cells take the solid temperature and a density filled afterwards.
+__global__ void advect_scalars_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                      const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                      float* __restrict__ T, float* __restrict__ rho, const float* __restrict__ solid_T,
+                                      const float* __restrict__ dt_ptr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    if (occ[i] != 0) {
+        T[i] = solid_T[i];
+        rho[i] = 0.0f;
+        return;
+    }
+    const float h = g.h;
+    const float3 p = trace_rk2(g, u, v, w, occ, *dt_ptr, (float(x) + 0.5f) * h, (float(y) + 0.5f) * h, (float(z) + 0.5f) * h);
+    const bool cubic = g.cubic != 0;
+    T[i] = sample_scalar(g, g.temperature, Ttmp, p.x, p.y, p.z, cubic);
+    rho[i] = sample_scalar(g, g.density, rtmp, p.x, p.y, p.z, cubic);
+}
+
+// B:1243-1290 as a gather through the nearest-fluid map built at occupancy update.
+__global__ void density_fill_kernel(float* __restrict__ rho, const int64_t* __restrict__ pairs, const unsigned* __restrict__ count) {
+    const unsigned n = *count;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int64_t cell = pairs[2 * q], src = pairs[2 * q + 1];
+        rho[cell] = src >= 0 ? rho[src] : 0.0f;
+    }
+}
+
+// ------------------------------------------------------------ occupancy-derived data
+__global__ void occupancy_float_kernel(const uint8_t* __restrict__ occ, float* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = float(occ[i]);
+}
+
+__global__ void anchor_reset_kernel(int* anchor, int cells, unsigned* solid_count) {
+    *anchor = cells;
+    *solid_count = 0u;
+}
+
+// B:1297-1302
+__global__ void anchor_kernel(const uint8_t* __restrict__ occ, int* __restrict__ anchor, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && occ[i] == 0) atomicMin(anchor, int(i));
+}
+
+// Per-cell stencil flags of the Poisson operator (the structure of
+// B:1343-1412): bit d set when the neighbour in direction d (x-,x+,y-,y+,z-,z+)
+// exists after periodic wrap and is fluid; bit 6 marks identity rows (solid
+// cells and the anchor).
+__global__ void stencil_flags_kernel(Grid g, const uint8_t* __restrict__ occ, const int* __restrict__ anchor, uint8_t* __restrict__ flags) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    if (occ[i] != 0 || i == int64_t(*anchor)) {
+        flags[i] = 0x40;
+        return;
+    }
+    uint8_t m = 0;
+    const int base[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int c[3] = {base[0], base[1], base[2]};
+            c[a] += s ? 1 : -1;
+            if (c[a] < 0 || c[a] >= g.n[a]) {
+                if (!g.flow_periodic[a]) continue;
+                c[a] = wrapi(c[a], g.n[a]);
+            }
+            if (occ[cidx(g, c[0], c[1], c[2])] == 0) m |= uint8_t(1u << (2 * a + s));
+        }
+    }
+    flags[i] = m;
+}
+
+// B:1243-1290 run once per occupancy: nearest fluid cell of every solid cell.
+__global__ void nearest_fluid_kernel(Grid g, const uint8_t* __restrict__ occ, int64_t* __restrict__ pairs, unsigned* __restrict__ count) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    if (occ[i] == 0) return;
+    const int max_r = max(g.n[0], max(g.n[1], g.n[2]));
+    int64_t best = -1;
+    for (int r = 1; r <= max_r && best < 0; ++r) {
+        int best_d2 = 0x7fffffff;
+        for (int dz = -r; dz <= r; ++dz)
+            for (int dy = -r; dy <= r; ++dy) {
+                const bool face_yz = abs(dz) == r || abs(dy) == r;
+                for (int dx = -r; dx <= r; dx += (face_yz ? 1 : 2 * r)) {
+                    int c[3] = {x + dx, y + dy, z + dz};
+                    if (!resolve_scalar(g, g.density, c)) continue;
+                    const int64_t ni = cidx(g, c[0], c[1], c[2]);
+                    if (occ[ni] != 0) continue;
+                    const int d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < best_d2) {
+                        best_d2 = d2;
+                        best = ni;
+                    }
+                }
+            }
+    }
+    const unsigned slot = atomicAdd(count, 1u);
+    pairs[2 * slot] = i;
+    pairs[2 * slot + 1] = best;
+}
+
+__global__ void zero_identity_rows_kernel(const uint8_t* __restrict__ flags, double* __restrict__ x0, double* __restrict__ x1, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && (flags[i] & 0x40)) {
+        x0[i] = 0.0;
+        x1[i] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------ diagnostics (views)
+// B:859-868, 1330-1341, 1602-1609.
+__global__ void diagnostics_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                   const uint8_t* __restrict__ occ, float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz,
+                                   float* __restrict__ vmag, float* __restrict__ div) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    const float u0 = u[fidx(g, 0, x, y, z)], u1 = u[fidx(g, 0, x + 1, y, z)];
+    const float v0 = v[fidx(g, 1, x, y, z)], v1 = v[fidx(g, 1, x, y + 1, z)];
+    const float w0 = w[fidx(g, 2, x, y, z)], w1 = w[fidx(g, 2, x, y, z + 1)];
+    const float a = 0.5f * (u0 + u1), b = 0.5f * (v0 + v1), c = 0.5f * (w0 + w1);
+    cx[i] = a;
+    cy[i] = b;
+    cz[i] = c;
+    vmag[i] = sqrtf(a * a + b * b + c * c);
+    div[i] = occ[i] != 0 ? 0.0f : (u1 - u0 + v1 - v0 + w1 - w0) / g.h;
+}
+
+// Spherical emitter of the scenario harness (same bits as the CPU oracle).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void plume_kernel(Grid g, SmokeSimulationPlumeSource src, uint32_t step, float* __restrict__ rs, float* __restrict__ ts) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+    const int64_t i = cidx(g, x, y, z);
+    const float dx = (float(x) + 0.5f) - src.center_x;
+    const float dy = (float(y) + 0.5f) - src.center_y;
+    const float dz = (float(z) + 0.5f) - src.center_z;
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __fmul_rn(dx, dx)));
+    if (r2 > __fmul_rn(src.radius, src.radius)) {
+        rs[i] = 0.0f;
+        ts[i] = 0.0f;
+        return;
+    }
+    const uint32_t bits = mix32(uint32_t(i) ^ mix32(step ^ mix32(src.seed)));
+    const float unit = __fsub_rn(__fmul_rn(__fmul_rn(float(bits >> 8), 1.0f / 16777216.0f), 2.0f), 1.0f);
+    rs[i] = src.density_rate;
+    ts[i] = __builtin_fmaf(src.temperature_noise, unit, src.temperature_rate);
+}
+
+template <int C>
+void launch_face_force(Context& c) {
+    const Grid& g = c.grid;
+    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock);
+    face_force_kernel<C><<<grid, kCellBlock, 0, c.stream>>>(g, c.face[C], c.cell_force[C], c.occupancy, c.solid_velocity[C], c.dt_device);
+}
+
+template <int C>
+void launch_boundary(Context& c, float* f) {
+    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+    const dim3 block(64, 4, 1);
+    const dim3 grid = grid3(c.grid.n[A1], c.grid.n[A2], 1, block);
+    boundary_faces_kernel<C><<<grid, block, 0, c.stream>>>(c.grid, f);
+}
+
+void launch_boundaries(Context& c, float* const f[3]) {
+    launch_boundary<0>(c, f[0]);
+    launch_boundary<1>(c, f[1]);
+    launch_boundary<2>(c, f[2]);
+}
+
+template <int C>
+void launch_advect(Context& c) {
+    const Grid& g = c.grid;
+    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock);
+    advect_face_kernel<C><<<grid, kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
+                                                             c.dt_device);
+}
+
+template <int C>
+void launch_project(Context& c) {
+    const Grid& g = c.grid;
+    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock);
+    project_kernel<C><<<grid, kCellBlock, 0, c.stream>>>(g, c.face_star[C], c.face[C], c.pressure, c.occupancy, c.solid_velocity[C], c.dt_device);
+}
+
+dim3 cell_grid(const Grid& g) { return grid3(g.n[0], g.n[1], g.n[2], kCellBlock); }
+
+}  // namespace
+
+void enqueue_fill(Context& c, float* p, float value, int64_t n) {
+    fill_kernel<<<grid1(n), kBlock, 0, c.stream>>>(p, value, n);
+    hip_check(hipGetLastError(), "fill_kernel");
+}
+
+void enqueue_fill_u8(Context& c, uint8_t* p, uint8_t value, int64_t n) {
+    fill_u8_kernel<<<grid1(n), kBlock, 0, c.stream>>>(p, value, n);
+    hip_check(hipGetLastError(), "fill_u8_kernel");
+}
+
+void enqueue_occupancy_derived(Context& c) {
+    const Grid& g = c.grid;
+    occupancy_float_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.occupancy, c.occupancy_float, g.cells);
+    anchor_reset_kernel<<<1, 1, 0, c.stream>>>(c.anchor, int(g.cells), c.solid_count);
+    anchor_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.occupancy, c.anchor, g.cells);
+    stencil_flags_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.occupancy, c.anchor, c.stencil_flags);
+    nearest_fluid_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.occupancy, c.solid_pairs, c.solid_count);
+    zero_identity_rows_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.stencil_flags, c.x64[0], c.x64[1], g.cells);
+    hip_check(hipGetLastError(), "occupancy-derived kernels");
+    c.diagnostics_dirty = true;
+}
+
+void enqueue_time_step(Context& c) {
+    const Grid& g = c.grid;
+    if (c.options.adaptive_dt) {
+        const int64_t total = g.faces[0] + g.faces[1] + g.faces[2];
+        const unsigned blocks = unsigned(std::min<int64_t>(2048, (total + kBlock - 1) / kBlock));
+        max_speed_kernel<<<blocks, kBlock, 0, c.stream>>>(c.face[0], g.faces[0], c.face[1], g.faces[1], c.face[2], g.faces[2], c.max_speed_bits);
+    }
+    time_step_kernel<<<1, 1, 0, c.stream>>>(c.max_speed_bits, c.max_speed, c.dt_device, c.options.adaptive_dt ? 1 : 0, c.options.cfl, g.h,
+                                            c.options.dt_max, c.config.dt);
+    hip_check(hipGetLastError(), "time_step_kernel");
+}
+
+void enqueue_forces(Context& c) {
+    const Grid& g = c.grid;
+    vorticity_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1],
+                                                                 c.vorticity[2], c.vorticity_magnitude, c.temperature, c.solid_temperature);
+    cell_force_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(
+        g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
+        c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor, c.config.ambient_temperature,
+        c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2]);
+    launch_face_force<0>(c);
+    launch_face_force<1>(c);
+    launch_face_force<2>(c);
+    launch_boundaries(c, c.face);
+    hip_check(hipGetLastError(), "force stage");
+}
+
+void enqueue_advect_velocity(Context& c) {
+    launch_advect<0>(c);
+    launch_advect<1>(c);
+    launch_advect<2>(c);
+    launch_boundaries(c, c.face_star);
+    hip_check(hipGetLastError(), "velocity advection");
+}
+
+void enqueue_project(Context& c) {
+    launch_project<0>(c);
+    launch_project<1>(c);
+    launch_project<2>(c);
+    launch_boundaries(c, c.face);
+    hip_check(hipGetLastError(), "projection");
+    c.diagnostics_dirty = true;
+}
+
+void enqueue_scalars(Context& c) {
+    const Grid& g = c.grid;
+    scalar_sources_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.temperature, c.temperature_source, c.density, c.density_source,
+                                                                     c.temperature_tmp, c.density_tmp, c.dt_device, g.cells);
+    advect_scalars_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
+                                                                      c.density_tmp, c.temperature, c.density, c.solid_temperature, c.dt_device);
+    density_fill_kernel<<<256, kBlock, 0, c.stream>>>(c.density, c.solid_pairs, c.solid_count);
+    hip_check(hipGetLastError(), "scalar stage");
+    c.diagnostics_dirty = true;
+}
+
+void enqueue_diagnostics(Context& c) {
+    const Grid& g = c.grid;
+    vorticity_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1],
+                                                                 c.vorticity[2], c.vorticity_magnitude, nullptr, nullptr);
+    diagnostics_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.center_velocity[0],
+                                                                   c.center_velocity[1], c.center_velocity[2], c.velocity_magnitude, c.divergence);
+    hip_check(hipGetLastError(), "diagnostics");
+    c.diagnostics_dirty = false;
+}
+
+void enqueue_plume_sources(Context& c, const SmokeSimulationPlumeSource& src, uint32_t step) {
+    plume_kernel<<<cell_grid(c.grid), kCellBlock, 0, c.stream>>>(c.grid, src, step, c.density_source, c.temperature_source);
+    hip_check(hipGetLastError(), "plume_kernel");
+}
+
+}  // namespace smoke
