@@ -676,7 +676,10 @@ void pressure_rhs(const Geo& g, const uint8_t* occ, int64_t anchor, const R* us,
     });
 }
 
-inline double guarded_ratio(double num, double den) { return std::fabs(den) > 1.0e-20 ? num / den : 0.0; }
+// B:1414-1418 guards the CG ratios with |den| > 1e-20 (fixed-K mode keeps it);
+// the tolerance solve only guards against an exact zero, since small right-hand
+// sides legitimately drive r.z below 1e-20 before the target is reached.
+inline double guarded_ratio(double num, double den, double limit = 1.0e-20) { return std::fabs(den) > limit ? num / den : 0.0; }
 
 // B:1800-1823 -- fixed-K unpreconditioned CG from x = 0 (dots in FP64).
 template <class R>
@@ -761,7 +764,7 @@ void solve_tolerance(const Geo& g, const PoissonRows& rows, const OracleOptions*
                 }
                 for (double v : part) rz_new += v;
             }
-            if (!first) beta = guarded_ratio(rz_new, rz);
+            if (!first) beta = guarded_ratio(rz_new, rz, 1.0e-300);
             rz = rz_new;
             first = false;
             const R rbeta = R(beta);
@@ -772,7 +775,7 @@ void solve_tolerance(const Geo& g, const PoissonRows& rows, const OracleOptions*
             }
             apply_rows(g, rows, p.data(), ap.data());
             const double pap = dot(g, p.data(), ap.data());
-            const R alpha = R(guarded_ratio(rz, pap));
+            const R alpha = R(guarded_ratio(rz, pap, 1.0e-300));
 #pragma omp parallel for schedule(static)
             for (int64_t i = 0; i < int64_t(n); ++i) {
                 e[size_t(i)] += alpha * p[size_t(i)];

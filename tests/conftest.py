@@ -8,6 +8,7 @@ REPO = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(REPO / "visual-simulation-of-smoke_amd"))
 sys.path.insert(0, str(REPO / "oracle"))
 sys.path.insert(0, str(REPO / "tests"))
+sys.path.insert(0, str(REPO))
 
 
 def pytest_configure(config):

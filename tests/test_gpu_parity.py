@@ -143,6 +143,16 @@ def test_full_step(name):
         assert errs[k] < 1e-3, errs
 
 
+@pytest.mark.parametrize("name", ["closed", "benchmark"])
+def test_full_step_ragged_multi_tile(name):
+    # wider than one 64-column tile, taller than one 8-row tile, ragged in every
+    # axis and deep enough for several z chunks of the PCG passes
+    errs = one_step_parity(cfg_for(name, n=(150, 37, 23)), seed=13, solids=3)
+    assert errs["solver_relative_residual"] <= 1e-6
+    for k in ("u", "v", "w", "density", "temperature"):
+        assert errs[k] < 1e-3, errs
+
+
 def test_fixed_iteration_cg_matches_reference_semantics():
     # reference mode: K plain CG iterations from p = 0 (B:1800-1823)
     cfg = cfg_for("closed")
@@ -249,3 +259,31 @@ def test_plume_sources_match_oracle_bits():
     ors, ots = orc.plume_sources(cfg, src, 5)
     np.testing.assert_array_equal(rs, ors.astype(np.float32))
     np.testing.assert_array_equal(ts, ots.astype(np.float32))
+
+
+def test_full_size_256_plume_properties():
+    """The bench workload at its full size (256^3): size-independent checks --
+    every solve reaches the true FP64 residual 1e-6 (re-evaluated on the host
+    with the oracle's operator), and the projection removes the divergence."""
+    import bench
+    n = 256
+    cfg = bench.workload_config(n)
+    src = bench.plume(n)
+    with abi.SmokeContext(cfg) as ctx:
+        bench.solver_options(ctx, 32)
+        for step in range(3):
+            ctx.plume_sources(src, step)
+            ctx.step()
+        st = ctx.stats()
+        rhs = ctx.view("FLOW_PRESSURE_RHS")
+        p64 = ctx.export_pressure_f64()
+        g = ctx.export_state()
+        div = ctx.view("FLOW_DIVERGENCE")
+    assert st.iterations > 0 and st.converged == 1 and st.relative_residual <= 1e-6
+    occ = np.zeros(abi.cell_shape(cfg), np.uint8)
+    b = rhs.astype(np.float64)
+    r = b - orc.pressure_apply(cfg, occ, p64)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0e-6 * 1.01
+    h = np.float32(cfg.cell_size)
+    div_star = (np.diff(g["velocity_star_x"], axis=2) + np.diff(g["velocity_star_y"], axis=1) + np.diff(g["velocity_star_z"], axis=0)) / h
+    assert np.abs(div).max() < 1e-3 * np.abs(div_star).max()

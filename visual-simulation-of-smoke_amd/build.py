@@ -14,7 +14,7 @@ REPO = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libvisual-simulation-of-smoke.so"
-BENCH_SRC = PKG / "bench" / "benchmark.hip"
+BENCH_SRC = PKG / "cli" / "benchmark.hip"
 BENCH_BIN = PKG / "lib" / "visual-simulation-of-smoke-benchmark"
 
 ARCH = os.environ.get("SMOKE_OFFLOAD_ARCH", "gfx950")

@@ -276,7 +276,8 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* _
     PcgScalars* __restrict__ S = a.s;
     const int nblocks = gridDim.x * gridDim.y * gridDim.z;
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int x = blockIdx.x * blockDim.x + (threadIdx.x & 63);
+    // 256 threads = 64 (x) x 4 (y) cells of one z plane
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
     double rr = 0.0, rz = 0.0, bb = 0.0;
