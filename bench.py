@@ -28,8 +28,11 @@ import smoke_abi as abi  # noqa: E402
 
 METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
-PASS_A_BYTES_PER_CELL = 13   # reads r, p_old (4+4), flags (1); writes p_new (4)
-PASS_B_BYTES_PER_CELL = 21   # reads p_new, e, r (4+4+4), flags (1); writes e, r (4+4)
+# algorithmic HBM bytes per cell of the three passes of one PCG iteration
+DIRECTION_BYTES_PER_CELL = 13   # pass 1: reads r, p (4+4), flags (1); writes p (4)
+CURVATURE_BYTES_PER_CELL = 5    # pass 2: reads p (4), flags (1)
+UPDATE_BYTES_PER_CELL = 21      # pass 3: reads p, e, r (4+4+4), flags (1); writes e, r (4+4)
+UPDATE_KERNEL = "pcg_stencil_kernel<64, 8, 2, 4, 1, true>"
 
 
 def workload_config(n: int) -> abi.Config:
@@ -71,7 +74,7 @@ def cpu_baseline(n: int, iterations_per_step: float, threads: int):
 
 
 def load_traffic():
-    f = REPO / "profiles" / "pcg_pass_b_256.traffic.json"
+    f = REPO / "profiles" / "pcg_update_256.traffic.json"
     if f.exists():
         try:
             return json.loads(f.read_text()).get("hbm_bytes_per_launch")
@@ -126,17 +129,29 @@ def main():
     cells = n ** 3
     mean_iters = sum(iters) / len(iters)
     roofline = None
-    if timing.pcg_pass_b_launches > 0:
-        b_ms = timing.pcg_pass_b_ms / timing.pcg_pass_b_launches
-        a_ms = timing.pcg_pass_a_ms / timing.pcg_pass_a_launches
-        achieved = PASS_B_BYTES_PER_CELL * cells / (b_ms * 1e-3) / 1e9
+    if timing.pcg_timed_iterations > 0:
+        k = timing.pcg_timed_iterations
+        ms = {"direction": timing.pcg_direction_ms / k, "curvature": timing.pcg_curvature_ms / k,
+              "update": timing.pcg_update_ms / k}
+        per_cell = {"direction": DIRECTION_BYTES_PER_CELL, "curvature": CURVATURE_BYTES_PER_CELL,
+                    "update": UPDATE_BYTES_PER_CELL}
+
+        def gbps(name):
+            return per_cell[name] * cells / (ms[name] * 1e-3) / 1e9
+
+        achieved = gbps("update")
+        reduction_ms = timing.pcg_reduction_ms / k
+        iteration_ms = sum(ms.values()) + reduction_ms
         roofline = {
-            "bound": "hbm", "kernel": "pcg_pass_kernel<64,8,false,true> (PCG pass B: e += alpha p, r -= alpha Ap, r.z, r.r)",
+            "bound": "hbm", "kernel": UPDATE_KERNEL + " (PCG pass 3: e += alpha p, r -= alpha Ap, r.z, r.r, beta)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": load_traffic(), "algorithmic_bytes_per_launch": PASS_B_BYTES_PER_CELL * cells,
-            "avg_launch_ms": round(b_ms, 5), "launches_timed": timing.pcg_pass_b_launches,
-            "pass_a": {"avg_launch_ms": round(a_ms, 5), "achieved": round(PASS_A_BYTES_PER_CELL * cells / (a_ms * 1e-3) / 1e9, 1),
-                       "algorithmic_bytes_per_launch": PASS_A_BYTES_PER_CELL * cells},
+            "traffic": load_traffic(), "algorithmic_bytes_per_launch": UPDATE_BYTES_PER_CELL * cells,
+            "avg_launch_ms": round(ms["update"], 5), "launches_timed": k,
+            "passes": {name: {"avg_launch_ms": round(ms[name], 5), "achieved_gbps": round(gbps(name), 1),
+                              "algorithmic_bytes_per_launch": per_cell[name] * cells} for name in ms},
+            "finalize_kernels_ms": round(reduction_ms, 5),
+            "pcg_iteration": {"ms": round(iteration_ms, 5),
+                              "achieved_gbps": round(sum(per_cell.values()) * cells / (iteration_ms * 1e-3) / 1e9, 1)},
             "stream_copy_gbps": round(copy_gbps, 1),
         }
     out = {

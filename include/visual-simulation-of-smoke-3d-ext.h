@@ -95,11 +95,12 @@ typedef enum SmokeSimulationStage {
 } SmokeSimulationStage;
 
 typedef struct SmokeSimulationKernelTiming {
-    uint64_t pcg_iterations;      /* iterations timed */
-    double pcg_pass_a_ms;         /* summed device time of the p-update + SpMV + p.Ap pass */
-    double pcg_pass_b_ms;         /* summed device time of the x,r update + r.z pass */
-    uint64_t pcg_pass_a_launches;
-    uint64_t pcg_pass_b_launches;
+    uint64_t pcg_iterations;      /* PCG iterations run while timing was on */
+    uint64_t pcg_timed_iterations;/* iterations whose three passes were bracketed by events */
+    double pcg_direction_ms;      /* summed device time of pass 1: p = z + beta p */
+    double pcg_curvature_ms;      /* summed device time of pass 2: p.Ap, alpha */
+    double pcg_update_ms;         /* summed device time of pass 3: x += alpha p, r -= alpha Ap, r.z, r.r */
+    double pcg_reduction_ms;      /* summed device time of the two one-block finalize kernels (alpha; beta, stop test) */
     double step_ms;               /* summed device time of whole steps */
     uint64_t steps;
 } SmokeSimulationKernelTiming;

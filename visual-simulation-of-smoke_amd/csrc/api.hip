@@ -77,7 +77,7 @@ void destroy_resources(Context& c) {
     }
     for (float** p : {&c.density, &c.temperature, &c.density_source, &c.temperature_source, &c.density_tmp, &c.temperature_tmp, &c.solid_temperature,
                       &c.occupancy_float, &c.vorticity_magnitude, &c.velocity_magnitude, &c.divergence, &c.pressure, &c.pressure_rhs, &c.pcg_r,
-                      &c.pcg_p[0], &c.pcg_p[1], &c.pcg_e, &c.dt_device, &c.max_speed})
+                      &c.pcg_p, &c.pcg_e, &c.dt_device, &c.max_speed})
         release(*p);
     release(c.occupancy);
     release(c.stencil_flags);
@@ -290,7 +290,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         }
         for (float** p : {&c.density, &c.temperature, &c.density_source, &c.temperature_source, &c.density_tmp, &c.temperature_tmp,
                           &c.solid_temperature, &c.occupancy_float, &c.vorticity_magnitude, &c.velocity_magnitude, &c.divergence, &c.pressure,
-                          &c.pressure_rhs, &c.pcg_r, &c.pcg_p[0], &c.pcg_p[1], &c.pcg_e})
+                          &c.pressure_rhs, &c.pcg_r, &c.pcg_p, &c.pcg_e})
             smoke::alloc(*p, cells);
         smoke::alloc(c.occupancy, cells);
         smoke::alloc(c.stencil_flags, cells);
@@ -320,7 +320,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
                 smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
         }
         for (float* p : {c.density_source, c.temperature_source, c.occupancy_float, c.vorticity_magnitude, c.velocity_magnitude, c.divergence,
-                         c.pressure, c.pressure_rhs, c.pcg_r, c.pcg_p[0], c.pcg_p[1], c.pcg_e})
+                         c.pressure, c.pressure_rhs, c.pcg_r, c.pcg_p, c.pcg_e})
             smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
         smoke::enqueue_fill(c, c.density, desc->initial_density, c.grid.cells);
         smoke::enqueue_fill(c, c.density_tmp, desc->initial_density, c.grid.cells);

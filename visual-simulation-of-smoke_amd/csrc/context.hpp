@@ -36,10 +36,9 @@ struct PcgScalars {
     int iteration;      // inner iterations performed in this solve
     int done;           // inner solve finished (tolerance reached or cap)
     int max_iterations;
-    int first;          // next pass A starts a fresh Krylov space (beta = 0)
+    int first;          // next direction pass starts a fresh Krylov space (beta = 0)
     int fixed_mode;     // reference semantics: no early exit
     int jacobi;
-    unsigned ticket[4]; // last-block counters of the reducing kernels
 };
 
 struct PcgStatus {      // what the host polls between chunks
@@ -87,14 +86,14 @@ struct Context {
 
     // pressure solve workspace
     float* pcg_r = nullptr;
-    float* pcg_p[2] = {};
+    float* pcg_p = nullptr;            // search direction, updated in place
     float* pcg_e = nullptr;
     double* x64[2] = {};
     int x64_current = 0;
-    int p_current = 0;
     PcgScalars* scalars = nullptr;
     double* partials = nullptr;
     size_t partial_capacity = 0;
+    int pcg_stencil_blocks = 0;          // blocks of the last stencil-pass launch (partials to sum)
     PcgStatus* host_status = nullptr;   // pinned, two slots
     hipEvent_t status_event[2] = {};
 
@@ -109,13 +108,9 @@ struct Context {
     hipGraphExec_t graph_exec = nullptr;
     bool diagnostics_dirty = true;
 
-    // PCG launch shape
-    int tile_x = 64, tile_y = 8, zchunk = 16;
-    int pcg_blocks = 0;
-
     bool timing = false;
     SmokeSimulationKernelTiming timing_totals{};
-    hipEvent_t timing_events[2][3] = {};
+    hipEvent_t timing_events[2][6] = {};
 };
 
 // Launch helpers implemented in the .hip translation units.

@@ -7,18 +7,22 @@
 // B:1343-1412 exactly: the anchor keeps its -1 coupling here, which is exact
 // because the anchor entry of every iterate is 0.
 //
-// One PCG iteration is two fused passes (34 bytes of compulsory HBM traffic
-// per cell):
-//   pass A: p <- z + beta p (z = r / diag on the fly), Ap, partial p.Ap
-//           reads r, p_old, flags; writes p_new                   (13 B/cell)
-//   pass B: Ap recomputed from p_new, e += alpha p, r -= alpha Ap,
-//           partial r.z and r.r
-//           reads p_new, flags, e, r; writes e, r                  (21 B/cell)
-// Both march along z with the current plane (plus an x/y halo ring) in LDS and
-// the z neighbours in registers.  Reductions are FP64: wave64 shuffles, LDS
-// across waves, per-block partials, and the last block to finish sums the
-// partials in block order (deterministic) and updates alpha / beta on the
-// device, so fixed-iteration solves never return to the host.
+// One PCG iteration is three passes (39 bytes of compulsory HBM traffic per
+// cell):
+//   pass 1: p <- z + beta p in place (z = r / diag on the fly)
+//           reads r, p, flags; writes p                            (13 B/cell)
+//   pass 2: Ap from p, partial p.Ap                        reads p, flags (5 B/cell)
+//   pass 3: Ap recomputed, e += alpha p, r -= alpha Ap, partial r.z and r.r
+//                      reads p, flags, e, r; writes e, r           (21 B/cell)
+// Pass 1 is a flat float4 stream.  Passes 2 and 3 give each thread four
+// consecutive x cells (one float4) on two z planes and read every neighbour
+// straight through the cache: with no LDS staging and no barriers all loads
+// of a thread are independent and in flight together, which keeps the HBM
+// pipe full where a z-marching LDS pass stalls on every plane.  Reductions
+// are FP64: wave64 shuffles, LDS across waves, per-block partials, and a
+// one-block finalize kernel after passes 2 and 3 sums the partials in block
+// order (deterministic) and updates alpha / beta on the device, so
+// fixed-iteration solves never return to the host.
 #include "context.hpp"
 
 #include <algorithm>
@@ -28,14 +32,15 @@ namespace smoke {
 namespace {
 
 constexpr uint8_t kIdentity = 0x40;
+constexpr int kResidualPlanes = 8;   // z planes per block of the FP64 residual kernel
 
 struct PcgArgs {
     int n[3];
     int periodic[3];
     int outflow[6];
     unsigned skip;        // off-diagonal directions ignored (periodic axes with n <= 2)
-    int zchunk;
-    int tiles[3];         // tile grid (x tiles, y tiles, z chunks)
+    int any_outflow;
+    int tiles[3];         // tile grid of a stencil pass (x tiles, y tiles, z runs)
     const uint8_t* flags;
     PcgScalars* s;
     double* partials;
@@ -104,39 +109,35 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (&red)[NV][NWA
     }
 }
 
-// Writes this block's partials, and returns true in the one block that
-// finishes last, with `tot` holding the sums over all blocks in block order.
+// Writes this block's sums to partials[q * nblocks + bid].  The partials are
+// summed by a one-block finalize kernel launched next on the same stream: a
+// kernel boundary makes them visible across the XCDs' L2 caches for the
+// price of one launch, where a last-block-done ticket would need an
+// agent-scope release fence (an L2 writeback) in every block.
 template <int NV, int NWAVES>
-__device__ bool last_block_total(double (&v)[NV], double (&tot)[NV], double* partials, unsigned* ticket, int nblocks, int bid) {
+__device__ __forceinline__ void store_block_partials(double (&v)[NV], double* partials, int nblocks, int bid) {
     __shared__ double red[NV][NWAVES];
-    __shared__ int is_last;
     block_sum<NV, NWAVES>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) partials[q * nblocks + bid] = v[q];
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        is_last = t == unsigned(nblocks - 1);
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[NV];
+}
+
+// Sums the partials of nblocks blocks in a fixed order (deterministic); the
+// totals are valid in thread 0 of the 256-thread finalize block.
+constexpr int kFinalizeThreads = 256;
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nblocks, double (&tot)[NV]) {
+    __shared__ double red[NV][kFinalizeThreads / 64];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += NWAVES * 64)
-            s += __hip_atomic_load(&partials[q * nblocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        acc[q] = s;
+        for (int b = threadIdx.x; b < nblocks; b += kFinalizeThreads) s += partials[q * nblocks + b];
+        tot[q] = s;
     }
-    __syncthreads();
-    block_sum<NV, NWAVES>(acc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) tot[q] = acc[q];
-        *ticket = 0u;
-    }
-    return threadIdx.x == 0;
+    block_sum<NV, kFinalizeThreads / 64>(tot, red);
 }
 
 __device__ __forceinline__ double guarded(double num, double den, bool fixed) {
@@ -144,127 +145,242 @@ __device__ __forceinline__ double guarded(double num, double den, bool fixed) {
     return fabs(den) > lim ? num / den : 0.0;
 }
 
-// One z-marching pass over a TX x TY x zchunk tile.
-template <int TX, int TY, bool PASS_A, bool JACOBI>
-__global__ __launch_bounds__(TX* TY) void pcg_pass_kernel(PcgArgs a, float* __restrict__ r, const float* __restrict__ p_in,
-                                                           float* __restrict__ p_out, float* __restrict__ e) {
-    PcgScalars* __restrict__ S = a.s;
+// Coordinates of a possibly out-of-range cell made safe to load from:
+// periodic axes wrap, others clamp; `valid` tells whether the cell exists.
+__device__ __forceinline__ int64_t safe_index(const PcgArgs& a, int x, int y, int z, bool& valid) {
+    bool ok = true;
+    if (x < 0 || x >= a.n[0]) {
+        ok = ok && a.periodic[0];
+        x = a.periodic[0] ? wrapi(x, a.n[0]) : clampi(x, 0, a.n[0] - 1);
+    }
+    if (y < 0 || y >= a.n[1]) {
+        ok = ok && a.periodic[1];
+        y = a.periodic[1] ? wrapi(y, a.n[1]) : clampi(y, 0, a.n[1] - 1);
+    }
+    if (z < 0 || z >= a.n[2]) {
+        ok = ok && a.periodic[2];
+        z = a.periodic[2] ? wrapi(z, a.n[2]) : clampi(z, 0, a.n[2] - 1);
+    }
+    valid = ok;
+    return lin(a, x, y, z);
+}
+
+// Tile of this block for the stencil passes: z fastest, consecutive tiles
+// kept on one XCD (blocks are dealt round-robin over the 8 XCDs), so that
+// the neighbours a tile reads through the cache were just loaded by the
+// tiles beside it in the same L2.
+__device__ __forceinline__ void stencil_tile(const PcgArgs& a, int& bx, int& by, int& bz) {
+    const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
+    const int b = tile_id(nb);
+    bz = b % a.tiles[2];
+    by = (b / a.tiles[2]) % a.tiles[1];
+    bx = b / (a.tiles[2] * a.tiles[1]);
+}
+
+// Pass 1 of an iteration: p <- z + beta p in place, z = r / diag (Jacobi)
+// or r; a fresh Krylov space (first) takes p = z.  Elementwise, 4 cells per
+// thread: reads r, p, flags, writes p (13 B/cell).
+template <bool JACOBI>
+__global__ __launch_bounds__(256) void pcg_update_p_kernel(PcgArgs a, const float* __restrict__ r, float* __restrict__ p) {
+    const PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
-    constexpr int NWAVES = TX * TY / 64;
-    const int nblocks = a.tiles[0] * a.tiles[1] * a.tiles[2];
-    const int tile = tile_id(nblocks);
-    const int bx = tile % a.tiles[0];
-    const int by = (tile / a.tiles[0]) % a.tiles[1];
-    const int bz = tile / (a.tiles[0] * a.tiles[1]);
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x0 = bx * TX, y0 = by * TY;
-    const int x = x0 + tx, y = y0 + ty;
-    const int z0 = bz * a.zchunk;
-    const int z1 = min(z0 + a.zchunk, a.n[2]);
-    const bool owner = x < a.n[0] && y < a.n[1];
     const bool first = S->first != 0;
     const float beta = first ? 0.0f : float(S->beta);
-    const float alpha = float(S->alpha);
-
-    // Operand of the stencil at a cell: pass A builds p_new = z + beta p_old,
-    // pass B reads p_new.
-    auto operand = [&](int cx, int cy, int cz) -> float {
-        if (!resolve(a, cx, cy, cz)) return 0.0f;
-        const int64_t i = lin(a, cx, cy, cz);
-        if constexpr (PASS_A) {
-            const float rv = r[i];
-            float zv = rv;
-            if constexpr (JACOBI) zv = rv / diag_of(a, a.flags[i], cx, cy, cz);
-            return first ? zv : zv + beta * p_in[i];
-        } else {
-            return p_in[i];
+    const int64_t cells = int64_t(a.n[0]) * a.n[1] * a.n[2];
+    const int64_t base = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    if (base >= cells) return;
+    auto zval = [&](float rv, uint8_t f, int64_t i) {
+        if constexpr (!JACOBI) return rv;
+        int x = 0, y = 0, z = 0;
+        if (a.any_outflow) {
+            x = int(i % a.n[0]);
+            y = int((i / a.n[0]) % a.n[1]);
+            z = int(i / (int64_t(a.n[0]) * a.n[1]));
         }
+        return rv / diag_of(a, f, x, y, z);
+    };
+    if (base + 4 <= cells) {
+        const float4 rv = *reinterpret_cast<const float4*>(r + base);
+        const float4 pv = *reinterpret_cast<const float4*>(p + base);
+        const uchar4 f = *reinterpret_cast<const uchar4*>(a.flags + base);
+        float4 o;
+        o.x = zval(rv.x, f.x, base) + beta * (first ? 0.0f : pv.x);
+        o.y = zval(rv.y, f.y, base + 1) + beta * (first ? 0.0f : pv.y);
+        o.z = zval(rv.z, f.z, base + 2) + beta * (first ? 0.0f : pv.z);
+        o.w = zval(rv.w, f.w, base + 3) + beta * (first ? 0.0f : pv.w);
+        *reinterpret_cast<float4*>(p + base) = o;
+    } else {
+        for (int64_t i = base; i < cells; ++i) p[i] = zval(r[i], a.flags[i], i) + beta * (first ? 0.0f : p[i]);
+    }
+}
+
+// Stencil passes on the stored direction p, VEC cells per thread along x
+// (float4 when nx % 4 == 0), ZR planes per thread, neighbours read through
+// the cache (no LDS, no barriers: every load of a thread is independent):
+//   MODE 0 (pass 2): partial p.Ap; the last block sets alpha = r.z / p.Ap.
+//                    Reads p, flags (5 B/cell).
+//   MODE 1 (pass 3): e += alpha p, r -= alpha Ap, partials r.z and r.r; the
+//                    last block sets beta, counts the iteration and raises
+//                    `done`.  Reads p, flags, e, r; writes e, r (21 B/cell).
+template <int VEC>
+struct Row {
+    float v[VEC];
+};
+
+template <int VEC>
+__device__ __forceinline__ Row<VEC> load_row(const float* __restrict__ f, int64_t i) {
+    Row<VEC> o;
+    if constexpr (VEC == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(f + i);
+        o.v[0] = q.x;
+        o.v[1] = q.y;
+        o.v[2] = q.z;
+        o.v[3] = q.w;
+    } else {
+        o.v[0] = f[i];
+    }
+    return o;
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_row(float* __restrict__ f, int64_t i, const Row<VEC>& o) {
+    if constexpr (VEC == 4) *reinterpret_cast<float4*>(f + i) = make_float4(o.v[0], o.v[1], o.v[2], o.v[3]);
+    else f[i] = o.v[0];
+}
+
+template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
+__global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r) {
+    PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    constexpr int NWAVES = TXV * TY / 64;
+    const int nblocks = a.tiles[0] * a.tiles[1] * a.tiles[2];
+    int bx, by, bz;
+    stencil_tile(a, bx, by, bz);
+    const int xv = (bx * TXV + int(threadIdx.x) % TXV) * VEC;
+    const int y = by * TY + int(threadIdx.x) / TXV;
+    const int z0 = bz * ZR;
+    const bool owner = xv < a.n[0] && y < a.n[1];
+    const int xs = owner ? xv : 0;   // safe x for padding threads
+    const bool first = S->first != 0;
+    const float alpha = float(S->alpha);
+    const uint8_t* __restrict__ flags = a.flags;
+
+    // a row segment of p at (xs.., cy, cz): zero when the row does not exist
+    auto prow = [&](int cy, int cz) {
+        bool ok;
+        const int64_t i = safe_index(a, xs, cy, cz, ok);
+        Row<VEC> q = load_row<VEC>(p, i);
+        if (!ok) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) q.v[k] = 0.0f;
+        }
+        return q;
+    };
+    auto pcell = [&](int cx, int cy, int cz) {
+        bool ok;
+        const float v = p[safe_index(a, cx, cy, cz, ok)];
+        return ok ? v : 0.0f;
     };
 
-    // halo ring slot of this thread (x halo first, then y halo)
-    const int q = threadIdx.x;
-    int ring_x = 0, ring_y = 0, ring_lx = 0, ring_ly = 0;
-    const bool ring = q < 2 * TY + 2 * TX;
-    if (q < 2 * TY) {
-        const int side = q / TY, row = q % TY;
-        ring_x = side ? x0 + TX : x0 - 1;
-        ring_y = y0 + row;
-        ring_lx = side ? TX + 1 : 0;
-        ring_ly = row + 1;
-    } else if (ring) {
-        const int qq = q - 2 * TY, side = qq / TX, col = qq % TX;
-        ring_x = x0 + col;
-        ring_y = side ? y0 + TY : y0 - 1;
-        ring_lx = col + 1;
-        ring_ly = side ? TY + 1 : 0;
-    }
-
-    __shared__ float plane[TY + 2][TX + 2];
-    float vm = operand(x, y, z0 - 1);
-    float vc = operand(x, y, z0);
-    float rc = ring ? operand(ring_x, ring_y, z0) : 0.0f;
+    Row<VEC> col[ZR + 2];
+#pragma unroll
+    for (int j = 0; j < ZR + 2; ++j) col[j] = prow(y, z0 - 1 + j);
     double acc0 = 0.0, acc1 = 0.0;
-
-    for (int z = z0; z < z1; ++z) {
-        const float vp = operand(x, y, z + 1);
-        const float rp = (ring && z + 1 < z1) ? operand(ring_x, ring_y, z + 1) : 0.0f;
-        plane[ty + 1][tx + 1] = vc;
-        if (ring) plane[ring_ly][ring_lx] = rc;
-        __syncthreads();
-        if (owner) {
-            const int64_t i = lin(a, x, y, z);
-            const uint8_t f = a.flags[i];
-            float ap;
-            float d = 1.0f;
-            if (f & kIdentity) {
-                ap = vc;
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int z = z0 + j;
+        const Row<VEC> ym = prow(y - 1, z), yp = prow(y + 1, z);
+        const float lft = pcell(xs - 1, y, z), rgt = pcell(xs + VEC, y, z);
+        bool ok;
+        const int64_t i = safe_index(a, xs, y, z, ok);
+        uint8_t fv[VEC];
+        if constexpr (VEC == 4) {
+            const uchar4 f4 = *reinterpret_cast<const uchar4*>(flags + i);
+            fv[0] = f4.x;
+            fv[1] = f4.y;
+            fv[2] = f4.z;
+            fv[3] = f4.w;
+        } else {
+            fv[0] = flags[i];
+        }
+        Row<VEC> ev{}, rv{};
+        if constexpr (MODE == 1) {
+            ev = load_row<VEC>(e, i);
+            rv = load_row<VEC>(r, i);
+        }
+        const bool active = owner && z < a.n[2];
+        Row<VEC> en, rn;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            // identity rows: d = 1 and no off-diagonal bits, so A p = p
+            const bool ident = (fv[k] & kIdentity) != 0;
+            const float d = ident ? 1.0f : diag_of(a, fv[k], xs + k, y, z);
+            const unsigned m = ident ? 0u : (unsigned(fv[k]) & ~a.skip);
+            const float c = col[j + 1].v[k];
+            const float xm = k == 0 ? lft : col[j + 1].v[k - 1 < 0 ? 0 : k - 1];
+            const float xp = k == VEC - 1 ? rgt : col[j + 1].v[k + 1 < VEC ? k + 1 : VEC - 1];
+            float ap = d * c;
+            ap -= (m & 1u) ? xm : 0.0f;
+            ap -= (m & 2u) ? xp : 0.0f;
+            ap -= (m & 4u) ? ym.v[k] : 0.0f;
+            ap -= (m & 8u) ? yp.v[k] : 0.0f;
+            ap -= (m & 16u) ? col[j].v[k] : 0.0f;
+            ap -= (m & 32u) ? col[j + 2].v[k] : 0.0f;
+            if constexpr (MODE == 0) {
+                s0 += double(c) * double(ap);
             } else {
-                d = diag_of(a, f, x, y, z);
-                const unsigned m = unsigned(f) & ~a.skip;
-                float s = d * vc;
-                s -= (m & 1u) ? plane[ty + 1][tx] : 0.0f;
-                s -= (m & 2u) ? plane[ty + 1][tx + 2] : 0.0f;
-                s -= (m & 4u) ? plane[ty][tx + 1] : 0.0f;
-                s -= (m & 8u) ? plane[ty + 2][tx + 1] : 0.0f;
-                s -= (m & 16u) ? vm : 0.0f;
-                s -= (m & 32u) ? vp : 0.0f;
-                ap = s;
-            }
-            if constexpr (PASS_A) {
-                p_out[i] = vc;
-                acc0 += double(vc) * double(ap);
-            } else {
-                e[i] = (first ? 0.0f : e[i]) + alpha * vc;
-                const float rn = r[i] - alpha * ap;
-                r[i] = rn;
-                const float zn = JACOBI ? rn / d : rn;
-                acc0 += double(rn) * double(zn);
-                acc1 += double(rn) * double(rn);
+                en.v[k] = (first ? 0.0f : ev.v[k]) + alpha * c;
+                rn.v[k] = rv.v[k] - alpha * ap;
+                const float zk = JACOBI ? rn.v[k] / d : rn.v[k];
+                s0 += double(rn.v[k]) * double(zk);
+                s1 += double(rn.v[k]) * double(rn.v[k]);
             }
         }
-        __syncthreads();
-        vm = vc;
-        vc = vp;
-        rc = rp;
+        if constexpr (MODE == 1) {
+            if (active) {
+                store_row<VEC>(e, i, en);
+                store_row<VEC>(r, i, rn);
+            }
+        }
+        acc0 += active ? s0 : 0.0;
+        acc1 += active ? s1 : 0.0;
     }
 
-    if constexpr (PASS_A) {
-        double v[1] = {acc0}, tot[1];
-        if (last_block_total<1, NWAVES>(v, tot, a.partials, &S->ticket[0], nblocks, tile)) {
-            S->alpha = guarded(S->rz, tot[0], S->fixed_mode != 0);
-        }
+    if constexpr (MODE == 0) {
+        double v[1] = {acc0};
+        store_block_partials<1, NWAVES>(v, a.partials, nblocks, blockIdx.x);
     } else {
-        double v[2] = {acc0, acc1}, tot[2];
-        if (last_block_total<2, NWAVES>(v, tot, a.partials, &S->ticket[1], nblocks, tile)) {
-            const double rz_new = tot[0];
-            S->beta = guarded(rz_new, S->rz, S->fixed_mode != 0);
-            S->rz = rz_new;
-            S->rr = tot[1];
-            S->first = 0;
-            const int it = S->iteration + 1;
-            S->iteration = it;
-            if (!S->fixed_mode && (tot[1] <= S->stop2 || it >= S->max_iterations)) S->done = 1;
-        }
+        double v[2] = {acc0, acc1};
+        store_block_partials<2, NWAVES>(v, a.partials, nblocks, blockIdx.x);
+    }
+}
+
+// alpha = r.z / p.Ap from the partials of pass 2.
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_kernel(PcgArgs a, int nblocks) {
+    PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    double tot[1];
+    sum_partials<1>(a.partials, nblocks, tot);
+    if (threadIdx.x == 0) S->alpha = guarded(S->rz, tot[0], S->fixed_mode != 0);
+}
+
+// beta = r.z_new / r.z_old, iteration count and stop test from the partials
+// of pass 3.
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, int nblocks) {
+    PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    double tot[2];
+    sum_partials<2>(a.partials, nblocks, tot);
+    if (threadIdx.x == 0) {
+        const double rz_new = tot[0];
+        S->beta = guarded(rz_new, S->rz, S->fixed_mode != 0);
+        S->rz = rz_new;
+        S->rr = tot[1];
+        S->first = 0;
+        const int it = S->iteration + 1;
+        S->iteration = it;
+        if (!S->fixed_mode && (tot[1] <= S->stop2 || it >= S->max_iterations)) S->done = 1;
     }
 }
 
@@ -272,14 +388,12 @@ __global__ __launch_bounds__(TX* TY) void pcg_pass_kernel(PcgArgs a, float* __re
 // FP64; optionally stores x' (FP64) and starts a fresh inner solve.
 __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* __restrict__ b, const double* __restrict__ x_old,
                                                        const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
-                                                       int compute_bb, int reset_iterations) {
-    PcgScalars* __restrict__ S = a.s;
+                                                       const PcgScalars* __restrict__ S) {
     const int nblocks = gridDim.x * gridDim.y * gridDim.z;
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    // 256 threads = 64 (x) x 4 (y) cells of one z plane
+    // 256 threads = 64 (x) x 4 (y) columns, kResidualPlanes z planes per block
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
     double rr = 0.0, rz = 0.0, bb = 0.0;
     auto xv = [&](int cx, int cy, int cz) -> double {
         if (!resolve(a, cx, cy, cz)) return 0.0;
@@ -288,7 +402,9 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* _
         if (e_add) s += double(e_add[i]);
         return s;
     };
-    if (x < a.n[0] && y < a.n[1]) {
+    const int zb = blockIdx.z * kResidualPlanes;
+    const int ze = min(zb + kResidualPlanes, a.n[2]);
+    for (int z = zb; z < ze && x < a.n[0] && y < a.n[1]; ++z) {
         const int64_t i = lin(a, x, y, z);
         const uint8_t f = a.flags[i];
         const double xc = xv(x, y, z);
@@ -313,12 +429,21 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* _
         const float rf = float(r64);
         r[i] = rf;
         const float zf = S->jacobi ? rf / d : rf;
-        rr = r64 * r64;
-        rz = double(rf) * double(zf);
-        bb = bv * bv;
+        rr += r64 * r64;
+        rz += double(rf) * double(zf);
+        bb += bv * bv;
     }
-    double v[3] = {rr, rz, bb}, tot[3];
-    if (last_block_total<3, 4>(v, tot, a.partials, &S->ticket[2], nblocks, bid)) {
+    double v[3] = {rr, rz, bb};
+    store_block_partials<3, 4>(v, a.partials, nblocks, bid);
+}
+
+// Totals of the residual kernel: true ||r||^2, r.z, ||b||^2; starts a fresh
+// inner solve (p = z at the next iteration) with its stop threshold.
+__global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(PcgArgs a, int nblocks, int compute_bb, int reset_iterations) {
+    PcgScalars* __restrict__ S = a.s;
+    double tot[3];
+    sum_partials<3>(a.partials, nblocks, tot);
+    if (threadIdx.x == 0) {
         if (compute_bb) S->bb = tot[2];
         S->rr_true = tot[0];
         S->rz = tot[1];
@@ -340,8 +465,8 @@ __global__ void pressure_from_x64_kernel(const double* __restrict__ x, float* __
 }
 
 __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // b = -(h^2/dt) div(u*) + sum of OUTFLOW face pressures, 0 on identity rows (B:1304-1328).
@@ -377,6 +502,7 @@ PcgArgs make_args(const Context& c) {
         a.periodic[k] = c.grid.flow_periodic[k];
     }
     for (int f = 0; f < 6; ++f) a.outflow[f] = c.grid.flow_type[f] == kOutflow;
+    a.any_outflow = a.outflow[0] | a.outflow[1] | a.outflow[2] | a.outflow[3] | a.outflow[4] | a.outflow[5];
     unsigned skip = 0;
     for (int k = 0; k < 3; ++k) {
         if (!a.periodic[k]) continue;
@@ -384,60 +510,81 @@ PcgArgs make_args(const Context& c) {
         else if (a.n[k] == 2) skip |= 2u << (2 * k);   // both neighbours are the same cell: one -1
     }
     a.skip = skip;
-    a.zchunk = c.zchunk;
-    a.tiles[0] = (c.grid.n[0] + c.tile_x - 1) / c.tile_x;
-    a.tiles[1] = (c.grid.n[1] + c.tile_y - 1) / c.tile_y;
-    a.tiles[2] = (c.grid.n[2] + c.zchunk - 1) / c.zchunk;
     a.flags = c.stencil_flags;
     a.s = c.scalars;
     a.partials = c.partials;
     return a;
 }
 
-template <int TX, int TY>
-void launch_pass(Context& c, bool pass_a) {
-    const PcgArgs a = make_args(c);
+// Launch shape of the stencil passes: 64 x 8 threads x 2 planes (256 x 8
+// cells per tile plane with float4), or 16 x 16 threads on narrow grids.
+template <int TXV, int TY, int ZR, int VEC, bool JACOBI>
+void launch_stencils(Context& c, PcgArgs a, int mode) {
+    a.tiles[0] = (c.grid.n[0] + TXV * VEC - 1) / (TXV * VEC);
+    a.tiles[1] = (c.grid.n[1] + TY - 1) / TY;
+    a.tiles[2] = (c.grid.n[2] + ZR - 1) / ZR;
     const unsigned blocks = unsigned(a.tiles[0] * a.tiles[1] * a.tiles[2]);
-    const float* p_old = c.pcg_p[c.p_current];
-    float* p_new = c.pcg_p[c.p_current ^ 1];
-    const bool jac = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE && c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
     float* e = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE ? c.pcg_e : c.pressure;
-    if (pass_a) {
-        if (jac) pcg_pass_kernel<TX, TY, true, true><<<blocks, TX * TY, 0, c.stream>>>(a, c.pcg_r, p_old, p_new, e);
-        else pcg_pass_kernel<TX, TY, true, false><<<blocks, TX * TY, 0, c.stream>>>(a, c.pcg_r, p_old, p_new, e);
-    } else {
-        if (jac) pcg_pass_kernel<TX, TY, false, true><<<blocks, TX * TY, 0, c.stream>>>(a, c.pcg_r, p_new, nullptr, e);
-        else pcg_pass_kernel<TX, TY, false, false><<<blocks, TX * TY, 0, c.stream>>>(a, c.pcg_r, p_new, nullptr, e);
-    }
+    if (mode == 0) pcg_stencil_kernel<TXV, TY, ZR, VEC, 0, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r);
+    else pcg_stencil_kernel<TXV, TY, ZR, VEC, 1, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r);
+    hip_check(hipGetLastError(), "pcg_stencil_kernel");
+    c.pcg_stencil_blocks = int(blocks);
 }
 
-void pass(Context& c, bool pass_a) {
-    if (c.tile_y == 8) launch_pass<64, 8>(c, pass_a);
-    else launch_pass<64, 4>(c, pass_a);
-    hip_check(hipGetLastError(), "pcg pass");
+template <bool JACOBI>
+void stencil_pass(Context& c, int mode) {
+    const PcgArgs a = make_args(c);
+    const bool vec = c.grid.n[0] % 4 == 0;
+    if (vec && c.grid.n[0] >= 128) launch_stencils<64, 8, 2, 4, JACOBI>(c, a, mode);
+    else if (vec) launch_stencils<16, 16, 2, 4, JACOBI>(c, a, mode);
+    else launch_stencils<64, 4, 2, 1, JACOBI>(c, a, mode);
+}
+
+bool jacobi_on(const Context& c) {
+    return c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE && c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
+}
+
+// pass 1: p <- z + beta p
+void pass_update_p(Context& c) {
+    const PcgArgs a = make_args(c);
+    const unsigned blocks = unsigned((c.grid.cells + 1023) / 1024);
+    if (jacobi_on(c)) pcg_update_p_kernel<true><<<blocks, 256, 0, c.stream>>>(a, c.pcg_r, c.pcg_p);
+    else pcg_update_p_kernel<false><<<blocks, 256, 0, c.stream>>>(a, c.pcg_r, c.pcg_p);
+    hip_check(hipGetLastError(), "pcg_update_p_kernel");
+}
+
+// passes 2 (mode 0: p.Ap) and 3 (mode 1: e, r update, r.z, r.r)
+void pass_stencil(Context& c, int mode) {
+    if (jacobi_on(c)) stencil_pass<true>(c, mode);
+    else stencil_pass<false>(c, mode);
+}
+
+// alpha after pass 2 (mode 0), beta and the stop test after pass 3 (mode 1)
+void pass_finalize(Context& c, int mode) {
+    const PcgArgs a = make_args(c);
+    if (mode == 0) pcg_alpha_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.pcg_stencil_blocks);
+    else pcg_beta_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.pcg_stencil_blocks);
+    hip_check(hipGetLastError(), "pcg finalize kernel");
 }
 
 void launch_residual(Context& c, const double* x_old, const float* e_add, double* x_new, bool bb, bool reset) {
     const PcgArgs a = make_args(c);
-    const dim3 grid((c.grid.n[0] + 63) / 64, (c.grid.n[1] + 3) / 4, c.grid.n[2]);
-    residual_kernel<<<grid, 256, 0, c.stream>>>(a, c.pressure_rhs, x_old, e_add, x_new, c.pcg_r, bb ? 1 : 0, reset ? 1 : 0);
+    const dim3 grid((c.grid.n[0] + 63) / 64, (c.grid.n[1] + 3) / 4, (c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
+    residual_kernel<<<grid, 256, 0, c.stream>>>(a, c.pressure_rhs, x_old, e_add, x_new, c.pcg_r, c.scalars);
     hip_check(hipGetLastError(), "residual_kernel");
+    residual_finalize_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, int(grid.x * grid.y * grid.z), bb ? 1 : 0, reset ? 1 : 0);
+    hip_check(hipGetLastError(), "residual_finalize_kernel");
 }
 
 }  // namespace
 
 void configure_pcg(Context& c) {
-    const int64_t plane_tiles = int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 7) / 8);
-    c.tile_x = 64;
-    c.tile_y = c.grid.n[1] >= 8 ? 8 : 4;
-    // enough tiles to fill 256 CUs several times over, z chunks of 4..32 planes
-    int zc = int(std::max<int64_t>(4, std::min<int64_t>(32, plane_tiles * c.grid.n[2] / 2048)));
-    c.zchunk = std::min(zc, c.grid.n[2]);
-    const int64_t tiles = int64_t((c.grid.n[0] + c.tile_x - 1) / c.tile_x) * ((c.grid.n[1] + c.tile_y - 1) / c.tile_y) *
-                          ((c.grid.n[2] + c.zchunk - 1) / c.zchunk);
-    const int64_t residual_blocks = int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 3) / 4) * c.grid.n[2];
-    c.pcg_blocks = int(tiles);
-    c.partial_capacity = size_t(std::max<int64_t>(2 * tiles, 3 * residual_blocks));
+    // upper bound of the stencil-pass block count over the launch shapes
+    const int64_t tiles_max = int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 3) / 4) * ((c.grid.n[2] + 1) / 2);
+    const int64_t residual_blocks =
+        int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 3) / 4) * ((c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
+    c.partial_capacity = size_t(std::max<int64_t>(2 * tiles_max, 3 * residual_blocks));
+    c.pcg_stencil_blocks = int(tiles_max);
 }
 
 void enqueue_pressure_rhs(Context& c) {
@@ -452,9 +599,11 @@ void enqueue_pcg_init(Context& c, bool use_x64) {
 }
 
 void enqueue_pcg_iteration(Context& c) {
-    pass(c, true);
-    pass(c, false);
-    c.p_current ^= 1;
+    pass_update_p(c);
+    pass_stencil(c, 0);
+    pass_finalize(c, 0);
+    pass_stencil(c, 1);
+    pass_finalize(c, 1);
 }
 
 void enqueue_pcg_refine(Context& c) {
@@ -509,12 +658,18 @@ void run_tolerance_pcg(Context& c) {
         auto enqueue_chunk = [&](int s) {
             for (int k = 0; k < chunk; ++k) {
                 if (c.timing && k == 0) {
-                    hip_check(hipEventRecord(c.timing_events[s][0], c.stream), "event");
-                    pass(c, true);
-                    hip_check(hipEventRecord(c.timing_events[s][1], c.stream), "event");
-                    pass(c, false);
-                    hip_check(hipEventRecord(c.timing_events[s][2], c.stream), "event");
-                    c.p_current ^= 1;
+                    hipEvent_t* ev = c.timing_events[s];
+                    hip_check(hipEventRecord(ev[0], c.stream), "event");
+                    pass_update_p(c);
+                    hip_check(hipEventRecord(ev[1], c.stream), "event");
+                    pass_stencil(c, 0);
+                    hip_check(hipEventRecord(ev[2], c.stream), "event");
+                    pass_finalize(c, 0);
+                    hip_check(hipEventRecord(ev[3], c.stream), "event");
+                    pass_stencil(c, 1);
+                    hip_check(hipEventRecord(ev[4], c.stream), "event");
+                    pass_finalize(c, 1);
+                    hip_check(hipEventRecord(ev[5], c.stream), "event");
                 } else {
                     enqueue_pcg_iteration(c);
                 }
@@ -528,13 +683,14 @@ void run_tolerance_pcg(Context& c) {
             hip_check(hipEventSynchronize(c.status_event[slot]), "status wait");
             const PcgStatus st = c.host_status[slot];
             if (c.timing && st.iteration > observed) {
-                float ma = 0.f, mb = 0.f;
-                hip_check(hipEventElapsedTime(&ma, c.timing_events[slot][0], c.timing_events[slot][1]), "elapsed");
-                hip_check(hipEventElapsedTime(&mb, c.timing_events[slot][1], c.timing_events[slot][2]), "elapsed");
-                c.timing_totals.pcg_pass_a_ms += ma;
-                c.timing_totals.pcg_pass_b_ms += mb;
-                c.timing_totals.pcg_pass_a_launches += 1;
-                c.timing_totals.pcg_pass_b_launches += 1;
+                float m[5] = {};
+                for (int q = 0; q < 5; ++q)
+                    hip_check(hipEventElapsedTime(&m[q], c.timing_events[slot][q], c.timing_events[slot][q + 1]), "elapsed");
+                c.timing_totals.pcg_direction_ms += m[0];
+                c.timing_totals.pcg_curvature_ms += m[1];
+                c.timing_totals.pcg_update_ms += m[3];
+                c.timing_totals.pcg_reduction_ms += m[2] + m[4];
+                c.timing_totals.pcg_timed_iterations += 1;
             }
             if (st.iteration > observed) c.timing_totals.pcg_iterations += uint64_t(st.iteration - observed);
             observed = st.iteration > observed ? st.iteration : observed;
@@ -571,10 +727,10 @@ double measure_copy(uint64_t bytes, int reps, double* ms_per_copy) {
     hipEvent_t t0, t1;
     hip_check(hipEventCreate(&t0), "event");
     hip_check(hipEventCreate(&t1), "event");
-    const unsigned blocks = 256u * 8u;
+    const unsigned blocks = unsigned((n4 + 255) / 256);
     copy_kernel<<<blocks, 256, 0, s>>>(a, b, n4);   // warm-up
     hip_check(hipEventRecord(t0, s), "event");
-    for (int r = 0; r < reps; ++r) copy_kernel<<<blocks, 256, 0, s>>>(r & 1 ? b : a, r & 1 ? a : b, n4);
+    for (int r = 0; r < reps; ++r) copy_kernel<<<blocks, 256, 0, s>>>(a, b, n4);
     hip_check(hipEventRecord(t1, s), "event");
     hip_check(hipEventSynchronize(t1), "event sync");
     float ms = 0.f;

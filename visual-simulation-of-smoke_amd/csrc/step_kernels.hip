@@ -295,10 +295,18 @@ __global__ void anchor_reset_kernel(int* anchor, int cells, unsigned* solid_coun
     *solid_count = 0u;
 }
 
-// B:1297-1302
+// B:1297-1302 (first fluid cell): block minimum first, one atomic per block
 __global__ void anchor_kernel(const uint8_t* __restrict__ occ, int* __restrict__ anchor, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n && occ[i] == 0) atomicMin(anchor, int(i));
+    int m = (i < n && occ[i] == 0) ? int(i) : 0x7fffffff;
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_down(m, off));
+    __shared__ int red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / 64; ++q) m = min(m, red[q]);
+        if (m != 0x7fffffff) atomicMin(anchor, m);
+    }
 }
 
 // Per-cell stencil flags of the Poisson operator (the structure of

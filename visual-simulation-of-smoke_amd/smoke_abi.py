@@ -103,8 +103,9 @@ class HostState(C.Structure):
 
 
 class KernelTiming(C.Structure):
-    _fields_ = [("pcg_iterations", C.c_uint64), ("pcg_pass_a_ms", C.c_double), ("pcg_pass_b_ms", C.c_double),
-                ("pcg_pass_a_launches", C.c_uint64), ("pcg_pass_b_launches", C.c_uint64),
+    _fields_ = [("pcg_iterations", C.c_uint64), ("pcg_timed_iterations", C.c_uint64),
+                ("pcg_direction_ms", C.c_double), ("pcg_curvature_ms", C.c_double), ("pcg_update_ms", C.c_double),
+                ("pcg_reduction_ms", C.c_double),
                 ("step_ms", C.c_double), ("steps", C.c_uint64)]
 
 
