@@ -100,6 +100,24 @@ __device__ __forceinline__ bool resolve_scalar(const Grid& g, const ScalarBC& bc
     return c[0] >= 0 && c[0] < g.n[0] && c[1] >= 0 && c[1] < g.n[1] && c[2] >= 0 && c[2] < g.n[2];
 }
 
+// Kernels take the Grid by value (it lives in the kernel-argument segment and
+// is read with scalar loads).  A runtime index into one of its arrays would
+// make the compiler copy the whole struct to per-thread scratch memory, so
+// per-face entries are picked with selects: `a` is a compile-time index after
+// unrolling, `side` and `comp` only choose between values.
+template <typename T>
+__device__ __forceinline__ T face_entry(const T (&v)[6], int a, int side) {
+    return side ? v[2 * a + 1] : v[2 * a];
+}
+
+__device__ __forceinline__ float pick3(const float (&v)[3], int comp) { return comp == 0 ? v[0] : (comp == 1 ? v[1] : v[2]); }
+
+__device__ __forceinline__ int pick3(const int (&v)[3], int comp) { return comp == 0 ? v[0] : (comp == 1 ? v[1] : v[2]); }
+
+__device__ __forceinline__ float face_velocity(const Grid& g, int a, int side, int comp) {
+    return side ? pick3(g.flow_velocity[2 * a + 1], comp) : pick3(g.flow_velocity[2 * a], comp);
+}
+
 // B:162-166 -- out of domain counts as solid.
 __device__ __forceinline__ bool occupied(const Grid& g, const uint8_t* occ, int x, int y, int z) {
     int c[3] = {x, y, z};
@@ -115,8 +133,8 @@ __device__ inline float load_scalar(const Grid& g, const ScalarBC& bc, const flo
         if (c[a] >= 0 && c[a] < g.n[a]) continue;
         const int side = c[a] < 0 ? 0 : 1;
         if (bc.periodic[a]) c[a] = wrapi(c[a], g.n[a]);
-        else if (bc.type[2 * a + side] == kZeroFlux) c[a] = side ? g.n[a] - 1 : 0;
-        else return bc.value[2 * a + side];
+        else if (face_entry(bc.type, a, side) == kZeroFlux) c[a] = side ? g.n[a] - 1 : 0;
+        else return face_entry(bc.value, a, side);
     }
     return f[cidx(g, c[0], c[1], c[2])];
 }
@@ -156,10 +174,10 @@ __device__ inline float load_center_component(const Grid& g, const float* __rest
 #pragma unroll
         for (int b = 0; b < 3; ++b) cl[b] = b == a ? (side ? g.n[a] - 1 : 0) : clampi(c[b], 0, g.n[b] - 1);
         const float interior = center_velocity(g, f, comp, cl[0], cl[1], cl[2]);
-        const uint32_t t = g.flow_type[2 * a + side];
+        const uint32_t t = face_entry(g.flow_type, a, side);
         if (t == kOutflow) return interior;
         if (t == kFreeSlip && comp != a) return interior;
-        return 2.0f * g.flow_velocity[2 * a + side][comp] - interior;
+        return 2.0f * face_velocity(g, a, side, comp) - interior;
     }
     return center_velocity(g, f, comp, c[0], c[1], c[2]);
 }
@@ -180,13 +198,13 @@ __device__ inline float load_face(const Grid& g, const float* __restrict__ f, in
 #pragma unroll
         for (int b = 0; b < 3; ++b) cl[b] = b == a ? (side ? hi : 0) : clampi(c[b], 0, g.n[b] - (b == comp ? 0 : 1));
         const float interior = f[fidx(g, comp, cl[0], cl[1], cl[2])];
-        const uint32_t t = g.flow_type[2 * a + side];
+        const uint32_t t = face_entry(g.flow_type, a, side);
         if (a == comp) {
             if (t == kOutflow) return interior;
         } else if (t == kOutflow || t == kFreeSlip) {
             return interior;
         }
-        return 2.0f * g.flow_velocity[2 * a + side][comp] - interior;
+        return 2.0f * face_velocity(g, a, side, comp) - interior;
     }
     return f[fidx(g, comp, c[0], c[1], c[2])];
 }
@@ -405,8 +423,8 @@ __device__ inline bool solid_face_override(const Grid& g, const uint8_t* __restr
 // Face (i,j,k) of component comp is a domain face handled by the boundary
 // pass: a non-periodic end face, or the duplicate end face n of a periodic axis.
 __device__ __forceinline__ bool boundary_face(const Grid& g, int comp, int idx) {
-    if (idx == 0) return !g.flow_periodic[comp];
-    return idx == g.n[comp];
+    if (idx == 0) return !pick3(g.flow_periodic, comp);
+    return idx == pick3(g.n, comp);
 }
 
 }  // namespace smoke
