@@ -49,9 +49,11 @@ struct PcgArgs {
 __device__ __forceinline__ float diag_of(const PcgArgs& a, uint8_t f, int x, int y, int z) {
     if (f & kIdentity) return 1.0f;
     int d = __popc(unsigned(f & 0x3f));
-    d += (x == 0 && a.outflow[0]) + (x == a.n[0] - 1 && a.outflow[1]);
-    d += (y == 0 && a.outflow[2]) + (y == a.n[1] - 1 && a.outflow[3]);
-    d += (z == 0 && a.outflow[4]) + (z == a.n[2] - 1 && a.outflow[5]);
+    if (a.any_outflow) {
+        d += (x == 0 && a.outflow[0]) + (x == a.n[0] - 1 && a.outflow[1]);
+        d += (y == 0 && a.outflow[2]) + (y == a.n[1] - 1 && a.outflow[3]);
+        d += (z == 0 && a.outflow[4]) + (z == a.n[2] - 1 && a.outflow[5]);
+    }
     return d > 0 ? float(d) : 1.0f;
 }
 
@@ -125,17 +127,28 @@ __device__ __forceinline__ void store_block_partials(double (&v)[NV], double* pa
 }
 
 // Sums the partials of nblocks blocks in a fixed order (deterministic); the
-// totals are valid in thread 0 of the 256-thread finalize block.
-constexpr int kFinalizeThreads = 256;
+// totals are valid in thread 0 of the finalize block.  Each thread keeps four
+// independent loads in flight: the partials were just written from every
+// XCD, so they come from memory and the sum is latency-bound.
+constexpr int kFinalizeThreads = 1024;
 
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nblocks, double (&tot)[NV]) {
     __shared__ double red[NV][kFinalizeThreads / 64];
+    constexpr int T = kFinalizeThreads;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += kFinalizeThreads) s += partials[q * nblocks + b];
-        tot[q] = s;
+        const double* __restrict__ row = partials + int64_t(q) * nblocks;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int b = threadIdx.x;
+        for (; b + 3 * T < nblocks; b += 4 * T) {
+            s0 += row[b];
+            s1 += row[b + T];
+            s2 += row[b + 2 * T];
+            s3 += row[b + 3 * T];
+        }
+        for (; b < nblocks; b += T) s0 += row[b];
+        tot[q] = (s0 + s1) + (s2 + s3);
     }
     block_sum<NV, kFinalizeThreads / 64>(tot, red);
 }
@@ -145,22 +158,31 @@ __device__ __forceinline__ double guarded(double num, double den, bool fixed) {
     return fabs(den) > lim ? num / den : 0.0;
 }
 
-// Coordinates of a possibly out-of-range cell made safe to load from:
-// periodic axes wrap, others clamp; `valid` tells whether the cell exists.
+// One coordinate of a neighbour made safe to load from.  Threads that own a
+// cell ask for coordinates in [-1, n]; the padding threads of a partial tile
+// may ask for more, so the coordinate is first clamped to [-1, n] (their
+// values are never used).  A periodic axis then wraps by one period and a
+// closed axis clamps; `ok` is cleared when the neighbour does not exist.
+// No integer division: this runs for every neighbour load of the stencil
+// passes.
+__device__ __forceinline__ int safe_axis(int v, int n, int periodic, bool& ok) {
+    v = v < -1 ? -1 : (v > n ? n : v);
+    if (v < 0) {
+        ok = ok && periodic;
+        return periodic ? v + n : 0;
+    }
+    if (v >= n) {
+        ok = ok && periodic;
+        return periodic ? v - n : n - 1;
+    }
+    return v;
+}
+
 __device__ __forceinline__ int64_t safe_index(const PcgArgs& a, int x, int y, int z, bool& valid) {
     bool ok = true;
-    if (x < 0 || x >= a.n[0]) {
-        ok = ok && a.periodic[0];
-        x = a.periodic[0] ? wrapi(x, a.n[0]) : clampi(x, 0, a.n[0] - 1);
-    }
-    if (y < 0 || y >= a.n[1]) {
-        ok = ok && a.periodic[1];
-        y = a.periodic[1] ? wrapi(y, a.n[1]) : clampi(y, 0, a.n[1] - 1);
-    }
-    if (z < 0 || z >= a.n[2]) {
-        ok = ok && a.periodic[2];
-        z = a.periodic[2] ? wrapi(z, a.n[2]) : clampi(z, 0, a.n[2] - 1);
-    }
+    x = safe_axis(x, a.n[0], a.periodic[0], ok);
+    y = safe_axis(y, a.n[1], a.periodic[1], ok);
+    z = safe_axis(z, a.n[2], a.periodic[2], ok);
     valid = ok;
     return lin(a, x, y, z);
 }
@@ -516,8 +538,9 @@ PcgArgs make_args(const Context& c) {
     return a;
 }
 
-// Launch shape of the stencil passes: 64 x 8 threads x 2 planes (256 x 8
-// cells per tile plane with float4), or 16 x 16 threads on narrow grids.
+// Launch shape of the stencil passes: 64 x 4 (p.Ap) or 64 x 8 (update)
+// threads x 2 planes with float4 rows (256 x 4 or 256 x 8 cells per tile
+// plane), 16 x 16 threads on narrow grids, scalar 64 x 4 when nx % 4 != 0.
 template <int TXV, int TY, int ZR, int VEC, bool JACOBI>
 void launch_stencils(Context& c, PcgArgs a, int mode) {
     a.tiles[0] = (c.grid.n[0] + TXV * VEC - 1) / (TXV * VEC);
@@ -535,9 +558,14 @@ template <bool JACOBI>
 void stencil_pass(Context& c, int mode) {
     const PcgArgs a = make_args(c);
     const bool vec = c.grid.n[0] % 4 == 0;
-    if (vec && c.grid.n[0] >= 128) launch_stencils<64, 8, 2, 4, JACOBI>(c, a, mode);
-    else if (vec) launch_stencils<16, 16, 2, 4, JACOBI>(c, a, mode);
-    else launch_stencils<64, 4, 2, 1, JACOBI>(c, a, mode);
+    if (vec && c.grid.n[0] >= 128) {
+        if (mode == 0) launch_stencils<64, 4, 2, 4, JACOBI>(c, a, mode);
+        else launch_stencils<64, 8, 2, 4, JACOBI>(c, a, mode);
+    } else if (vec) {
+        launch_stencils<16, 16, 2, 4, JACOBI>(c, a, mode);
+    } else {
+        launch_stencils<64, 4, 2, 1, JACOBI>(c, a, mode);
+    }
 }
 
 bool jacobi_on(const Context& c) {
