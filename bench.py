@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--check-interval", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--autotune", action="store_true", help="time candidate launch shapes on this grid first (after warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -111,6 +112,13 @@ def main():
             ctx.step()
             step_index += 1
         hip.synchronize()
+        launches = None
+        if args.autotune:
+            rep = ctx.autotune_launches()
+            launches = {"pcg_curvature": rep.pcg_curvature.tuple(), "pcg_update": rep.pcg_update.tuple(),
+                        "advect": [rep.advect[k].tuple() for k in range(4)],
+                        "pcg_curvature_us": list(rep.pcg_curvature_us), "pcg_update_us": list(rep.pcg_update_us),
+                        "advect_us": [list(rep.advect_us[k]) for k in range(4)], "candidates_timed": rep.candidates_timed}
         ctx.set_kernel_timing(not args.no_timing)
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -163,6 +171,8 @@ def main():
         "pcg_iterations_per_step": round(mean_iters, 1), "max_relative_residual": max(resid),
         "dt_range": [min(dts), max(dts)], "roofline": roofline, "cpu_baseline": None,
     }
+    if launches is not None:
+        out["launches"] = launches
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, mean_iters, threads)

@@ -149,6 +149,42 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_plume_sources(Sm
    that roofline fractions are read against). */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gb_per_s, double* out_ms_per_copy);
 
+/* Launch shapes.  Replaces the reference's occupancy-derived block choice
+   (choose_block, backend-cuda.cu:1881-1911).  By default every kernel uses a
+   fixed table chosen from measurements on MI355X, which keeps results
+   bitwise reproducible across contexts.  autotune_launches times candidate
+   shapes on this context's own grid and state -- the tile of each of the two
+   PCG stencil passes and the block of each advection kernel -- applies the
+   fastest and caches the choice for the grid class (dimensions and boundary
+   kinds) for the rest of the process; later calls for the same class reuse
+   it without timing.  It is synchronous, leaves the simulation state
+   unchanged (it writes only the intermediate advected velocity and solver
+   scratch, which the next step recomputes) and rebuilds the step graph.  A
+   PCG tile other than the table's changes the grouping of the FP64 partial
+   sums, so results may differ from the table's in the last bits.
+   reset_launches returns to the table. */
+typedef struct SmokeSimulationLaunchShape {
+    int32_t threads_x;  /* threads along x (each owns `cells_x` consecutive cells) */
+    int32_t threads_y;
+    int32_t threads_z;  /* advection: threads along z; PCG: z planes per tile */
+    int32_t cells_x;    /* PCG: 4 (float4 rows) or 1; advection: 1 */
+} SmokeSimulationLaunchShape;
+
+typedef struct SmokeSimulationLaunchReport {
+    SmokeSimulationLaunchShape pcg_curvature;   /* PCG pass 2 (p.Ap) */
+    SmokeSimulationLaunchShape pcg_update;      /* PCG pass 3 (x, r update) */
+    SmokeSimulationLaunchShape advect[4];       /* advection of u, v, w, and of T + rho */
+    float pcg_curvature_us[2];                  /* [0] table shape, [1] chosen shape (average per launch) */
+    float pcg_update_us[2];
+    float advect_us[4][2];
+    int32_t candidates_timed;                   /* 0 when the grid class was already tuned in this process */
+    int32_t from_cache;
+} SmokeSimulationLaunchReport;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_autotune_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_reset_launches(SmokeSimulationContext context);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
+
 #ifdef __cplusplus
 }
 #endif

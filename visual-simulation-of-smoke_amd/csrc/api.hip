@@ -11,8 +11,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "context.hpp"
@@ -181,6 +184,48 @@ SmokeSimulationResult guarded_call(Context* c, auto&& fn) {
     }
 }
 
+// Launch shapes chosen by the tuner, per device and grid class, for the whole
+// process: dimensions, boundary kinds and interpolation mode decide both the
+// work per kernel and the candidate set.
+struct LaunchChoice {
+    int pcg_shape[2];
+    int advect_block[4];
+    SmokeSimulationLaunchReport report;
+};
+
+using GridClass = std::tuple<int, int, int, int, uint32_t, uint32_t, uint32_t, int>;
+
+GridClass grid_class(const Context& c) {
+    uint32_t flow = 0, dens = 0, temp = 0;
+    for (int f = 0; f < 6; ++f) {
+        flow = flow * 8u + c.grid.flow_type[f];
+        dens = dens * 8u + c.grid.density.type[f];
+        temp = temp * 8u + c.grid.temperature.type[f];
+    }
+    return {c.device, c.grid.n[0], c.grid.n[1], c.grid.n[2], flow, dens, temp, c.grid.cubic};
+}
+
+std::mutex& launch_cache_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+std::map<GridClass, LaunchChoice>& launch_cache() {
+    static std::map<GridClass, LaunchChoice> cache;
+    return cache;
+}
+
+void describe_launches(const Context& c, SmokeSimulationLaunchReport& r) {
+    describe_pcg_shape(c.pcg_shape[0], r.pcg_curvature);
+    describe_pcg_shape(c.pcg_shape[1], r.pcg_update);
+    for (int k = 0; k < 4; ++k) describe_advect_block(c.advect_block[k], r.advect[k]);
+}
+
+void table_launches(Context& c) {
+    table_pcg_shapes(c);
+    table_advect_blocks(c);
+}
+
 void copy_or_fill(Context& c, float* dst, const float* src, float fill, int64_t n) {
     if (src) hip_check(hipMemcpyAsync(dst, src, size_t(n) * sizeof(float), hipMemcpyDeviceToDevice, c.stream), "hipMemcpyAsync D2D");
     else enqueue_fill(c, dst, fill, n);
@@ -301,6 +346,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         smoke::alloc(c.x64[1], cells);
         smoke::alloc(c.scalars, 1);
         smoke::configure_pcg(c);
+        smoke::table_launches(c);
         smoke::alloc(c.partials, c.partial_capacity);
         smoke::alloc(c.dt_device, 1);
         smoke::alloc(c.max_speed_bits, 1);
@@ -331,6 +377,8 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         hip_check(hipMemsetAsync(c.x64[0], 0, cells * sizeof(double), c.stream), "zero x64");
         hip_check(hipMemsetAsync(c.x64[1], 0, cells * sizeof(double), c.stream), "zero x64");
         hip_check(hipMemsetAsync(c.max_speed_bits, 0, sizeof(unsigned), c.stream), "zero max");
+        smoke::enqueue_fill(c, c.dt_device, c.config.dt, 1);
+        smoke::enqueue_fill(c, c.max_speed, 0.0f, 1);
         smoke::enqueue_occupancy_derived(c);
         c.options = smoke::default_options(c.config);
         smoke::upload_solver_constants(c);
@@ -643,6 +691,59 @@ SmokeSimulationResult smoke_simulation_ext_get_kernel_timing(SmokeSimulationCont
 SmokeSimulationResult smoke_simulation_ext_plume_sources(SmokeSimulationContext context, const SmokeSimulationPlumeSource* src, uint32_t step) {
     if (!src) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) { smoke::enqueue_plume_sources(c, *src, step); });
+}
+
+SmokeSimulationResult smoke_simulation_ext_autotune_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        const smoke::GridClass key = smoke::grid_class(c);
+        SmokeSimulationLaunchReport report{};
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> lock(smoke::launch_cache_mutex());
+            auto it = smoke::launch_cache().find(key);
+            if (it != smoke::launch_cache().end()) {
+                std::copy(std::begin(it->second.pcg_shape), std::end(it->second.pcg_shape), c.pcg_shape);
+                std::copy(std::begin(it->second.advect_block), std::end(it->second.advect_block), c.advect_block);
+                report = it->second.report;
+                cached = true;
+            }
+        }
+        if (!cached) {
+            smoke::table_launches(c);
+            hip_check(hipStreamSynchronize(c.stream), "sync");
+            report.candidates_timed = smoke::tune_advect_blocks(c, report);
+            report.candidates_timed += smoke::tune_pcg_shapes(c, report);
+            smoke::upload_solver_constants(c);   // the PCG tuner ran with timing scalars
+            std::lock_guard<std::mutex> lock(smoke::launch_cache_mutex());
+            smoke::LaunchChoice choice{};
+            std::copy(std::begin(c.pcg_shape), std::end(c.pcg_shape), choice.pcg_shape);
+            std::copy(std::begin(c.advect_block), std::end(c.advect_block), choice.advect_block);
+            choice.report = report;
+            smoke::launch_cache()[key] = choice;
+        }
+        smoke::describe_launches(c, report);
+        report.from_cache = cached ? 1 : 0;
+        if (cached) report.candidates_timed = 0;
+        smoke::rebuild_graph(c);
+        hip_check(hipStreamSynchronize(c.stream), "sync");
+        if (out) *out = report;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_reset_launches(SmokeSimulationContext context) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::table_launches(c);
+        smoke::rebuild_graph(c);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out) {
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        SmokeSimulationLaunchReport r{};
+        smoke::describe_launches(c, r);
+        *out = r;
+    });
 }
 
 SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gbps, double* out_ms) {

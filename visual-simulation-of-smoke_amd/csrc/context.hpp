@@ -108,6 +108,12 @@ struct Context {
     hipGraphExec_t graph_exec = nullptr;
     bool diagnostics_dirty = true;
 
+    // launch shapes of the tunable kernels: indices into the candidate tables
+    // of pcg.hip (stencil passes 2 and 3) and step_kernels.hip (advection of
+    // u, v, w and of the scalars); set from the fixed table at creation
+    int pcg_shape[2] = {0, 0};
+    int advect_block[4] = {0, 0, 0, 0};   // set from the table at creation
+
     bool timing = false;
     SmokeSimulationKernelTiming timing_totals{};
     hipEvent_t timing_events[2][6] = {};
@@ -136,5 +142,15 @@ void run_tolerance_pcg(Context& c);                        // host-polled tolera
 void enqueue_true_residual(Context& c);                    // rr_true, bb for the FP32 pressure
 void configure_pcg(Context& c);
 double measure_copy(uint64_t bytes, int reps, double* ms_per_copy);
+
+// Launch-shape table and tuner (pcg.hip, step_kernels.hip).  The tuners time
+// candidates on the context's own grid and return the average microseconds
+// per launch of the table shape and of the chosen one.
+void table_pcg_shapes(Context& c);
+void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out);
+int tune_pcg_shapes(Context& c, SmokeSimulationLaunchReport& report);   // returns candidates timed
+void table_advect_blocks(Context& c);
+void describe_advect_block(int index, SmokeSimulationLaunchShape& out);
+int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report);
 
 }  // namespace smoke

@@ -538,9 +538,37 @@ PcgArgs make_args(const Context& c) {
     return a;
 }
 
-// Launch shape of the stencil passes: 64 x 4 (p.Ap) or 64 x 8 (update)
-// threads x 2 planes with float4 rows (256 x 4 or 256 x 8 cells per tile
-// plane), 16 x 16 threads on narrow grids, scalar 64 x 4 when nx % 4 != 0.
+// Candidate launch shapes of the stencil passes: threads along x (each owning
+// VEC consecutive cells: float4 rows need nx % 4 == 0), threads along y, z
+// planes per tile.  Each grid class has its own candidates; the table
+// default (measured on MI355X at 256^3) is 64 x 4 x 2 for pass 2 and
+// 64 x 8 x 2 for pass 3 with float4 rows, 16 x 16 x 2 on narrow grids, scalar
+// 64 x 4 x 2 when nx % 4 != 0.  Every candidate tile covers at least 64 x 4 x 2
+// cells, which bounds the partials buffer.
+struct StencilShape {
+    int txv, ty, zr, vec;
+};
+
+constexpr StencilShape kStencilShapes[] = {
+    {64, 4, 2, 4}, {64, 8, 2, 4}, {32, 8, 2, 4}, {64, 4, 4, 4},   // 0-3: float4 rows, nx >= 128
+    {16, 16, 2, 4}, {16, 8, 2, 4},                                 // 4-5: float4 rows, narrow
+    {64, 4, 2, 1}, {64, 8, 2, 1}, {32, 8, 2, 1},                   // 6-8: scalar rows
+};
+constexpr int kStencilShapeCount = int(sizeof(kStencilShapes) / sizeof(kStencilShapes[0]));
+
+// [first, last) candidates of this grid's class
+void stencil_candidates(const Context& c, int& first, int& last) {
+    const bool vec = c.grid.n[0] % 4 == 0;
+    if (vec && c.grid.n[0] >= 128) first = 0, last = 4;
+    else if (vec) first = 4, last = 6;
+    else first = 6, last = 9;
+}
+
+int64_t stencil_tiles(const Context& c, const StencilShape& sh) {
+    return int64_t((c.grid.n[0] + sh.txv * sh.vec - 1) / (sh.txv * sh.vec)) * ((c.grid.n[1] + sh.ty - 1) / sh.ty) *
+           ((c.grid.n[2] + sh.zr - 1) / sh.zr);
+}
+
 template <int TXV, int TY, int ZR, int VEC, bool JACOBI>
 void launch_stencils(Context& c, PcgArgs a, int mode) {
     a.tiles[0] = (c.grid.n[0] + TXV * VEC - 1) / (TXV * VEC);
@@ -555,17 +583,25 @@ void launch_stencils(Context& c, PcgArgs a, int mode) {
 }
 
 template <bool JACOBI>
-void stencil_pass(Context& c, int mode) {
+void stencil_pass_shape(Context& c, int mode, int shape) {
     const PcgArgs a = make_args(c);
-    const bool vec = c.grid.n[0] % 4 == 0;
-    if (vec && c.grid.n[0] >= 128) {
-        if (mode == 0) launch_stencils<64, 4, 2, 4, JACOBI>(c, a, mode);
-        else launch_stencils<64, 8, 2, 4, JACOBI>(c, a, mode);
-    } else if (vec) {
-        launch_stencils<16, 16, 2, 4, JACOBI>(c, a, mode);
-    } else {
-        launch_stencils<64, 4, 2, 1, JACOBI>(c, a, mode);
+    switch (shape) {
+        case 0: launch_stencils<64, 4, 2, 4, JACOBI>(c, a, mode); break;
+        case 1: launch_stencils<64, 8, 2, 4, JACOBI>(c, a, mode); break;
+        case 2: launch_stencils<32, 8, 2, 4, JACOBI>(c, a, mode); break;
+        case 3: launch_stencils<64, 4, 4, 4, JACOBI>(c, a, mode); break;
+        case 4: launch_stencils<16, 16, 2, 4, JACOBI>(c, a, mode); break;
+        case 5: launch_stencils<16, 8, 2, 4, JACOBI>(c, a, mode); break;
+        case 6: launch_stencils<64, 4, 2, 1, JACOBI>(c, a, mode); break;
+        case 7: launch_stencils<64, 8, 2, 1, JACOBI>(c, a, mode); break;
+        case 8: launch_stencils<32, 8, 2, 1, JACOBI>(c, a, mode); break;
+        default: throw HipError("invalid stencil shape");
     }
+}
+
+template <bool JACOBI>
+void stencil_pass(Context& c, int mode) {
+    stencil_pass_shape<JACOBI>(c, mode, c.pcg_shape[mode]);
 }
 
 bool jacobi_on(const Context& c) {
@@ -607,12 +643,74 @@ void launch_residual(Context& c, const double* x_old, const float* e_add, double
 }  // namespace
 
 void configure_pcg(Context& c) {
-    // upper bound of the stencil-pass block count over the launch shapes
-    const int64_t tiles_max = int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 3) / 4) * ((c.grid.n[2] + 1) / 2);
+    // upper bound of the stencil-pass block count over all candidate shapes
+    int64_t tiles_max = 0;
+    for (const StencilShape& sh : kStencilShapes) tiles_max = std::max(tiles_max, stencil_tiles(c, sh));
     const int64_t residual_blocks =
         int64_t((c.grid.n[0] + 63) / 64) * ((c.grid.n[1] + 3) / 4) * ((c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
     c.partial_capacity = size_t(std::max<int64_t>(2 * tiles_max, 3 * residual_blocks));
     c.pcg_stencil_blocks = int(tiles_max);
+}
+
+void table_pcg_shapes(Context& c) {
+    int first, last;
+    stencil_candidates(c, first, last);
+    c.pcg_shape[0] = first;                                 // 64x4x2 / 16x16x2 / scalar 64x4x2
+    c.pcg_shape[1] = first == 0 ? 1 : first;                // 64x8x2 for the update pass on wide grids
+}
+
+void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out) {
+    if (index < 0 || index >= kStencilShapeCount) throw HipError("invalid stencil shape");
+    const StencilShape& sh = kStencilShapes[index];
+    out = {sh.txv, sh.ty, sh.zr, sh.vec};
+}
+
+// Times the candidate tiles of passes 2 and 3 on this grid and keeps the
+// fastest.  The scalars are set so that every launch does its full work and
+// leaves the vectors unchanged: not done, no stop test, alpha = 0 (e and r
+// are updated by zero) and not the first iteration.  The caller restores the
+// solver constants afterwards.
+int tune_pcg_shapes(Context& c, SmokeSimulationLaunchReport& report) {
+    PcgScalars s{};
+    s.fixed_mode = 1;
+    s.max_iterations = 1 << 30;
+    s.rz = 1.0;
+    hip_check(hipStreamSynchronize(c.stream), "sync");
+    hip_check(hipMemcpy(c.scalars, &s, sizeof(s), hipMemcpyHostToDevice), "tuning scalars");
+    hipEvent_t t0, t1;
+    hip_check(hipEventCreate(&t0), "event");
+    hip_check(hipEventCreate(&t1), "event");
+    constexpr int kReps = 5;
+    int first, last, timed = 0;
+    stencil_candidates(c, first, last);
+    const int table[2] = {c.pcg_shape[0], c.pcg_shape[1]};
+    float* out_us[2] = {report.pcg_curvature_us, report.pcg_update_us};
+    for (int mode = 0; mode < 2; ++mode) {
+        float best_us = 0.0f;
+        int best = table[mode];
+        for (int shape = first; shape < last; ++shape) {
+            auto launch = [&] {
+                if (jacobi_on(c)) stencil_pass_shape<true>(c, mode, shape);
+                else stencil_pass_shape<false>(c, mode, shape);
+            };
+            launch();   // warm-up
+            hip_check(hipEventRecord(t0, c.stream), "event");
+            for (int r = 0; r < kReps; ++r) launch();
+            hip_check(hipEventRecord(t1, c.stream), "event");
+            hip_check(hipEventSynchronize(t1), "event sync");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+            const float us = 1e3f * ms / kReps;
+            ++timed;
+            if (shape == table[mode]) out_us[mode][0] = us;
+            if (shape == first || us < best_us) best_us = us, best = shape;
+        }
+        c.pcg_shape[mode] = best;
+        out_us[mode][1] = best_us;
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return timed;
 }
 
 void enqueue_pressure_rhs(Context& c) {

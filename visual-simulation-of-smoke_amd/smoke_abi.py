@@ -109,6 +109,19 @@ class KernelTiming(C.Structure):
                 ("step_ms", C.c_double), ("steps", C.c_uint64)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("threads_x", C.c_int32), ("threads_y", C.c_int32), ("threads_z", C.c_int32), ("cells_x", C.c_int32)]
+
+    def tuple(self):
+        return (self.threads_x, self.threads_y, self.threads_z, self.cells_x)
+
+
+class LaunchReport(C.Structure):
+    _fields_ = [("pcg_curvature", LaunchShape), ("pcg_update", LaunchShape), ("advect", LaunchShape * 4),
+                ("pcg_curvature_us", C.c_float * 2), ("pcg_update_us", C.c_float * 2), ("advect_us", (C.c_float * 2) * 4),
+                ("candidates_timed", C.c_int32), ("from_cache", C.c_int32)]
+
+
 class PlumeSource(C.Structure):
     _fields_ = [("center_x", C.c_float), ("center_y", C.c_float), ("center_z", C.c_float),
                 ("radius", C.c_float), ("density_rate", C.c_float), ("temperature_rate", C.c_float),
@@ -256,6 +269,9 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_get_kernel_timing": [C.c_void_p, C.POINTER(KernelTiming)],
         "smoke_simulation_ext_measure_copy_bandwidth": [C.c_uint64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "smoke_simulation_ext_plume_sources": [C.c_void_p, C.POINTER(PlumeSource), C.c_uint32],
+        "smoke_simulation_ext_autotune_launches": [C.c_void_p, C.POINTER(LaunchReport)],
+        "smoke_simulation_ext_reset_launches": [C.c_void_p],
+        "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -413,6 +429,19 @@ class SmokeContext:
         t = KernelTiming()
         check(self.lib.smoke_simulation_ext_get_kernel_timing(self.handle, C.byref(t)), "get_kernel_timing")
         return t
+
+    def autotune_launches(self) -> LaunchReport:
+        r = LaunchReport()
+        check(self.lib.smoke_simulation_ext_autotune_launches(self.handle, C.byref(r)), "autotune_launches")
+        return r
+
+    def reset_launches(self):
+        check(self.lib.smoke_simulation_ext_reset_launches(self.handle), "reset_launches")
+
+    def launches(self) -> LaunchReport:
+        r = LaunchReport()
+        check(self.lib.smoke_simulation_ext_get_launches(self.handle, C.byref(r)), "get_launches")
+        return r
 
 
 def measure_copy_bandwidth(nbytes: int, reps: int):
