@@ -185,6 +185,35 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_autotune_launche
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_reset_launches(SmokeSimulationContext context);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
 
+/* Training pairs of the pressure solve, for a learned preconditioner: crops
+   of the last solve's right-hand side r = b, its solution z = A^-1 b (the
+   FP64 solution in tolerance mode, the FP32 pressure in fixed mode) and the
+   operator's per-cell flag byte, packed on the device and appended to a
+   binary file.
+
+   File (little-endian): a 64-byte header written when the file is empty --
+     char magic[8] = "SMOKETD1"; uint32 version = 1; uint32 crop[3];
+     uint32 grid[3]; uint32 record_bytes; uint32 reserved[6] (0)
+   -- then one record per crop:
+     int32 origin[3]; uint32 step; double scale; uint32 reserved[2];
+     uint16 r[N]; uint16 z[N]; uint8 flags[N]      (N = crop[0]*crop[1]*crop[2])
+   r and z are bfloat16 (round to nearest even) of b / scale and x / scale,
+   x fastest; scale is the RMS of b over the crop (1 when b is zero there),
+   so r has unit RMS and A z = r holds with the same scale.  flags: bits 0-5 =
+   neighbour x-, x+, y-, y+, z-, z+ is a fluid cell of the system, bit 6 =
+   identity row (solid cell or the pressure anchor).  Every crop must lie
+   inside the grid.  A line per record is appended to `path` + ".jsonl"
+   (offset, origin, step, scale, crop, grid).  Synchronous. */
+typedef struct SmokeSimulationTrainingCrops {
+    const char* path;
+    const int32_t* origins;   /* count x (x, y, z) lower corners */
+    uint32_t count;
+    uint32_t size[3];         /* crop extent, the same for every record of a file */
+    uint32_t step;            /* stored with each record */
+} SmokeSimulationTrainingCrops;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops);
+
 #ifdef __cplusplus
 }
 #endif

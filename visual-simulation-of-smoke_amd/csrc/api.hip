@@ -746,6 +746,11 @@ SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext c
     });
 }
 
+SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops) {
+    if (!crops) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) { smoke::write_training_crops(c, *crops); });
+}
+
 SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gbps, double* out_ms) {
     if (!out_gbps || bytes < 16 || repetitions <= 0) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     try {

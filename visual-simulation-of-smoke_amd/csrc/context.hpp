@@ -150,6 +150,9 @@ void table_pcg_shapes(Context& c);
 void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out);
 int tune_pcg_shapes(Context& c, SmokeSimulationLaunchReport& report);   // returns candidates timed
 void table_advect_blocks(Context& c);
+
+// Training-pair crops of the last pressure solve (training_data.hip).
+void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& crops);
 void describe_advect_block(int index, SmokeSimulationLaunchShape& out);
 int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report);
 

@@ -122,6 +122,10 @@ class LaunchReport(C.Structure):
                 ("candidates_timed", C.c_int32), ("from_cache", C.c_int32)]
 
 
+class TrainingCrops(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("origins", C.c_void_p), ("count", C.c_uint32), ("size", C.c_uint32 * 3), ("step", C.c_uint32)]
+
+
 class PlumeSource(C.Structure):
     _fields_ = [("center_x", C.c_float), ("center_y", C.c_float), ("center_z", C.c_float),
                 ("radius", C.c_float), ("density_rate", C.c_float), ("temperature_rate", C.c_float),
@@ -272,6 +276,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_autotune_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_reset_launches": [C.c_void_p],
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
+        "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -437,6 +442,13 @@ class SmokeContext:
 
     def reset_launches(self):
         check(self.lib.smoke_simulation_ext_reset_launches(self.handle), "reset_launches")
+
+    def write_training_crops(self, path, origins, size, step=0):
+        """Appends crops (lower corners `origins`, extent `size` = (sx, sy, sz))
+        of the last solve's (b, x, flags) to `path`; see training_data.read."""
+        o = np.ascontiguousarray(np.asarray(origins, np.int32).reshape(-1, 3))
+        d = TrainingCrops(str(path).encode(), o.ctypes.data, o.shape[0], (C.c_uint32 * 3)(*size), step)
+        check(self.lib.smoke_simulation_ext_write_training_crops(self.handle, C.byref(d)), "write_training_crops")
 
     def launches(self) -> LaunchReport:
         r = LaunchReport()
