@@ -10,6 +10,8 @@ iterative refinement to a true relative residual ||b - Ax|| / ||b|| <= 1e-6.
 
 One "step" is one full smoke step (forces, advection, pressure solve to
 tolerance, projection, scalar advection) with all inputs resident in HBM.
+The default run times 200 steps after 3 warm-up steps (the workload's length);
+each solve starts from the linear extrapolation of the two previous pressures.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -45,9 +47,9 @@ def plume(n: int) -> abi.PlumeSource:
     return abi.PlumeSource(128.0 * s, 32.0 * s, 128.0 * s, 24.0 * s, 1.0, 1.0, 0.05, 2)
 
 
-def solver_options(ctx: abi.SmokeContext, check_interval: int):
+def solver_options(ctx: abi.SmokeContext, check_interval: int, warm_start: int = 2):
     ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI,
-                           relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=1,
+                           relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=warm_start,
                            adaptive_dt=1, cfl=0.5, dt_max=0.01, check_interval=check_interval)
 
 
@@ -86,13 +88,15 @@ def load_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--check-interval", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--autotune", action="store_true", help="time candidate launch shapes on this grid first (after warm-up)")
+    ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
+                    help="initial guess of each solve: zero, previous pressure, extrapolation of the last two")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -105,7 +109,7 @@ def main():
     src = plume(n)
     iters, resid, dts = [], [], []
     with abi.SmokeContext(cfg, hip=hip) as ctx:
-        solver_options(ctx, args.check_interval)
+        solver_options(ctx, args.check_interval, args.warm_start)
         step_index = 0
         for _ in range(args.warmup):
             ctx.plume_sources(src, step_index)
@@ -167,7 +171,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n}^3 plume, plain Jacobi-PCG (FP64 reductions + refinement) to true rel-resid 1e-6, cubic advection, adaptive dt",
-                   "grid": [n, n, n], "solver": "jacobi_pcg", "relative_tolerance": 1e-6, "learned_fpcg": None},
+                   "grid": [n, n, n], "solver": "jacobi_pcg", "relative_tolerance": 1e-6,
+                   "warm_start": ["zero", "previous", "extrapolated"][args.warm_start], "learned_fpcg": None},
         "pcg_iterations_per_step": round(mean_iters, 1), "max_relative_residual": max(resid),
         "dt_range": [min(dts), max(dts)], "roofline": roofline, "cpu_baseline": None,
     }

@@ -49,7 +49,10 @@ typedef struct SmokeSimulationSolverOptions {
     double relative_tolerance;    /* e.g. 1e-6 */
     int32_t max_iterations;       /* inner PCG iterations per solve, all passes */
     int32_t max_refinements;      /* FP64 refinement passes after the first inner solve */
-    uint32_t warm_start;          /* start from the previous step's FP64 pressure */
+    uint32_t warm_start;          /* initial guess: 0 = zero, 1 = the previous step's FP64 pressure,
+                                     2 = linear extrapolation 2 x_n - x_(n-1) of the last two solutions
+                                     (1 until two exist; the history restarts when the occupancy or the
+                                     imported pressure changes) */
     uint32_t adaptive_dt;         /* dt = min(dt_max, cfl * h / max|u_face|) */
     float cfl;
     float dt_max;

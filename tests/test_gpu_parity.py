@@ -229,24 +229,51 @@ def test_adaptive_time_step():
     assert stats.dt == pytest.approx(min(0.01, 0.5 * np.float32(cfg.cell_size) / vmax), rel=1e-6)
 
 
-def test_checkpoint_roundtrip_and_determinism(tmp_path):
+@pytest.mark.parametrize("warm", [1, 2])
+def test_checkpoint_roundtrip_and_determinism(tmp_path, warm):
+    """Restoring a checkpoint continues the run bitwise -- with the
+    extrapolated warm start (warm = 2) the checkpoint carries the solution
+    history it needs."""
     cfg = cfg_for("mixed")
     st = make_state(cfg, 12, solids=2)
     path = str(tmp_path / "state.ck").encode()
     with abi.SmokeContext(cfg) as a:
         load_gpu(a, st)
-        tolerance_options(a)
-        a.step(2)
+        tolerance_options(a, warm=warm)
+        a.step(3)
         assert a.lib.smoke_simulation_ext_save_checkpoint(a.handle, path) == 0
         a.step(2)
         ref = a.export_state()
     with abi.SmokeContext(cfg) as b:
-        tolerance_options(b)
+        tolerance_options(b, warm=warm)
         assert b.lib.smoke_simulation_ext_load_checkpoint(b.handle, path) == 0
         b.step(2)
         got = b.export_state()
     for k in ref:
         np.testing.assert_array_equal(ref[k], got[k], err_msg=k)
+
+
+def test_extrapolated_warm_start_converges_in_fewer_iterations():
+    """warm_start = 2 starts each solve from 2 x_n - x_(n-1): every solve
+    still reaches the true residual 1e-6, and over a smooth plume run the
+    extrapolated guess needs fewer PCG iterations than the previous pressure."""
+    import bench
+    n = 48
+    cfg = bench.workload_config(n)
+    src = bench.plume(n)
+    totals = {}
+    for warm in (1, 2):
+        its = []
+        with abi.SmokeContext(cfg) as ctx:
+            bench.solver_options(ctx, 16, warm)
+            for k in range(12):
+                ctx.plume_sources(src, k)
+                ctx.step()
+                st = ctx.stats()
+                assert st.converged == 1 and st.relative_residual <= 1e-6
+                its.append(st.iterations)
+        totals[warm] = sum(its[4:])
+    assert totals[2] < totals[1], totals
 
 
 def test_plume_sources_match_oracle_bits():

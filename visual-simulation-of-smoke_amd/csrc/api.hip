@@ -89,6 +89,7 @@ void destroy_resources(Context& c) {
     release(c.solid_count);
     release(c.x64[0]);
     release(c.x64[1]);
+    release(c.x64_hist);
     release(c.scalars);
     release(c.partials);
     release(c.max_speed_bits);
@@ -108,7 +109,7 @@ SmokeSimulationSolverOptions default_options(const SmokeSimulationConfig& cfg) {
     o.relative_tolerance = 1.0e-6;
     o.max_iterations = 20000;
     o.max_refinements = 2;
-    o.warm_start = 1;
+    o.warm_start = 2;
     o.adaptive_dt = 0;
     o.cfl = 0.5f;
     o.dt_max = cfg.dt;
@@ -296,6 +297,7 @@ std::vector<CheckpointField> checkpoint_fields(Context& c) {
         {"occupancy", c.occupancy, cells},
         {"pressure", c.pressure, cb},
         {"pressure_f64", c.x64[c.x64_current], cells * sizeof(double)},
+        {"pressure_f64_history", c.x64_hist, cells * sizeof(double)},
     };
 }
 
@@ -344,6 +346,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         smoke::alloc(c.solid_count, 1);
         smoke::alloc(c.x64[0], cells);
         smoke::alloc(c.x64[1], cells);
+        smoke::alloc(c.x64_hist, cells);
         smoke::alloc(c.scalars, 1);
         smoke::configure_pcg(c);
         smoke::table_launches(c);
@@ -376,6 +379,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         smoke::enqueue_fill_u8(c, c.occupancy, 0, c.grid.cells);
         hip_check(hipMemsetAsync(c.x64[0], 0, cells * sizeof(double), c.stream), "zero x64");
         hip_check(hipMemsetAsync(c.x64[1], 0, cells * sizeof(double), c.stream), "zero x64");
+        hip_check(hipMemsetAsync(c.x64_hist, 0, cells * sizeof(double), c.stream), "zero x64");
         hip_check(hipMemsetAsync(c.max_speed_bits, 0, sizeof(unsigned), c.stream), "zero max");
         smoke::enqueue_fill(c, c.dt_device, c.config.dt, 1);
         smoke::enqueue_fill(c, c.max_speed, 0.0f, 1);
@@ -625,7 +629,7 @@ SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContex
         FILE* f = std::fopen(path, "wb");
         if (!f) throw smoke::HipError("cannot open checkpoint for writing");
         auto fields = smoke::checkpoint_fields(c);
-        const uint32_t version = 1, count = uint32_t(fields.size());
+        const uint32_t version = 2, count = uint32_t(fields.size());
         const int32_t dims[3] = {c.config.nx, c.config.ny, c.config.nz};
         bool ok = std::fwrite("SMOKECK1", 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 && std::fwrite(dims, 4, 3, f) == 3 &&
                   std::fwrite(&count, 4, 1, f) == 1;
@@ -639,6 +643,8 @@ SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContex
             if (hipMemcpy(host.data(), fd.device, fd.bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
             ok = ok && std::fwrite(name, 1, 24, f) == 24 && std::fwrite(&bytes, 8, 1, f) == 1 && std::fwrite(host.data(), 1, fd.bytes, f) == fd.bytes;
         }
+        const int32_t solves_done = c.solves_done;   // validity of the warm-start history
+        ok = ok && std::fwrite(&solves_done, 4, 1, f) == 1;
         ok = (std::fclose(f) == 0) && ok;
         if (!ok) throw smoke::HipError("checkpoint write failed");
     });
@@ -653,7 +659,7 @@ SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContex
         char magic[8];
         uint32_t version = 0, count = 0;
         int32_t dims[3];
-        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SMOKECK1", 8) == 0 && std::fread(&version, 4, 1, f) == 1 && version == 1 &&
+        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SMOKECK1", 8) == 0 && std::fread(&version, 4, 1, f) == 1 && version == 2 &&
                   std::fread(dims, 4, 3, f) == 3 && dims[0] == c.config.nx && dims[1] == c.config.ny && dims[2] == c.config.nz &&
                   std::fread(&count, 4, 1, f) == 1;
         auto fields = smoke::checkpoint_fields(c);
@@ -668,9 +674,12 @@ SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContex
             host.resize(fd.bytes);
             ok = std::fread(host.data(), 1, fd.bytes, f) == fd.bytes && hipMemcpy(fd.device, host.data(), fd.bytes, hipMemcpyHostToDevice) == hipSuccess;
         }
+        int32_t solves_done = 0;
+        ok = ok && std::fread(&solves_done, 4, 1, f) == 1 && solves_done >= 0 && solves_done <= 2;
         std::fclose(f);
         if (!ok) throw smoke::HipError("checkpoint read failed");
         smoke::enqueue_occupancy_derived(c);
+        c.solves_done = solves_done;
         hip_check(hipStreamSynchronize(c.stream), "sync");
     });
 }

@@ -90,6 +90,8 @@ struct Context {
     float* pcg_e = nullptr;
     double* x64[2] = {};
     int x64_current = 0;
+    double* x64_hist = nullptr;          // FP64 solution of the solve before the last one
+    int solves_done = 0;                 // tolerance solves since the history was last valid (capped at 2)
     PcgScalars* scalars = nullptr;
     double* partials = nullptr;
     size_t partial_capacity = 0;

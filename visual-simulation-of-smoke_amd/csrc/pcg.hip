@@ -481,6 +481,12 @@ __global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(Pcg
     }
 }
 
+// Warm start extrapolated from the two previous solutions: g = 2 x_n - x_{n-1}.
+__global__ void extrapolate_kernel(const double* __restrict__ x, const double* __restrict__ prev, double* __restrict__ g, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) g[i] = 2.0 * x[i] - prev[i];
+}
+
 __global__ void pressure_from_x64_kernel(const double* __restrict__ x, float* __restrict__ p, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = float(x[i]);
@@ -760,6 +766,21 @@ void run_tolerance_pcg(Context& c) {
     const int64_t n = c.grid.cells;
     if (!o.warm_start) {
         hip_check(hipMemsetAsync(c.x64[c.x64_current], 0, size_t(n) * sizeof(double), c.stream), "zero x64");
+    } else if (o.warm_start == 2) {
+        // linear extrapolation in time once two solutions exist: the buffer
+        // roles rotate so that the history keeps the last solution
+        if (c.solves_done >= 2) {
+            double* cur = c.x64[c.x64_current];
+            double* spare = c.x64[c.x64_current ^ 1];
+            extrapolate_kernel<<<unsigned((n + 255) / 256), 256, 0, c.stream>>>(cur, c.x64_hist, spare, n);
+            hip_check(hipGetLastError(), "extrapolate_kernel");
+            c.x64[c.x64_current ^ 1] = c.x64_hist;
+            c.x64_hist = cur;
+            c.x64[c.x64_current] = spare;
+        } else if (c.solves_done == 1) {
+            hip_check(hipMemcpyAsync(c.x64_hist, c.x64[c.x64_current], size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+                      "warm-start history");
+        }
     }
     enqueue_pcg_init(c, true);
     const int chunk = std::max(1, o.check_interval);
@@ -832,6 +853,7 @@ void run_tolerance_pcg(Context& c) {
         if (passes > o.max_refinements || host.iteration >= o.max_iterations) break;
     }
     enqueue_pressure_from_x64(c);
+    c.solves_done = o.warm_start == 2 ? std::min(c.solves_done + 1, 2) : 1;
     c.stats.iterations = host.iteration;
     c.stats.refinements = passes > 0 ? passes - 1 : 0;
     c.stats.rhs_norm = std::sqrt(host.bb);
