@@ -1,0 +1,82 @@
+/*
+ * Building blocks of the learned pressure preconditioner (MI355X
+ * implementation): 3x3x3 convolutions over channels-last bfloat16 volumes on
+ * the gfx950 matrix cores, with FP32 accumulation.  No counterpart exists in
+ * the reference; these entry points serve the preconditioner's training and
+ * its tests.  All pointers are device pointers; every call is ordered on
+ * `stream` (a hipStream_t, NULL = the default stream) and asynchronous.
+ *
+ * Volume layout: element (b, z, y, x, c) of a batch of volumes with C
+ * channels is at ((((b * D + z) * H + y) * W + x) * C + c), bfloat16.
+ *
+ * Convolution modes, output voxel o, tap t in {0,1,2}^3, input voxel i:
+ *   SAME (stride 1):            i = o + t - 1                 (out dims = in dims)
+ *   DOWN (stride 2):            i = 2 o + t - 1               (out dims = ceil(in / 2))
+ *   UP   (transposed, stride 2): o = 2 i + t - 1, i.e. i = (o + 1 - t) / 2 when
+ *                               even                          (out dims = 2 in)
+ * Input voxels outside the volume read as zero.  y[o][n] = bias[n] +
+ * sum_{t, c} W[t][c][n] * x[i(o, t)][c].  Weights W are FP32 [27][cin][cout]
+ * with t = (tz * 3 + ty) * 3 + tx; the kernels read them packed to bfloat16
+ * by smoke_nn_pack_weights.  cin must be a multiple of 8 and cout of 16
+ * (pad with zero channels).
+ */
+#ifndef VISUAL_SIMULATION_OF_SMOKE_3D_NN_H
+#define VISUAL_SIMULATION_OF_SMOKE_3D_NN_H
+
+#include <stdint.h>
+
+#include "visual-simulation-of-smoke-3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum SmokeNnConvMode {
+    SMOKE_NN_CONV_SAME = 0,
+    SMOKE_NN_CONV_DOWN = 1,
+    SMOKE_NN_CONV_UP = 2,
+} SmokeNnConvMode;
+
+typedef struct SmokeNnConvDesc {
+    int32_t batch;
+    int32_t in[3];    /* W, H, D of the input volume */
+    int32_t out[3];   /* W, H, D of the output volume (must match the mode) */
+    int32_t cin;
+    int32_t cout;
+    int32_t mode;     /* SmokeNnConvMode */
+} SmokeNnConvDesc;
+
+/* Bytes of one packed weight array (forward or data-gradient layout). */
+SMOKE_SIMULATION_API uint64_t smoke_nn_packed_weight_bytes(const SmokeNnConvDesc* desc);
+
+/* FP32 weights [27][cin][cout] -> bfloat16 forward layout and data-gradient
+   layout (either output may be NULL). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_pack_weights(const SmokeNnConvDesc* desc, const float* weights, uint16_t* forward_packed,
+                                                                 uint16_t* backward_packed, void* stream);
+
+/* y = act(bias + conv(x) + skip): bias (FP32, cout) and skip (bfloat16, the
+   output's shape) may be NULL; relu != 0 applies max(., 0).  Writes the
+   bfloat16 output y and/or, when y_f32 is not NULL, the FP32 values of the
+   first `f32_channels` channels as a [voxels][f32_channels] array. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_forward(const SmokeNnConvDesc* desc, const uint16_t* x, const uint16_t* forward_packed,
+                                                                 const float* bias, const uint16_t* skip, int32_t relu, uint16_t* y, float* y_f32,
+                                                                 int32_t f32_channels, void* stream);
+
+/* Gradient with respect to the input: dx = (conv^T(dy) + add) * [mask > 0],
+   where add (bfloat16, the input's shape) and mask (bfloat16, the input's
+   shape: the forward activation whose ReLU the gradient passes back through)
+   may be NULL. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_data(const SmokeNnConvDesc* desc, const uint16_t* dy, const uint16_t* backward_packed,
+                                                                       const uint16_t* add, const uint16_t* mask, uint16_t* dx, void* stream);
+
+/* Accumulates dW[27][cin][cout] += sum_o x[i(o,t)][c] dy[o][n] and, when db is
+   not NULL, db[n] += sum_o dy[o][n] (FP32, atomically: the summation order
+   is not fixed). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_weights(const SmokeNnConvDesc* desc, const uint16_t* x, const uint16_t* dy, float* dw,
+                                                                          float* db, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISUAL_SIMULATION_OF_SMOKE_3D_NN_H */

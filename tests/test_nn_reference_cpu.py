@@ -1,0 +1,53 @@
+"""Pins the float64 convolution restatement (nn_reference.py) that checks the
+matrix-core kernels: brute-force loops on tiny volumes for every mode, and
+the adjoint identities <dy, conv(x)> = <conv^T(dy), x> = <dW(x, dy), W> that
+tie its data and weight gradients to its forward pass."""
+import numpy as np
+import pytest
+
+import nn_reference as ref
+
+
+def brute(x, w, mode, out_dims):
+    B, D, H, W, _ = x.shape
+    Do, Ho, Wo = out_dims
+    y = np.zeros((B, Do, Ho, Wo, w.shape[2]))
+    for b in range(B):
+        for oz in range(Do):
+            for oy in range(Ho):
+                for ox in range(Wo):
+                    for t in range(27):
+                        tx, ty, tz = t % 3, (t // 3) % 3, t // 9
+                        src = []
+                        for o, tt, n in ((ox, tx, W), (oy, ty, H), (oz, tz, D)):
+                            if mode == ref.SAME:
+                                i = o + tt - 1
+                            elif mode == ref.DOWN:
+                                i = 2 * o + tt - 1
+                            else:
+                                q = o + 1 - tt
+                                i = q // 2 if q % 2 == 0 else -1
+                            src.append(i if 0 <= i < n else None)
+                        if None in src:
+                            continue
+                        ix, iy, iz = src
+                        y[b, oz, oy, ox] += x[b, iz, iy, ix] @ w[t]
+    return y
+
+
+@pytest.mark.parametrize("mode,size", [(ref.SAME, (4, 3, 2)), (ref.DOWN, (5, 4, 3)), (ref.UP, (3, 2, 2))])
+def test_forward_matches_brute_force_and_adjoints_hold(mode, size):
+    rng = np.random.default_rng(mode)
+    W, H, D = size
+    out = tuple(ref.out_size(mode, s) for s in size)[::-1]
+    x = rng.normal(size=(2, D, H, W, 3))
+    w = rng.normal(size=(27, 3, 4))
+    y = ref.forward(x, w, None, mode, out)
+    np.testing.assert_allclose(y, brute(x, w, mode, out), rtol=1e-12, atol=1e-12)
+    dy = rng.normal(size=y.shape)
+    dx = ref.backward_data(dy, w, mode, x.shape)
+    dw, db = ref.backward_weights(x, dy, mode)
+    inner = float((dy * y).sum())
+    assert abs(inner - float((dx * x).sum())) <= 1e-10 * abs(inner)
+    assert abs(inner - float((dw * w).sum())) <= 1e-10 * abs(inner)
+    np.testing.assert_allclose(db, dy.sum(axis=(0, 1, 2, 3)))
