@@ -75,6 +75,46 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_data(const Smo
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_weights(const SmokeNnConvDesc* desc, const uint16_t* x, const uint16_t* dy, float* dw,
                                                                           float* db, void* stream);
 
+/* The learned preconditioner: a 3-level convolutional encoder-decoder with
+   3x3x3 kernels and 16-32-64 channels (stride-2 down, transposed stride-2 up,
+   ReLU, skip additions), mapping a residual r to z ~ A^-1 r.
+     input (8 channels: r / s, identity-row mask, 6 zero channels)
+     c0a SAME 8->16, c0b SAME 16->16 = e0;  d1 DOWN 16->32, c1 SAME 32->32 = e1;
+     d2 DOWN 32->64, c2 SAME 64->64 = e2;   u1 = UP(e2) 64->32 + e1, c1u SAME 32->32;
+     u0 = UP(c1u) 32->16 + e0, c0u SAME 16->16;  out SAME 16->1 (no ReLU)
+   every layer but `out` followed by ReLU; s is the RMS of r over the volume,
+   z = s * out, and z = 0 on identity rows (solid cells and the pressure
+   anchor), so M(a r) = a M(r).  Parameters: per layer W [27][cin][cout] then
+   bias [cout], FP32 masters (the output layer is stored with 16 channels of
+   which only channel 0 is used), bfloat16 copies for the matrix cores.
+   Training minimises mean_b ||A z_b - r_b||^2 / ||r_b||^2 over a batch of
+   volumes with unit-RMS r, A the pressure operator restricted to the volume
+   (flags as in the training-pair files; neighbours outside the volume read as
+   zero), with Adam (beta 0.9, 0.999, eps 1e-8). */
+typedef struct SmokeNnPreconditioner SmokeNnPreconditioner;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_create(uint64_t seed, SmokeNnPreconditioner** out);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_destroy(SmokeNnPreconditioner* net);
+SMOKE_SIMULATION_API int64_t smoke_nn_preconditioner_parameter_count(const SmokeNnPreconditioner* net);
+/* host arrays of parameter_count floats */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_get_parameters(SmokeNnPreconditioner* net, float* out);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_set_parameters(SmokeNnPreconditioner* net, const float* values);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_save(SmokeNnPreconditioner* net, const char* path);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_load(SmokeNnPreconditioner* net, const char* path);
+
+/* One training evaluation on `batch` volumes of dims (W, H, D): r FP32 and
+   flags uint8, [batch][D][H][W] device arrays.  Writes the loss to *loss;
+   copies the parameter gradients to `host_gradients` (parameter_count floats)
+   when not NULL; applies one Adam step with learning rate `lr` when lr > 0.
+   Synchronous. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_train_step(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags,
+                                                                              int32_t batch, const int32_t dims[3], float lr, double* loss,
+                                                                              float* host_gradients, void* stream);
+
+/* z = M(r) for one volume of dims (W, H, D) (FP32 device arrays, flags uint8). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_apply(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags,
+                                                                         const int32_t dims[3], float* z, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

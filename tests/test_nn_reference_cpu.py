@@ -51,3 +51,34 @@ def test_forward_matches_brute_force_and_adjoints_hold(mode, size):
     assert abs(inner - float((dx * x).sum())) <= 1e-10 * abs(inner)
     assert abs(inner - float((dw * w).sum())) <= 1e-10 * abs(inner)
     np.testing.assert_allclose(db, dy.sum(axis=(0, 1, 2, 3)))
+
+
+def test_network_gradients_match_finite_differences():
+    """The float64 network restatement (rounding off) against central
+    differences of its own loss, for parameters of every layer."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent))
+    from test_gpu_preconditioner import volumes
+    r, flags = volumes(1, (8, 6, 4), 5)
+    rng = np.random.default_rng(1)
+    sizes = [27 * ci * co + co for _, ci, co, *_ in ref.LAYERS]
+    params = np.zeros(sum(sizes))
+    off = 0
+    for (_, ci, co, *_), n in zip(ref.LAYERS, sizes):
+        params[off:off + n] = rng.normal(size=n) * np.sqrt(2.0 / (27 * ci))
+        off += n
+    loss, grads = ref.unet_loss_and_gradients(params, r, flags, rounding=False)
+    off = 0
+    for li, n in enumerate(sizes):
+        # the largest-gradient weight of the layer and its first bias
+        w_part = grads[off:off + n - ref.LAYERS[li][2]]
+        picks = [off + int(np.argmax(np.abs(w_part))), off + n - ref.LAYERS[li][2]]
+        for k in picks:
+            h = 1e-6 * max(1.0, abs(params[k]))
+            p1, p2 = params.copy(), params.copy()
+            p1[k] += h
+            p2[k] -= h
+            fd = (ref.unet_loss_and_gradients(p1, r, flags, rounding=False)[0] - ref.unet_loss_and_gradients(p2, r, flags, rounding=False)[0]) / (2 * h)
+            assert abs(fd - grads[k]) <= 1e-4 * max(abs(fd), 1e-3 * abs(loss)), (li, k, fd, grads[k])
+        off += n

@@ -12,7 +12,7 @@
 // the dual mode (SAME with flipped taps, DOWN <-> UP) on a second packing of
 // the weights; the weight gradient runs the matrix core with K = voxels.
 #include "context.hpp"
-#include "visual-simulation-of-smoke-3d-nn.h"
+#include "nn.hpp"
 
 namespace smoke {
 namespace nn {
