@@ -26,6 +26,8 @@ from pathlib import Path
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "visual-simulation-of-smoke_amd"))
 
+import numpy as np  # noqa: E402
+
 import smoke_abi as abi  # noqa: E402
 
 METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
@@ -47,10 +49,25 @@ def plume(n: int) -> abi.PlumeSource:
     return abi.PlumeSource(128.0 * s, 32.0 * s, 128.0 * s, 24.0 * s, 1.0, 1.0, 0.05, 2)
 
 
-def solver_options(ctx: abi.SmokeContext, check_interval: int, warm_start: int = 2):
-    ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI,
+def solver_options(ctx: abi.SmokeContext, check_interval: int, warm_start: int = 2, preconditioner: int = abi.PRECONDITIONER_JACOBI):
+    ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=preconditioner,
                            relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=warm_start,
                            adaptive_dt=1, cfl=0.5, dt_max=0.01, check_interval=check_interval)
+
+
+def spheres_occupancy(n: int, count: int = 12, seed: int = 5) -> np.ndarray:
+    """The obstacle scene: `count` seeded solid spheres of radius U(8, 24) *
+    n / 256 cells, centred above the source (y in [0.3 n, 0.9 n])."""
+    rng = np.random.default_rng(seed)
+    s = n / 256.0
+    z, y, x = np.meshgrid(np.arange(n) + 0.5, np.arange(n) + 0.5, np.arange(n) + 0.5, indexing="ij")
+    occ = np.zeros((n, n, n), np.uint8)
+    for _ in range(count):
+        r = rng.uniform(8.0, 24.0) * s
+        cx, cz = rng.uniform(r, n - r, size=2)
+        cy = rng.uniform(0.3 * n, 0.9 * n - r)
+        occ[(x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= r * r] = 1
+    return occ
 
 
 def cpu_baseline(n: int, iterations_per_step: float, threads: int):
@@ -97,6 +114,11 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="time candidate launch shapes on this grid first (after warm-up)")
     ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
                     help="initial guess of each solve: zero, previous pressure, extrapolation of the last two")
+    ap.add_argument("--scene", choices=("plume", "spheres"), default="plume",
+                    help="plume: the closed box; spheres: the same with 12 seeded solid spheres and confinement 2.5")
+    ap.add_argument("--solver", choices=("jacobi", "learned"), default="jacobi",
+                    help="Jacobi PCG, or flexible PCG with the learned preconditioner (Jacobi fallback)")
+    ap.add_argument("--weights", default=str(REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_plume256_crop64.bin"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -106,10 +128,19 @@ def main():
     hip = abi.Hip()
     copy_gbps, copy_ms = abi.measure_copy_bandwidth(1 << 30, 20)
     cfg = workload_config(n)
+    if args.scene == "spheres":
+        cfg.vorticity_confinement = 2.5
     src = plume(n)
-    iters, resid, dts = [], [], []
+    iters, resid, dts, learned_its, fallbacks = [], [], [], [], 0
+    net = abi.LearnedPreconditioner(seed=0, path=args.weights) if args.solver == "learned" else None
     with abi.SmokeContext(cfg, hip=hip) as ctx:
-        solver_options(ctx, args.check_interval, args.warm_start)
+        if args.scene == "spheres":
+            ctx.set_occupancy(spheres_occupancy(n))
+        if net is not None:
+            ctx.set_learned_preconditioner(net)
+            solver_options(ctx, 1, args.warm_start, abi.PRECONDITIONER_LEARNED)
+        else:
+            solver_options(ctx, args.check_interval, args.warm_start)
         step_index = 0
         for _ in range(args.warmup):
             ctx.plume_sources(src, step_index)
@@ -132,10 +163,19 @@ def main():
             iters.append(st.iterations)
             resid.append(st.relative_residual)
             dts.append(st.dt)
+            learned_its.append(st.learned_iterations)
+            fallbacks += st.learned_fallback
             step_index += 1
         hip.synchronize()
         elapsed = time.perf_counter() - t0
         timing = ctx.kernel_timing()
+        div = ctx.view("FLOW_DIVERGENCE").astype(np.float64)
+        fluid = ctx.view("OCCUPANCY") == 0
+        divergence_l2 = float(np.sqrt(np.sum(div[fluid] ** 2)))
+        if net is not None:
+            ctx.set_learned_preconditioner(None)
+    if net is not None:
+        net.close()
 
     steps_per_s = args.steps / elapsed
     cells = n ** 3
@@ -170,10 +210,15 @@ def main():
         "metric": METRIC, "value": round(steps_per_s, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{n}^3 plume, plain Jacobi-PCG (FP64 reductions + refinement) to true rel-resid 1e-6, cubic advection, adaptive dt",
-                   "grid": [n, n, n], "solver": "jacobi_pcg", "relative_tolerance": 1e-6,
-                   "warm_start": ["zero", "previous", "extrapolated"][args.warm_start], "learned_fpcg": None},
+        "config": {"workload": (f"{n}^3 plume" + (" with 12 solid spheres" if args.scene == "spheres" else "") +
+                                (", learned-preconditioner flexible PCG (Jacobi fallback)" if net is not None else ", plain Jacobi-PCG") +
+                                " (FP64 reductions + refinement) to true rel-resid 1e-6, cubic advection, adaptive dt"),
+                   "grid": [n, n, n], "scene": args.scene, "solver": "learned_fpcg" if net is not None else "jacobi_pcg",
+                   "relative_tolerance": 1e-6, "warm_start": ["zero", "previous", "extrapolated"][args.warm_start],
+                   "learned_fpcg": ({"weights": Path(args.weights).name} if net is not None else None)},
         "pcg_iterations_per_step": round(mean_iters, 1), "max_relative_residual": max(resid),
+        "learned_iterations_per_step": round(sum(learned_its) / len(learned_its), 1), "learned_fallbacks": fallbacks,
+        "divergence_l2_after_projection": divergence_l2,
         "dt_range": [min(dts), max(dts)], "roofline": roofline, "cpu_baseline": None,
     }
     if launches is not None:

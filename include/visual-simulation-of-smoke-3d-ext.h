@@ -41,6 +41,12 @@ typedef enum SmokeSimulationPressureMode {
 typedef enum SmokeSimulationPreconditioner {
     SMOKE_SIMULATION_PRECONDITIONER_NONE = 0,
     SMOKE_SIMULATION_PRECONDITIONER_JACOBI = 1,
+    /* flexible PCG (Polak-Ribiere beta) with the learned network attached by
+       smoke_simulation_ext_set_learned_preconditioner (visual-simulation-of-
+       smoke-3d-nn.h); when the inner residual drops by less than 10 % over
+       5 iterations, or turns non-finite, the rest of that solve runs
+       Jacobi PCG.  The host checks convergence after every iteration. */
+    SMOKE_SIMULATION_PRECONDITIONER_LEARNED = 2,
 } SmokeSimulationPreconditioner;
 
 typedef struct SmokeSimulationSolverOptions {
@@ -70,6 +76,8 @@ typedef struct SmokeSimulationSolverStats {
     float dt;                     /* time step used by the last step */
     float max_face_speed;         /* max |u_face| at the start of the last step */
     uint64_t steps;               /* steps taken by this context */
+    int32_t learned_iterations;   /* inner iterations of the last solve preconditioned by the network */
+    int32_t learned_fallback;     /* the last solve fell back to Jacobi PCG */
 } SmokeSimulationSolverStats;
 
 /* Host-side copy of the simulation state.  Any pointer may be NULL (skipped).
@@ -216,6 +224,12 @@ typedef struct SmokeSimulationTrainingCrops {
 } SmokeSimulationTrainingCrops;
 
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops);
+
+/* Attaches (or, with NULL, detaches) a learned preconditioner created by
+   smoke_nn_preconditioner_create (an opaque SmokeNnPreconditioner*); the
+   context does not own it: keep it alive while attached.  Used when the
+   solver options select SMOKE_SIMULATION_PRECONDITIONER_LEARNED. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_learned_preconditioner(SmokeSimulationContext context, void* network);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,7 @@ void destroy_resources(Context& c) {
     release(c.x64[0]);
     release(c.x64[1]);
     release(c.x64_hist);
+    release(c.pcg_z);
     release(c.scalars);
     release(c.partials);
     release(c.max_speed_bits);
@@ -128,7 +129,7 @@ void upload_solver_constants(Context& c) {
     s.inner_reduction = kInnerReduction;
     s.max_iterations = c.options.max_iterations;
     s.fixed_mode = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS;
-    s.jacobi = !s.fixed_mode && c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
+    s.jacobi = !s.fixed_mode && c.options.preconditioner != SMOKE_SIMULATION_PRECONDITIONER_NONE;
     hip_check(hipStreamSynchronize(c.stream), "sync");
     hip_check(hipMemcpy(c.scalars, &s, sizeof(s), hipMemcpyHostToDevice), "upload solver constants");
 }
@@ -527,7 +528,7 @@ SmokeSimulationResult smoke_simulation_ext_default_solver_options(SmokeSimulatio
 
 SmokeSimulationResult smoke_simulation_ext_set_solver_options(SmokeSimulationContext context, const SmokeSimulationSolverOptions* o) {
     if (!o) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
-    if (o->pressure_mode > 1u || o->preconditioner > 1u || !(o->relative_tolerance > 0.0) || o->max_iterations <= 0 || o->max_refinements < 0 ||
+    if (o->pressure_mode > 1u || o->preconditioner > 2u || !(o->relative_tolerance > 0.0) || o->max_iterations <= 0 || o->max_refinements < 0 ||
         (o->adaptive_dt && (!(o->cfl > 0.0f) || !(o->dt_max > 0.0f))))
         return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) {
@@ -758,6 +759,16 @@ SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext c
 SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops) {
     if (!crops) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) { smoke::write_training_crops(c, *crops); });
+}
+
+SmokeSimulationResult smoke_simulation_ext_set_learned_preconditioner(SmokeSimulationContext context, void* network) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        if (network && !c.pcg_z) {
+            smoke::alloc(c.pcg_z, size_t(c.grid.cells));
+            smoke::enqueue_fill(c, c.pcg_z, 0.0f, c.grid.cells);
+        }
+        c.learned = static_cast<SmokeNnPreconditioner*>(network);
+    });
 }
 
 SmokeSimulationResult smoke_simulation_ext_measure_copy_bandwidth(uint64_t bytes, int32_t repetitions, double* out_gbps, double* out_ms) {

@@ -9,6 +9,8 @@
 #include <string>
 
 #include "device_common.hpp"
+
+struct SmokeNnPreconditioner;
 #include "visual-simulation-of-smoke-3d-ext.h"
 
 namespace smoke {
@@ -39,6 +41,7 @@ struct PcgScalars {
     int first;          // next direction pass starts a fresh Krylov space (beta = 0)
     int fixed_mode;     // reference semantics: no early exit
     int jacobi;
+    double rz_old_new;  // flexible PCG: r_(k+1) . z_k
 };
 
 struct PcgStatus {      // what the host polls between chunks
@@ -88,6 +91,8 @@ struct Context {
     float* pcg_r = nullptr;
     float* pcg_p = nullptr;            // search direction, updated in place
     float* pcg_e = nullptr;
+    float* pcg_z = nullptr;              // flexible PCG: preconditioned residual (allocated with the network)
+    SmokeNnPreconditioner* learned = nullptr;   // attached learned preconditioner, not owned
     double* x64[2] = {};
     int x64_current = 0;
     double* x64_hist = nullptr;          // FP64 solution of the solve before the last one

@@ -24,8 +24,10 @@
 // order (deterministic) and updates alpha / beta on the device, so
 // fixed-iteration solves never return to the host.
 #include "context.hpp"
+#include "nn.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace smoke {
@@ -270,8 +272,12 @@ __device__ __forceinline__ void store_row(float* __restrict__ f, int64_t i, cons
     else f[i] = o.v[0];
 }
 
+//   MODE 2 (flexible PCG update): as MODE 1, with the partial r.z taken
+//                    against the stored previous preconditioned residual z
+//                    (r_(k+1) . z_k, for the Polak-Ribiere beta).
 template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
-__global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r) {
+__global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r,
+                                                              const float* __restrict__ zold) {
     PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
     constexpr int NWAVES = TXV * TY / 64;
@@ -325,11 +331,12 @@ __global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const f
         } else {
             fv[0] = flags[i];
         }
-        Row<VEC> ev{}, rv{};
-        if constexpr (MODE == 1) {
+        Row<VEC> ev{}, rv{}, zv{};
+        if constexpr (MODE >= 1) {
             ev = load_row<VEC>(e, i);
             rv = load_row<VEC>(r, i);
         }
+        if constexpr (MODE == 2) zv = load_row<VEC>(zold, i);
         const bool active = owner && z < a.n[2];
         Row<VEC> en, rn;
         double s0 = 0.0, s1 = 0.0;
@@ -354,12 +361,12 @@ __global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const f
             } else {
                 en.v[k] = (first ? 0.0f : ev.v[k]) + alpha * c;
                 rn.v[k] = rv.v[k] - alpha * ap;
-                const float zk = JACOBI ? rn.v[k] / d : rn.v[k];
+                const float zk = MODE == 2 ? zv.v[k] : (JACOBI ? rn.v[k] / d : rn.v[k]);
                 s0 += double(rn.v[k]) * double(zk);
                 s1 += double(rn.v[k]) * double(rn.v[k]);
             }
         }
-        if constexpr (MODE == 1) {
+        if constexpr (MODE >= 1) {
             if (active) {
                 store_row<VEC>(e, i, en);
                 store_row<VEC>(r, i, rn);
@@ -403,6 +410,61 @@ __global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, i
         const int it = S->iteration + 1;
         S->iteration = it;
         if (!S->fixed_mode && (tot[1] <= S->stop2 || it >= S->max_iterations)) S->done = 1;
+    }
+}
+
+// ---- flexible PCG (learned preconditioner): z is a stored vector
+// p <- z (first) or z + beta p
+__global__ __launch_bounds__(256) void fpcg_direction_kernel(PcgArgs a, const float* __restrict__ z, float* __restrict__ p) {
+    const PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    const bool first = S->first != 0;
+    const float beta = first ? 0.0f : float(S->beta);
+    const int64_t cells = int64_t(a.n[0]) * a.n[1] * a.n[2];
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cells; i += int64_t(gridDim.x) * 256)
+        p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// partial r.z per block
+__global__ __launch_bounds__(256) void fpcg_dot_kernel(PcgArgs a, const float* __restrict__ r, const float* __restrict__ z) {
+    const int64_t cells = int64_t(a.n[0]) * a.n[1] * a.n[2];
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cells; i += int64_t(gridDim.x) * 256) s += double(r[i]) * double(z[i]);
+    double v[1] = {s};
+    store_block_partials<1, 4>(v, a.partials, int(gridDim.x), blockIdx.x);
+}
+
+// after the update pass: r_(k+1).z_k, ||r||^2, iteration count, stop test
+__global__ __launch_bounds__(kFinalizeThreads) void fpcg_update_finalize_kernel(PcgArgs a, int nblocks) {
+    PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    double tot[2];
+    sum_partials<2>(a.partials, nblocks, tot);
+    if (threadIdx.x == 0) {
+        S->rz_old_new = tot[0];
+        S->rr = tot[1];
+        S->first = 0;
+        const int it = S->iteration + 1;
+        S->iteration = it;
+        if (tot[1] <= S->stop2 || it >= S->max_iterations) S->done = 1;
+    }
+}
+
+// after z = M(r): r.z; beta = (r.z - r.z_old) / rz_old (Polak-Ribiere), or a
+// fresh start (first) that only records r.z
+__global__ __launch_bounds__(kFinalizeThreads) void fpcg_beta_kernel(PcgArgs a, int nblocks, int start) {
+    PcgScalars* __restrict__ S = a.s;
+    if (S->done) return;
+    double tot[1];
+    sum_partials<1>(a.partials, nblocks, tot);
+    if (threadIdx.x == 0) {
+        if (start) {
+            S->first = 1;
+            S->beta = 0.0;
+        } else {
+            S->beta = guarded(tot[0] - S->rz_old_new, S->rz, false);
+        }
+        S->rz = tot[0];
     }
 }
 
@@ -582,8 +644,9 @@ void launch_stencils(Context& c, PcgArgs a, int mode) {
     a.tiles[2] = (c.grid.n[2] + ZR - 1) / ZR;
     const unsigned blocks = unsigned(a.tiles[0] * a.tiles[1] * a.tiles[2]);
     float* e = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE ? c.pcg_e : c.pressure;
-    if (mode == 0) pcg_stencil_kernel<TXV, TY, ZR, VEC, 0, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r);
-    else pcg_stencil_kernel<TXV, TY, ZR, VEC, 1, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r);
+    if (mode == 0) pcg_stencil_kernel<TXV, TY, ZR, VEC, 0, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r, nullptr);
+    else if (mode == 1) pcg_stencil_kernel<TXV, TY, ZR, VEC, 1, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r, nullptr);
+    else pcg_stencil_kernel<TXV, TY, ZR, VEC, 2, false><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r, c.pcg_z);
     hip_check(hipGetLastError(), "pcg_stencil_kernel");
     c.pcg_stencil_blocks = int(blocks);
 }
@@ -607,11 +670,12 @@ void stencil_pass_shape(Context& c, int mode, int shape) {
 
 template <bool JACOBI>
 void stencil_pass(Context& c, int mode) {
-    stencil_pass_shape<JACOBI>(c, mode, c.pcg_shape[mode]);
+    stencil_pass_shape<JACOBI>(c, mode, c.pcg_shape[mode == 0 ? 0 : 1]);
 }
 
+// Jacobi kernels: the Jacobi preconditioner, and the learned one's fallback
 bool jacobi_on(const Context& c) {
-    return c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE && c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_JACOBI;
+    return c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE && c.options.preconditioner != SMOKE_SIMULATION_PRECONDITIONER_NONE;
 }
 
 // pass 1: p <- z + beta p
@@ -635,6 +699,34 @@ void pass_finalize(Context& c, int mode) {
     if (mode == 0) pcg_alpha_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.pcg_stencil_blocks);
     else pcg_beta_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.pcg_stencil_blocks);
     hip_check(hipGetLastError(), "pcg finalize kernel");
+}
+
+// ---- flexible PCG with the learned preconditioner
+// z = M(r)
+void learned_apply(Context& c) {
+    const int dims[3] = {c.grid.n[0], c.grid.n[1], c.grid.n[2]};
+    preconditioner_apply(*c.learned, c.pcg_r, c.stencil_flags, dims, c.pcg_z, c.stream);
+}
+
+// r.z, then beta (or a fresh Krylov start)
+void learned_rz(Context& c, bool start) {
+    const PcgArgs a = make_args(c);
+    const unsigned blocks = unsigned(std::min<size_t>(1024, c.partial_capacity));
+    fpcg_dot_kernel<<<blocks, 256, 0, c.stream>>>(a, c.pcg_r, c.pcg_z);
+    fpcg_beta_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, int(blocks), start ? 1 : 0);
+    hip_check(hipGetLastError(), "flexible pcg r.z");
+}
+
+// p = z + beta p; alpha = r.z / p.Ap; e += alpha p, r -= alpha Ap with r_(k+1).z_k and ||r||^2
+void learned_iteration(Context& c) {
+    const PcgArgs a = make_args(c);
+    fpcg_direction_kernel<<<unsigned(std::min<int64_t>((c.grid.cells + 255) / 256, 8192)), 256, 0, c.stream>>>(a, c.pcg_z, c.pcg_p);
+    hip_check(hipGetLastError(), "fpcg_direction_kernel");
+    pass_stencil(c, 0);
+    pass_finalize(c, 0);
+    pass_stencil(c, 2);
+    fpcg_update_finalize_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.pcg_stencil_blocks);
+    hip_check(hipGetLastError(), "fpcg_update_finalize_kernel");
 }
 
 void launch_residual(Context& c, const double* x_old, const float* e_add, double* x_new, bool bb, bool reset) {
@@ -798,7 +890,47 @@ void run_tolerance_pcg(Context& c) {
         host.done = 1;
     }
     int observed = host.iteration;
+    bool learned = o.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_LEARNED && c.learned && c.pcg_z;
+    int learned_iterations = 0;
+    bool fell_back = false;
+    // Inner solve with the learned preconditioner: flexible PCG, convergence
+    // checked after every iteration (an application of the network costs far
+    // more than the host round trip).  Returns false when the residual fell
+    // by less than 10 % over 5 iterations or turned non-finite.
+    auto learned_inner = [&]() -> bool {
+        learned_apply(c);
+        learned_rz(c, true);
+        std::vector<double> rr{host.rr};
+        for (;;) {
+            learned_iteration(c);
+            read_scalars();
+            ++learned_iterations;
+            if (!std::isfinite(host.rr) || !std::isfinite(host.rz)) return false;
+            rr.push_back(host.rr);
+            if (host.done) return true;
+            const size_t k = rr.size() - 1;
+            if (k >= 5 && rr[k] > 0.81 * rr[k - 5]) return false;
+            learned_apply(c);
+            learned_rz(c, false);
+        }
+    };
     while (!host.done) {
+        if (learned) {
+            if (!learned_inner()) {
+                // the rest of this solve runs Jacobi PCG; a non-finite
+                // correction is dropped before the refinement adds it
+                learned = false;
+                fell_back = true;
+                if (!std::isfinite(host.rr) || !std::isfinite(host.rz))
+                    hip_check(hipMemsetAsync(c.pcg_e, 0, size_t(n) * sizeof(float), c.stream), "drop correction");
+            }
+            enqueue_pcg_refine(c);
+            ++passes;
+            read_scalars();
+            if (host.done) break;
+            if (passes > o.max_refinements + (fell_back ? 1 : 0) || host.iteration >= o.max_iterations) break;
+            continue;
+        }
         // inner solve: chunks of `chunk` iterations, at most two in flight; in
         // timing mode the first iteration of every chunk is bracketed by events
         int slot = 0;
@@ -859,6 +991,8 @@ void run_tolerance_pcg(Context& c) {
     c.stats.rhs_norm = std::sqrt(host.bb);
     c.stats.relative_residual = host.bb > 0 ? std::sqrt(host.rr_true / host.bb) : 0.0;
     c.stats.converged = host.bb > 0 ? (std::sqrt(host.rr_true) <= o.relative_tolerance * std::sqrt(host.bb)) : 1;
+    c.stats.learned_iterations = learned_iterations;
+    c.stats.learned_fallback = fell_back ? 1 : 0;
 }
 
 double measure_copy(uint64_t bytes, int reps, double* ms_per_copy) {

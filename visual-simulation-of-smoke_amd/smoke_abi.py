@@ -33,7 +33,7 @@ VIEW_KINDS = {
 LAYOUT_F32_3D, LAYOUT_F32_3D_SOA3 = 0, 1
 
 PRESSURE_FIXED_ITERATIONS, PRESSURE_TOLERANCE = 0, 1
-PRECONDITIONER_NONE, PRECONDITIONER_JACOBI = 0, 1
+PRECONDITIONER_NONE, PRECONDITIONER_JACOBI, PRECONDITIONER_LEARNED = 0, 1, 2
 STAGE_FORCES, STAGE_ADVECT_VELOCITY, STAGE_PRESSURE, STAGE_PROJECT, STAGE_SCALARS = range(5)
 
 
@@ -93,7 +93,8 @@ class SolverOptions(C.Structure):
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
                 ("anchor", C.c_int32), ("relative_residual", C.c_double), ("rhs_norm", C.c_double),
-                ("dt", C.c_float), ("max_face_speed", C.c_float), ("steps", C.c_uint64)]
+                ("dt", C.c_float), ("max_face_speed", C.c_float), ("steps", C.c_uint64),
+                ("learned_iterations", C.c_int32), ("learned_fallback", C.c_int32)]
 
 
 class HostState(C.Structure):
@@ -277,6 +278,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_reset_launches": [C.c_void_p],
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
+        "smoke_simulation_ext_set_learned_preconditioner": [C.c_void_p, C.c_void_p],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -450,10 +452,71 @@ class SmokeContext:
         d = TrainingCrops(str(path).encode(), o.ctypes.data, o.shape[0], (C.c_uint32 * 3)(*size), step)
         check(self.lib.smoke_simulation_ext_write_training_crops(self.handle, C.byref(d)), "write_training_crops")
 
+    def set_learned_preconditioner(self, network):
+        """Attaches a learned preconditioner (a LearnedPreconditioner or None)."""
+        self._network = network
+        check(self.lib.smoke_simulation_ext_set_learned_preconditioner(self.handle, network.handle if network else None),
+              "set_learned_preconditioner")
+
     def launches(self) -> LaunchReport:
         r = LaunchReport()
         check(self.lib.smoke_simulation_ext_get_launches(self.handle, C.byref(r)), "get_launches")
         return r
+
+
+class LearnedPreconditioner:
+    """The learned pressure preconditioner (visual-simulation-of-smoke-3d-nn.h):
+    create / load weights, train on device volumes, apply."""
+
+    def __init__(self, seed: int = 0, path=None):
+        self.lib = L = load_library()
+        V = C.c_void_p
+        L.smoke_nn_preconditioner_create.argtypes = [C.c_uint64, C.POINTER(V)]
+        L.smoke_nn_preconditioner_destroy.argtypes = [V]
+        L.smoke_nn_preconditioner_parameter_count.argtypes = [V]
+        L.smoke_nn_preconditioner_parameter_count.restype = C.c_int64
+        L.smoke_nn_preconditioner_get_parameters.argtypes = [V, V]
+        L.smoke_nn_preconditioner_set_parameters.argtypes = [V, V]
+        L.smoke_nn_preconditioner_save.argtypes = [V, C.c_char_p]
+        L.smoke_nn_preconditioner_load.argtypes = [V, C.c_char_p]
+        L.smoke_nn_preconditioner_train_step.argtypes = [V, V, V, C.c_int32, C.POINTER(C.c_int32), C.c_float, C.POINTER(C.c_double), V, V]
+        L.smoke_nn_preconditioner_apply.argtypes = [V, V, V, C.POINTER(C.c_int32), V, V]
+        self.handle = V()
+        check(L.smoke_nn_preconditioner_create(seed, C.byref(self.handle)), "preconditioner create")
+        if path is not None:
+            self.load(path)
+
+    def close(self):
+        if self.handle:
+            self.lib.smoke_nn_preconditioner_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def parameter_count(self) -> int:
+        return int(self.lib.smoke_nn_preconditioner_parameter_count(self.handle))
+
+    def parameters(self) -> np.ndarray:
+        out = np.empty(self.parameter_count(), np.float32)
+        check(self.lib.smoke_nn_preconditioner_get_parameters(self.handle, out.ctypes.data), "get parameters")
+        return out
+
+    def save(self, path):
+        check(self.lib.smoke_nn_preconditioner_save(self.handle, str(path).encode()), "save weights")
+
+    def load(self, path):
+        check(self.lib.smoke_nn_preconditioner_load(self.handle, str(path).encode()), "load weights")
+
+    def train_step(self, d_r: int, d_flags: int, batch: int, dims, lr: float) -> float:
+        """One Adam step on `batch` device volumes (dims = (W, H, D)); returns the loss."""
+        loss = C.c_double()
+        check(self.lib.smoke_nn_preconditioner_train_step(self.handle, d_r, d_flags, batch, (C.c_int32 * 3)(*dims), lr, C.byref(loss),
+                                                          None, None), "train step")
+        return loss.value
 
 
 def measure_copy_bandwidth(nbytes: int, reps: int):

@@ -1,0 +1,129 @@
+"""Trains the learned pressure preconditioner on pressure solves of the plume
+workload, with the library's own kernels (BF16 matrix-core convolutions,
+FP32 Adam) on one GPU.
+
+1. runs the plume workload (`--grid`^3, the 256^3 benchmark plume scaled) for
+   `--steps` steps with Jacobi-PCG to a true residual of 1e-6 and appends
+   `--crops-per-step` random crops of `--crop`^3 cells of every solve's
+   right-hand side and operator flags to a SMOKETD1 file
+   (smoke_simulation_ext_write_training_crops);
+2. trains the network (Adam, `--batch` crops per step, learning rate `--lr`)
+   for `--epochs` passes over the crops or until `--time-limit` seconds,
+   printing the loss ||A z - r||^2 / ||r||^2 as it goes;
+3. saves the weights (SMOKENN1) to `--out`.
+
+Usage: python train_preconditioner.py --out weights.bin [--grid 256 --steps 40 ...]
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+
+import smoke_abi as abi  # noqa: E402
+import training_data as td  # noqa: E402
+
+
+def plume_config(n: int):
+    cfg = abi.make_config(n, n, n, cell_size=1.0 / n, dt=0.01, pressure_iterations=64, ambient=0.0, buoyancy_density=0.05,
+                          buoyancy_temperature=0.25, vorticity=2.0, cubic=True)
+    s = n / 256.0
+    src = abi.PlumeSource(128.0 * s, 32.0 * s, 128.0 * s, 24.0 * s, 1.0, 1.0, 0.05, 2)
+    return cfg, src
+
+
+def generate(path: Path, n: int, steps: int, crop: int, per_step: int, seed: int = 0, log=print):
+    """Runs the plume and logs crops of every solve's right-hand side."""
+    cfg, src = plume_config(n)
+    rng = np.random.default_rng(seed)
+    if path.exists():
+        path.unlink()
+    manifest = Path(str(path) + ".jsonl")
+    if manifest.exists():
+        manifest.unlink()
+    with abi.SmokeContext(cfg) as ctx:
+        ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI, relative_tolerance=1e-6,
+                               max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1, cfl=0.5, dt_max=0.01,
+                               check_interval=32)
+        for k in range(steps):
+            ctx.plume_sources(src, k)
+            ctx.step()
+            if k < 2:
+                continue   # the first solves have a zero or tiny right-hand side
+            origins = rng.integers(0, n - crop + 1, size=(per_step, 3))
+            ctx.write_training_crops(path, origins, (crop, crop, crop), step=k)
+            if k % 10 == 0:
+                log(f"data: step {k}, {ctx.stats().iterations} PCG iterations", flush=True)
+
+
+def train(net: abi.LearnedPreconditioner, data: dict, batch: int, lr: float, epochs: int, time_limit: float, seed: int = 0, log=print):
+    """Adam on the crops (r scaled back from bfloat16, unit RMS); returns the loss history."""
+    hip = abi.Hip()
+    r = data["r"].astype(np.float32)
+    flags = data["flags"]
+    count = r.shape[0]
+    dims = tuple(int(v) for v in data["crop"])
+    d_r = hip.upload(np.ascontiguousarray(r))
+    d_f = hip.upload(np.ascontiguousarray(flags))
+    vol = int(np.prod(r.shape[1:]))
+    rng = np.random.default_rng(seed)
+    history = []
+    t0 = time.time()
+    last = t0
+    try:
+        # batches are contiguous runs of a shuffled copy, uploaded once per epoch
+        for epoch in range(epochs):
+            order = rng.permutation(count)
+            hip.free(d_r)
+            hip.free(d_f)
+            d_r = hip.upload(np.ascontiguousarray(r[order]))
+            d_f = hip.upload(np.ascontiguousarray(flags[order]))
+            for b0 in range(0, count - batch + 1, batch):
+                loss = net.train_step(d_r + 4 * vol * b0, d_f + vol * b0, batch, dims, lr)
+                history.append(loss)
+                now = time.time()
+                if now - last > 10.0:
+                    log(f"train: epoch {epoch} batch {b0 // batch} loss {np.mean(history[-20:]):.4f} ({now - t0:.0f} s)", flush=True)
+                    last = now
+                if now - t0 > time_limit:
+                    return history
+    finally:
+        hip.free(d_r)
+        hip.free(d_f)
+    return history
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--data", default="/tmp/smoke_training_pairs.bin")
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--crop", type=int, default=64)
+    ap.add_argument("--crops-per-step", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--reuse-data", action="store_true")
+    args = ap.parse_args()
+    path = Path(args.data)
+    if not (args.reuse_data and path.exists()):
+        generate(path, args.grid, args.steps, args.crop, args.crops_per_step, args.seed)
+    data = td.read(path)
+    print(f"data: {len(data['scale'])} crops of {data['crop']} from a {data['grid']} grid", flush=True)
+    with abi.LearnedPreconditioner(seed=args.seed) as net:
+        hist = train(net, data, args.batch, args.lr, args.epochs, args.time_limit, args.seed)
+        net.save(args.out)
+    print(f"trained: {len(hist)} Adam steps, loss {np.mean(hist[:10]):.4f} -> {np.mean(hist[-10:]):.4f}; weights in {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
