@@ -29,7 +29,9 @@ def _require_gpu():
 
 # (grid, candidate PCG shapes of its class, table shapes of passes 2 and 3)
 CLASSES = {
-    "float4_wide": ((128, 40, 24), {(64, 4, 2, 4), (64, 8, 2, 4), (32, 8, 2, 4), (64, 4, 4, 4)}, ((64, 4, 2, 4), (64, 8, 2, 4))),
+    "float4_wide": ((256, 20, 12), {(64, 4, 2, 4), (64, 8, 2, 4), (32, 8, 2, 4), (64, 4, 4, 4)}, ((64, 4, 2, 4), (64, 8, 2, 4))),
+    # nx not a multiple of 256: the table takes 32-thread (128-cell) rows
+    "float4_128": ((128, 40, 24), {(64, 4, 2, 4), (64, 8, 2, 4), (32, 8, 2, 4), (64, 4, 4, 4)}, ((32, 8, 2, 4), (32, 8, 2, 4))),
     "float4_narrow": ((36, 28, 20), {(16, 16, 2, 4), (16, 8, 2, 4)}, ((16, 16, 2, 4), (16, 16, 2, 4))),
     "scalar_rows": ((45, 30, 22), {(64, 4, 2, 1), (64, 8, 2, 1), (32, 8, 2, 1)}, ((64, 4, 2, 1), (64, 4, 2, 1))),
 }
@@ -95,6 +97,6 @@ def test_autotune_cache_reset_and_fixed_mode_graph():
                 assert rel_l2(x[f], y[f]) < 1e-5, f
             b.reset_launches()
             r = b.launches()
-            assert (r.pcg_curvature.tuple(), r.pcg_update.tuple()) == ((64, 4, 2, 4), (64, 8, 2, 4))
+            assert (r.pcg_curvature.tuple(), r.pcg_update.tuple()) == ((32, 8, 2, 4), (32, 8, 2, 4))
             assert [r.advect[k].tuple() for k in range(4)] == ADVECT_TABLE
             b.step()

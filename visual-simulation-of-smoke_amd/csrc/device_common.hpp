@@ -226,6 +226,19 @@ __device__ __forceinline__ float monotonic_cubic(float p0, float p1, float p2, f
     return (2.0f * t3 - 3.0f * t2 + 1.0f) * p1 + (t3 - 2.0f * t2 + t) * m1 + (-2.0f * t3 + 3.0f * t2) * p2 + (t3 - t2) * m2;
 }
 
+// fmodf(p, extent) then + extent when negative (the periodic wrap of the
+// reference's samplers and hit test), with one subtraction or addition when
+// |p| < 2 extent, which gives the same value: for extent <= p < 2 extent,
+// p - extent is exact (Sterbenz) and equals fmodf's exact result; for
+// -extent < p < 0, fmodf returns p itself.
+__device__ __forceinline__ float wrap_position(float p, float extent) {
+    if (p >= 0.0f && p < extent) return p;
+    if (p >= extent && p < 2.0f * extent) return p - extent;
+    if (p < 0.0f && p > -extent) return p + extent;
+    p = fmodf(p, extent);
+    return p < 0.0f ? p + extent : p;
+}
+
 // Sampling position -> base index and fraction on each axis.  `comp` is the
 // staggered axis of a face component, -1 for cell-centred data; `periodic`
 // selects the fmod wrap (B:394-410 and the other samplers' preambles).
@@ -240,9 +253,7 @@ __device__ __forceinline__ Stencil locate(const Grid& g, const int periodic[3], 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (periodic[a]) {
-            const float extent = float(g.n[a]) * g.h;
-            p[a] = fmodf(p[a], extent);
-            if (p[a] < 0.0f) p[a] += extent;
+            p[a] = wrap_position(p[a], float(g.n[a]) * g.h);
         }
         // Beyond two cells outside the domain every tap is the same ghost
         // value, so clamping there changes no result; it keeps b +- 2 (and a
@@ -352,11 +363,7 @@ __device__ __forceinline__ bool hits_solid(const Grid& g, const uint8_t* __restr
     float q[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (g.flow_periodic[a]) {
-            const float extent = float(g.n[a]) * g.h;
-            q[a] = fmodf(q[a], extent);
-            if (q[a] < 0.0f) q[a] += extent;
-        }
+        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]) * g.h);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)

@@ -755,6 +755,9 @@ void table_pcg_shapes(Context& c) {
     stencil_candidates(c, first, last);
     c.pcg_shape[0] = first;                                 // 64x4x2 / 16x16x2 / scalar 64x4x2
     c.pcg_shape[1] = first == 0 ? 1 : first;                // 64x8x2 for the update pass on wide grids
+    // a float4 tile row of 64 threads covers 256 x cells: below 256 (or not a
+    // multiple of it) a 32-thread row (128 cells) leaves fewer threads idle
+    if (first == 0 && c.grid.n[0] % 256 != 0) c.pcg_shape[0] = c.pcg_shape[1] = 2;   // 32x8x2
 }
 
 void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out) {
