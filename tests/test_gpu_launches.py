@@ -35,7 +35,8 @@ CLASSES = {
     "float4_narrow": ((36, 28, 20), {(16, 16, 2, 4), (16, 8, 2, 4)}, ((16, 16, 2, 4), (16, 16, 2, 4))),
     "scalar_rows": ((45, 30, 22), {(64, 4, 2, 1), (64, 8, 2, 1), (32, 8, 2, 1)}, ((64, 4, 2, 1), (64, 4, 2, 1))),
 }
-ADVECT_BLOCKS = {(64, 4, 1, 1), (32, 8, 1, 1), (16, 16, 1, 1), (32, 4, 2, 1), (64, 2, 2, 1), (128, 2, 1, 1)}
+ADVECT_BLOCKS = {(64, 4, 1, 1), (32, 8, 1, 1), (16, 16, 1, 1), (32, 4, 2, 1), (64, 2, 2, 1), (128, 2, 1, 1), (8, 8, 4, 1), (8, 8, 8, 1),
+                 (16, 4, 4, 1), (8, 16, 2, 1)}
 ADVECT_TABLE = [(16, 16, 1, 1), (128, 2, 1, 1), (128, 2, 1, 1), (128, 2, 1, 1)]
 
 

@@ -23,7 +23,8 @@ const dim3 kCellBlock(64, 4, 1);
 // cubic advection): 16 x 16 x 1 for u (0.60 ms against 0.72 ms with 64 x 4 x 1),
 // 128 x 2 x 1 for v, w and the scalars (1.08, 1.13 and 2.34 ms against 1.52,
 // 1.56 and 3.11 ms).
-const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1)};
+const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
+                              dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
 constexpr int kAdvectTable[4] = {2, 5, 5, 5};
 
