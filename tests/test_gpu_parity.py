@@ -100,6 +100,28 @@ def test_stage_pressure_solve(name):
     assert rel_l2(p64, s["pressure"]) < 1e-3
 
 
+@pytest.mark.parametrize("n", [(20, 24, 2), (20, 24, 1), (2, 24, 18), (1, 24, 18), (8, 24, 2)])
+def test_stage_pressure_solve_thin_periodic_axes(n):
+    # periodic axes of one or two cells drop the duplicate couplings
+    # (B:1343-1412); the benchmark boundaries are periodic in x and z
+    cfg = cfg_for("benchmark", n=n)
+    st = make_state(cfg, 9)
+    st["occupancy"][:, 9:12, : max(1, n[0] // 2)] = 1   # solid rows: identity rows in every plane
+    ctx = abi.SmokeContext(cfg)
+    with ctx:
+        load_gpu(ctx, st)
+        tolerance_options(ctx, tol=1e-6, warm=0)
+        ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_PRESSURE)
+        rhs = ctx.view("FLOW_PRESSURE_RHS")
+        p64 = ctx.export_pressure_f64()
+        stats = ctx.stats()
+    r = rhs.astype(np.float64) - orc.pressure_apply(cfg, st["occupancy"], p64)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(rhs.astype(np.float64))
+    assert stats.converged == 1
+    assert true_rel <= 1.0e-6 * 1.01, true_rel
+    assert abs(true_rel - stats.relative_residual) <= 1e-3 * true_rel + 1e-12
+
+
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_stage_projection(name):
     cfg = cfg_for(name)

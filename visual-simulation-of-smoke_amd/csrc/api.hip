@@ -338,8 +338,12 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         }
         for (float** p : {&c.density, &c.temperature, &c.density_source, &c.temperature_source, &c.density_tmp, &c.temperature_tmp,
                           &c.solid_temperature, &c.occupancy_float, &c.vorticity_magnitude, &c.velocity_magnitude, &c.divergence, &c.pressure,
-                          &c.pressure_rhs, &c.pcg_r, &c.pcg_p, &c.pcg_e})
+                          &c.pressure_rhs, &c.pcg_r, &c.pcg_e})
             smoke::alloc(*p, cells);
+        // the search direction is followed by one plane of zeros, read by the
+        // stencil passes for neighbours outside the domain
+        const int64_t plane = int64_t(c.grid.n[0]) * c.grid.n[1];
+        smoke::alloc(c.pcg_p, cells + plane);
         smoke::alloc(c.occupancy, cells);
         smoke::alloc(c.stencil_flags, cells);
         smoke::alloc(c.anchor, 1);
@@ -370,8 +374,9 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
                 smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
         }
         for (float* p : {c.density_source, c.temperature_source, c.occupancy_float, c.vorticity_magnitude, c.velocity_magnitude, c.divergence,
-                         c.pressure, c.pressure_rhs, c.pcg_r, c.pcg_p, c.pcg_e})
+                         c.pressure, c.pressure_rhs, c.pcg_r, c.pcg_e})
             smoke::enqueue_fill(c, p, 0.0f, c.grid.cells);
+        smoke::enqueue_fill(c, c.pcg_p, 0.0f, c.grid.cells + plane);
         smoke::enqueue_fill(c, c.density, desc->initial_density, c.grid.cells);
         smoke::enqueue_fill(c, c.density_tmp, desc->initial_density, c.grid.cells);
         smoke::enqueue_fill(c, c.temperature, desc->initial_temperature, c.grid.cells);

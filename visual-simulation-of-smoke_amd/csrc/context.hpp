@@ -89,7 +89,7 @@ struct Context {
 
     // pressure solve workspace
     float* pcg_r = nullptr;
-    float* pcg_p = nullptr;            // search direction, updated in place
+    float* pcg_p = nullptr;            // search direction, updated in place; one plane of zeros follows the cells
     float* pcg_e = nullptr;
     float* pcg_z = nullptr;              // flexible PCG: preconditioned residual (allocated with the network)
     SmokeNnPreconditioner* learned = nullptr;   // attached learned preconditioner, not owned

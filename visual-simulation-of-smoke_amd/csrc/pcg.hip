@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace smoke {
@@ -36,6 +37,31 @@ namespace {
 constexpr uint8_t kIdentity = 0x40;
 constexpr int kResidualPlanes = 8;   // z planes per block of the FP64 residual kernel
 
+// Division by a divisor d >= 1 known at launch, without a divide
+// instruction: q = (t + ((n - t) >> s1)) >> s2 with t = mulhi(n, m), exact for
+// every 32-bit n (Granlund and Montgomery, "Division by invariant integers
+// using multiplication", 1994, in the form that keeps m at 32 bits by
+// splitting the shift).  For a uniform n it runs on the scalar unit.
+struct Divisor {
+    unsigned m;
+    int s1, s2;
+};
+
+Divisor make_divisor(unsigned d) {
+    int l = 0;   // ceil(log2 d)
+    while ((uint64_t(1) << l) < d) ++l;
+    Divisor v;
+    v.m = unsigned(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    v.s1 = l > 0 ? 1 : 0;
+    v.s2 = l > 0 ? l - 1 : 0;
+    return v;
+}
+
+__device__ __forceinline__ unsigned divide(unsigned n, const Divisor& v) {
+    const unsigned t = __umulhi(n, v.m);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
 struct PcgArgs {
     int n[3];
     int periodic[3];
@@ -43,21 +69,28 @@ struct PcgArgs {
     unsigned skip;        // off-diagonal directions ignored (periodic axes with n <= 2)
     int any_outflow;
     int tiles[3];         // tile grid of a stencil pass (x tiles, y tiles, z runs)
+    Divisor tile_div[2];  // division by tiles[2] and by tiles[1]
     const uint8_t* flags;
     PcgScalars* s;
     double* partials;
 };
 
-__device__ __forceinline__ float diag_of(const PcgArgs& a, uint8_t f, int x, int y, int z) {
-    if (f & kIdentity) return 1.0f;
-    int d = __popc(unsigned(f & 0x3f));
-    if (a.any_outflow) {
-        d += (x == 0 && a.outflow[0]) + (x == a.n[0] - 1 && a.outflow[1]);
-        d += (y == 0 && a.outflow[2]) + (y == a.n[1] - 1 && a.outflow[3]);
-        d += (z == 0 && a.outflow[4]) + (z == a.n[2] - 1 && a.outflow[5]);
-    }
-    return d > 0 ? float(d) : 1.0f;
+// OUTFLOW faces of cell (x, y, z) (each adds 1 to the diagonal of a fluid row).
+__device__ __forceinline__ int outflow_faces(const PcgArgs& a, int x, int y, int z) {
+    if (!a.any_outflow) return 0;
+    return (x == 0 && a.outflow[0]) + (x == a.n[0] - 1 && a.outflow[1]) + (y == 0 && a.outflow[2]) + (y == a.n[1] - 1 && a.outflow[3]) +
+           (z == 0 && a.outflow[4]) + (z == a.n[2] - 1 && a.outflow[5]);
 }
+
+// Diagonal of the row with flag byte f: fluid neighbours plus OUTFLOW faces,
+// at least 1.  An identity row's byte is exactly kIdentity (no neighbour
+// bits), so its bit count is already its diagonal 1 and `extra` is ignored.
+__device__ __forceinline__ float diag_from(unsigned f, int extra) {
+    const int d = __popc(f) + ((f & kIdentity) ? 0 : extra);
+    return float(d > 1 ? d : 1);
+}
+
+__device__ __forceinline__ float diag_of(const PcgArgs& a, uint8_t f, int x, int y, int z) { return diag_from(f, outflow_faces(a, x, y, z)); }
 
 // Resolves a possibly out-of-range cell: periodic axes wrap, otherwise the
 // cell does not exist (its value is never used by the stencil).
@@ -160,33 +193,21 @@ __device__ __forceinline__ double guarded(double num, double den, bool fixed) {
     return fabs(den) > lim ? num / den : 0.0;
 }
 
-// One coordinate of a neighbour made safe to load from.  Threads that own a
-// cell ask for coordinates in [-1, n]; the padding threads of a partial tile
-// may ask for more, so the coordinate is first clamped to [-1, n] (their
-// values are never used).  A periodic axis then wraps by one period and a
-// closed axis clamps; `ok` is cleared when the neighbour does not exist.
-// No integer division: this runs for every neighbour load of the stencil
-// passes.
-__device__ __forceinline__ int safe_axis(int v, int n, int periodic, bool& ok) {
-    v = v < -1 ? -1 : (v > n ? n : v);
-    if (v < 0) {
-        ok = ok && periodic;
-        return periodic ? v + n : 0;
-    }
-    if (v >= n) {
-        ok = ok && periodic;
-        return periodic ? v - n : n - 1;
-    }
+// Row (or column, or plane) to load the neighbour at coordinate v in [-1, n]
+// from: wrapped by one period on a periodic axis; on a closed axis some row
+// that exists, because a neighbour outside the domain has no flag bit and
+// its value is never used.  No integer division.
+__device__ __forceinline__ int neighbour_row(int v, int n, int periodic) {
+    if (v < 0) return periodic ? v + n : 0;
+    if (v >= n) return periodic ? v - n : n - 1;
     return v;
 }
 
-__device__ __forceinline__ int64_t safe_index(const PcgArgs& a, int x, int y, int z, bool& valid) {
-    bool ok = true;
-    x = safe_axis(x, a.n[0], a.periodic[0], ok);
-    y = safe_axis(y, a.n[1], a.periodic[1], ok);
-    z = safe_axis(z, a.n[2], a.periodic[2], ok);
-    valid = ok;
-    return lin(a, x, y, z);
+// v when bit `pos` of w is set, else +0.0f: the bit sign-extended to a mask
+// (one bit-field extract) ANDed with the bits of v, the value of the select
+// `(w >> pos) & 1 ? v : 0.0f` in two instructions.
+__device__ __forceinline__ float if_bit(unsigned w, int pos, float v) {
+    return __int_as_float(__float_as_int(v) & __builtin_amdgcn_sbfe(int(w), pos, 1));
 }
 
 // Tile of this block for the stencil passes: z fastest, consecutive tiles
@@ -195,10 +216,12 @@ __device__ __forceinline__ int64_t safe_index(const PcgArgs& a, int x, int y, in
 // tiles beside it in the same L2.
 __device__ __forceinline__ void stencil_tile(const PcgArgs& a, int& bx, int& by, int& bz) {
     const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
-    const int b = tile_id(nb);
-    bz = b % a.tiles[2];
-    by = (b / a.tiles[2]) % a.tiles[1];
-    bx = b / (a.tiles[2] * a.tiles[1]);
+    const unsigned b = unsigned(tile_id(nb));
+    const unsigned t = divide(b, a.tile_div[0]);
+    const unsigned u = divide(t, a.tile_div[1]);
+    bz = int(b - t * unsigned(a.tiles[2]));
+    by = int(t - u * unsigned(a.tiles[1]));
+    bx = int(u);
 }
 
 // Pass 1 of an iteration: p <- z + beta p in place, z = r / diag (Jacobi)
@@ -241,18 +264,33 @@ __global__ __launch_bounds__(256) void pcg_update_p_kernel(PcgArgs a, const floa
 // Stencil passes on the stored direction p, VEC cells per thread along x
 // (float4 when nx % 4 == 0), ZR planes per thread, neighbours read through
 // the cache (no LDS, no barriers: every load of a thread is independent):
-//   MODE 0 (pass 2): partial p.Ap; the last block sets alpha = r.z / p.Ap.
-//                    Reads p, flags (5 B/cell).
-//   MODE 1 (pass 3): e += alpha p, r -= alpha Ap, partials r.z and r.r; the
-//                    last block sets beta, counts the iteration and raises
-//                    `done`.  Reads p, flags, e, r; writes e, r (21 B/cell).
+//   MODE 0 (pass 2): partial p.Ap per block.  Reads p, flags (5 B/cell).
+//   MODE 1 (pass 3): e += alpha p, r -= alpha Ap, partials r.z and r.r per
+//                    block.  Reads p, flags, e, r; writes e, r (21 B/cell).
+//   MODE 2 (flexible PCG update): as MODE 1, with the partial r.z taken
+//                    against the stored previous preconditioned residual z
+//                    (r_(k+1) . z_k, for the Polak-Ribiere beta).
+// Addressing is 32-bit (fewer than 2^31 cells): the z planes of a block are
+// uniform, and each lane resolves its row, its y neighbours and its two x
+// edge cells once; every load is then one add from those offsets.
+//
+// Without skipped directions (every periodic axis longer than 2 cells) the
+// operator needs no per-cell neighbour masks.  p is exactly +0 on identity
+// rows: the right-hand side and the FP64 iterate are 0 there, so r, z
+// (Jacobi or learned) and every direction stay +0, and subtracting the p of
+// a solid neighbour subtracts +0 -- the same value, in the same order, as
+// leaving the neighbour out.  A neighbour outside the domain is read from a
+// zero plane stored after p (elements cells .. cells + nx ny - 1), chosen
+// per lane (y rows, x edge cells) or per block (z planes).  Only the r
+// update masks the identity rows themselves.  With skipped directions each
+// neighbour is masked by its flag bit instead.
 template <int VEC>
 struct Row {
     float v[VEC];
 };
 
 template <int VEC>
-__device__ __forceinline__ Row<VEC> load_row(const float* __restrict__ f, int64_t i) {
+__device__ __forceinline__ Row<VEC> load_row(const float* __restrict__ f, unsigned i) {
     Row<VEC> o;
     if constexpr (VEC == 4) {
         const float4 q = *reinterpret_cast<const float4*>(f + i);
@@ -267,14 +305,11 @@ __device__ __forceinline__ Row<VEC> load_row(const float* __restrict__ f, int64_
 }
 
 template <int VEC>
-__device__ __forceinline__ void store_row(float* __restrict__ f, int64_t i, const Row<VEC>& o) {
+__device__ __forceinline__ void store_row(float* __restrict__ f, unsigned i, const Row<VEC>& o) {
     if constexpr (VEC == 4) *reinterpret_cast<float4*>(f + i) = make_float4(o.v[0], o.v[1], o.v[2], o.v[3]);
     else f[i] = o.v[0];
 }
 
-//   MODE 2 (flexible PCG update): as MODE 1, with the partial r.z taken
-//                    against the stored previous preconditioned residual z
-//                    (r_(k+1) . z_k, for the Polak-Ribiere beta).
 template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
 __global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r,
                                                               const float* __restrict__ zold) {
@@ -284,97 +319,184 @@ __global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const f
     const int nblocks = a.tiles[0] * a.tiles[1] * a.tiles[2];
     int bx, by, bz;
     stencil_tile(a, bx, by, bz);
+    const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
     const int xv = (bx * TXV + int(threadIdx.x) % TXV) * VEC;
     const int y = by * TY + int(threadIdx.x) / TXV;
     const int z0 = bz * ZR;
-    const bool owner = xv < a.n[0] && y < a.n[1];
-    const int xs = owner ? xv : 0;   // safe x for padding threads
+    const bool owner = xv < nx && y < ny;
+    // a padding thread of a partial tile reads the cells of (0, 0): in bounds,
+    // never used
+    const int xs = owner ? xv : 0;
+    const int ys = owner ? y : 0;
     const bool first = S->first != 0;
     const float alpha = float(S->alpha);
     const uint8_t* __restrict__ flags = a.flags;
 
-    // a row segment of p at (xs.., cy, cz): zero when the row does not exist
-    auto prow = [&](int cy, int cz) {
-        bool ok;
-        const int64_t i = safe_index(a, xs, cy, cz, ok);
-        Row<VEC> q = load_row<VEC>(p, i);
-        if (!ok) {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) q.v[k] = 0.0f;
-        }
-        return q;
-    };
-    auto pcell = [&](int cx, int cy, int cz) {
-        bool ok;
-        const float v = p[safe_index(a, cx, cy, cz, ok)];
-        return ok ? v : 0.0f;
-    };
+    const unsigned plane = unsigned(nx) * unsigned(ny);
+    const unsigned zero = plane * unsigned(nz);   // first element of the zero plane after p
+    const unsigned row = unsigned(ys) * unsigned(nx);
+    const unsigned cen = row + unsigned(xs);
+    const unsigned ym = unsigned(neighbour_row(ys - 1, ny, a.periodic[1])) * unsigned(nx) + unsigned(xs);
+    const unsigned yp = unsigned(neighbour_row(ys + 1, ny, a.periodic[1])) * unsigned(nx) + unsigned(xs);
+    const unsigned xl = row + unsigned(neighbour_row(xs - 1, nx, a.periodic[0]));
+    const unsigned xr = row + unsigned(neighbour_row(xs + VEC, nx, a.periodic[0]));
 
     Row<VEC> col[ZR + 2];
-#pragma unroll
-    for (int j = 0; j < ZR + 2; ++j) col[j] = prow(y, z0 - 1 + j);
     double acc0 = 0.0, acc1 = 0.0;
-#pragma unroll
-    for (int j = 0; j < ZR; ++j) {
-        const int z = z0 + j;
-        const Row<VEC> ym = prow(y - 1, z), yp = prow(y + 1, z);
-        const float lft = pcell(xs - 1, y, z), rgt = pcell(xs + VEC, y, z);
-        bool ok;
-        const int64_t i = safe_index(a, xs, y, z, ok);
-        uint8_t fv[VEC];
-        if constexpr (VEC == 4) {
-            const uchar4 f4 = *reinterpret_cast<const uchar4*>(flags + i);
-            fv[0] = f4.x;
-            fv[1] = f4.y;
-            fv[2] = f4.z;
-            fv[3] = f4.w;
-        } else {
-            fv[0] = flags[i];
-        }
-        Row<VEC> ev{}, rv{}, zv{};
+
+    // Everything a plane reads besides the column of p, loaded for all ZR
+    // planes before any arithmetic so that a thread's loads are in flight
+    // together.
+    struct Plane {
+        Row<VEC> rm, rp, ev, rv, zv;
+        float lft, rgt;
+        unsigned w, i;
+    };
+    auto load_rest = [&](Plane& q, unsigned i, unsigned ym_at, unsigned yp_at, unsigned xl_at, unsigned xr_at) {
+        q.i = i;
+        q.rm = load_row<VEC>(p, ym_at);
+        q.rp = load_row<VEC>(p, yp_at);
+        q.lft = p[xl_at];
+        q.rgt = p[xr_at];
+        if constexpr (VEC == 4) q.w = *reinterpret_cast<const unsigned*>(flags + i);
+        else q.w = flags[i];
         if constexpr (MODE >= 1) {
-            ev = load_row<VEC>(e, i);
-            rv = load_row<VEC>(r, i);
+            q.ev = load_row<VEC>(e, i);
+            q.rv = load_row<VEC>(r, i);
         }
-        if constexpr (MODE == 2) zv = load_row<VEC>(zold, i);
-        const bool active = owner && z < a.n[2];
+        if constexpr (MODE == 2) q.zv = load_row<VEC>(zold, i);
+    };
+
+    // Plane j of the tile from its VEC products ap (+0 on identity rows) and
+    // diagonals d.
+    auto finish = [&](int j, const Plane& q, const float (&ap)[VEC], const float (&d)[VEC]) {
+        const int z = z0 + j;
+        const bool active = owner && z < nz;
         Row<VEC> en, rn;
         double s0 = 0.0, s1 = 0.0;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            // identity rows: d = 1 and no off-diagonal bits, so A p = p
-            const bool ident = (fv[k] & kIdentity) != 0;
-            const float d = ident ? 1.0f : diag_of(a, fv[k], xs + k, y, z);
-            const unsigned m = ident ? 0u : (unsigned(fv[k]) & ~a.skip);
             const float c = col[j + 1].v[k];
-            const float xm = k == 0 ? lft : col[j + 1].v[k - 1 < 0 ? 0 : k - 1];
-            const float xp = k == VEC - 1 ? rgt : col[j + 1].v[k + 1 < VEC ? k + 1 : VEC - 1];
-            float ap = d * c;
-            ap -= (m & 1u) ? xm : 0.0f;
-            ap -= (m & 2u) ? xp : 0.0f;
-            ap -= (m & 4u) ? ym.v[k] : 0.0f;
-            ap -= (m & 8u) ? yp.v[k] : 0.0f;
-            ap -= (m & 16u) ? col[j].v[k] : 0.0f;
-            ap -= (m & 32u) ? col[j + 2].v[k] : 0.0f;
             if constexpr (MODE == 0) {
-                s0 += double(c) * double(ap);
+                s0 += double(c) * double(ap[k]);
             } else {
-                en.v[k] = (first ? 0.0f : ev.v[k]) + alpha * c;
-                rn.v[k] = rv.v[k] - alpha * ap;
-                const float zk = MODE == 2 ? zv.v[k] : (JACOBI ? rn.v[k] / d : rn.v[k]);
+                en.v[k] = (first ? 0.0f : q.ev.v[k]) + alpha * c;
+                rn.v[k] = q.rv.v[k] - alpha * ap[k];
+                const float zk = MODE == 2 ? q.zv.v[k] : (JACOBI ? rn.v[k] / d[k] : rn.v[k]);
                 s0 += double(rn.v[k]) * double(zk);
                 s1 += double(rn.v[k]) * double(rn.v[k]);
             }
         }
         if constexpr (MODE >= 1) {
+            // e + alpha p taken before the branch: otherwise the compiler sinks
+            // the e load into it, after the stencil's loads have all returned
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) asm volatile("" ::"v"(en.v[k]));
             if (active) {
-                store_row<VEC>(e, i, en);
-                store_row<VEC>(r, i, rn);
+                store_row<VEC>(e, q.i, en);
+                store_row<VEC>(r, q.i, rn);
             }
         }
-        acc0 += active ? s0 : 0.0;
-        acc1 += active ? s1 : 0.0;
-    }
+        // a product, not a select: the loads stay unconditional (a padding
+        // thread's cells are in bounds and finite, so keep = 0 adds +0)
+        const double keep = active ? 1.0 : 0.0;
+        acc0 += s0 * keep;
+        acc1 += s1 * keep;
+    };
+
+    // No skipped direction.  OUT: OUTFLOW faces add to the diagonals.
+    auto unmasked = [&](auto out_tag) {
+        constexpr bool OUT = decltype(out_tag)::value;
+        const bool hxm = xs > 0 || a.periodic[0], hxp = xs + VEC < nx || a.periodic[0];
+        const bool hym = ys > 0 || a.periodic[1], hyp = ys < ny - 1 || a.periodic[1];
+        unsigned zoff[ZR + 2];   // planes z0 - 1 .. z0 + ZR, the zero plane where none exists
+#pragma unroll
+        for (int j = 0; j < ZR + 2; ++j) {
+            const int v = z0 - 1 + j;
+            unsigned off = zero;
+            if (v >= 0 && v < nz) off = unsigned(v) * plane;
+            else if (a.periodic[2] && v == -1) off = unsigned(nz - 1) * plane;
+            else if (a.periodic[2] && v == nz) off = 0;
+            zoff[j] = off;
+        }
+        Plane q[ZR];
+#pragma unroll
+        for (int j = 0; j < ZR + 2; ++j) col[j] = load_row<VEC>(p, zoff[j] + cen);
+#pragma unroll
+        for (int j = 0; j < ZR; ++j) {
+            const unsigned zp = zoff[j + 1];
+            // flags, e, r of a padding plane: the last plane (in bounds, unused)
+            load_rest(q[j], unsigned(min(z0 + j, nz - 1)) * plane + cen, hym ? zp + ym : zero + unsigned(xs), hyp ? zp + yp : zero + unsigned(xs),
+                      hxm ? zp + xl : zero, hxp ? zp + xr : zero);
+        }
+#pragma unroll
+        for (int j = 0; j < ZR; ++j) {
+            float ap[VEC], d[VEC];
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                const unsigned f = (q[j].w >> (8 * k)) & 0xffu;
+                if constexpr (OUT) d[k] = diag_from(f, outflow_faces(a, xs + k, ys, z0 + j));
+                else d[k] = diag_from(f, 0);
+                const float c = col[j + 1].v[k];
+                const float xm = k == 0 ? q[j].lft : col[j + 1].v[k - 1 < 0 ? 0 : k - 1];
+                const float xp = k == VEC - 1 ? q[j].rgt : col[j + 1].v[k + 1 < VEC ? k + 1 : VEC - 1];
+                float v = d[k] * c;
+                v -= xm;
+                v -= xp;
+                v -= q[j].rm.v[k];
+                v -= q[j].rp.v[k];
+                v -= col[j].v[k];
+                v -= col[j + 2].v[k];
+                // identity row: A p = p = +0 (the p.Ap term c * v is +0 already)
+                if constexpr (MODE >= 1) v = __int_as_float(__float_as_int(v) & ~__builtin_amdgcn_sbfe(int(q[j].w), 8 * k + 6, 1));
+                ap[k] = v;
+            }
+            finish(j, q[j], ap, d);
+        }
+    };
+
+    // Skipped directions: every neighbour masked by its flag bit.
+    auto masked = [&]() {
+        // neighbours that do not exist read some cell that does (never used)
+        const unsigned skip_mask = ~(a.skip * (VEC == 4 ? 0x01010101u : 1u));
+        unsigned zoff[ZR + 2];
+#pragma unroll
+        for (int j = 0; j < ZR + 2; ++j) zoff[j] = unsigned(min(neighbour_row(z0 - 1 + j, nz, a.periodic[2]), nz - 1)) * plane;
+        Plane q[ZR];
+#pragma unroll
+        for (int j = 0; j < ZR + 2; ++j) col[j] = load_row<VEC>(p, zoff[j] + cen);
+#pragma unroll
+        for (int j = 0; j < ZR; ++j) {
+            const unsigned zp = zoff[j + 1];
+            load_rest(q[j], zp + cen, zp + ym, zp + yp, zp + xl, zp + xr);
+        }
+#pragma unroll
+        for (int j = 0; j < ZR; ++j) {
+            const unsigned w = q[j].w;
+            const unsigned wm = w & skip_mask;   // the diagonal counts the skipped bits: it uses w
+            float ap[VEC], d[VEC];
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                d[k] = diag_from((w >> (8 * k)) & 0xffu, outflow_faces(a, xs + k, ys, z0 + j));
+                const float c = col[j + 1].v[k];
+                const float xm = k == 0 ? q[j].lft : col[j + 1].v[k - 1 < 0 ? 0 : k - 1];
+                const float xp = k == VEC - 1 ? q[j].rgt : col[j + 1].v[k + 1 < VEC ? k + 1 : VEC - 1];
+                float v = d[k] * c;   // identity rows: no neighbour bits, d = 1
+                v -= if_bit(wm, 8 * k + 0, xm);
+                v -= if_bit(wm, 8 * k + 1, xp);
+                v -= if_bit(wm, 8 * k + 2, q[j].rm.v[k]);
+                v -= if_bit(wm, 8 * k + 3, q[j].rp.v[k]);
+                v -= if_bit(wm, 8 * k + 4, col[j].v[k]);
+                v -= if_bit(wm, 8 * k + 5, col[j + 2].v[k]);
+                ap[k] = v;
+            }
+            finish(j, q[j], ap, d);
+        }
+    };
+
+    if (a.skip != 0) masked();
+    else if (a.any_outflow) unmasked(std::true_type{});
+    else unmasked(std::false_type{});
 
     if constexpr (MODE == 0) {
         double v[1] = {acc0};
@@ -642,6 +764,8 @@ void launch_stencils(Context& c, PcgArgs a, int mode) {
     a.tiles[0] = (c.grid.n[0] + TXV * VEC - 1) / (TXV * VEC);
     a.tiles[1] = (c.grid.n[1] + TY - 1) / TY;
     a.tiles[2] = (c.grid.n[2] + ZR - 1) / ZR;
+    a.tile_div[0] = make_divisor(unsigned(a.tiles[2]));
+    a.tile_div[1] = make_divisor(unsigned(a.tiles[1]));
     const unsigned blocks = unsigned(a.tiles[0] * a.tiles[1] * a.tiles[2]);
     float* e = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE ? c.pcg_e : c.pressure;
     if (mode == 0) pcg_stencil_kernel<TXV, TY, ZR, VEC, 0, JACOBI><<<blocks, TXV * TY, 0, c.stream>>>(a, c.pcg_p, e, c.pcg_r, nullptr);
