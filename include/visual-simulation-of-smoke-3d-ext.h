@@ -47,6 +47,19 @@ typedef enum SmokeSimulationPreconditioner {
        5 iterations, or turns non-finite, the rest of that solve runs
        Jacobi PCG.  The host checks convergence after every iteration. */
     SMOKE_SIMULATION_PRECONDITIONER_LEARNED = 2,
+    /* flexible PCG (Polak-Ribiere beta) preconditioned by one V-cycle of a
+       geometric hierarchy (smoke_simulation_ext_set_multilevel_options):
+       damped-Jacobi smoothing on every level but the coarsest, and at the
+       coarsest level either the learned network attached by
+       smoke_simulation_ext_set_learned_preconditioner (the network is the
+       coarse-grid solver) or damped-Jacobi sweeps only.  FP64 refinement as
+       for SMOKE_SIMULATION_PRECONDITIONER_LEARNED; the solve falls back to
+       Jacobi PCG when, after 20 iterations, the smallest ||r|| of the last 10
+       is not 10 % below the smallest before them, or on a non-finite value
+       (||r|| is not monotone under this preconditioner: it rises over the
+       first iterations while the near-null mode of the anchored operator is
+       built). */
+    SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL = 3,
 } SmokeSimulationPreconditioner;
 
 typedef struct SmokeSimulationSolverOptions {
@@ -230,6 +243,73 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_write_training_c
    context does not own it: keep it alive while attached.  Used when the
    solver options select SMOKE_SIMULATION_PRECONDITIONER_LEARNED. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_learned_preconditioner(SmokeSimulationContext context, void* network);
+
+/* Multilevel preconditioner (SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL).
+
+   Level 0 is the pressure grid; level l+1 has ceil(n/2) cells per axis of
+   level l, and its cell is solid when all of its (up to 8) children are
+   solid; the cell holding the previous level's pressure anchor is the
+   anchor (an identity row) of the next level.  Every level's operator is the
+   7-point matrix of the pressure system rediscretised on that level's
+   occupancy (same boundary kinds; a periodic axis must stay even down to the
+   coarsest level).  One application z = M(r) is a V-cycle:
+     l < L-1: x = `pre_sweeps` damped-Jacobi sweeps from 0;
+              r_(l+1) = 1/2 * (sum over the children of r - A x)   (restriction);
+              z_(l+1) = M_(l+1)(r_(l+1));
+              x += z_(l+1) of the parent cell                   (prolongation);
+              z = `post_sweeps` damped-Jacobi sweeps from x;
+     l = L-1: z = network(r) followed by `coarse_sweeps` sweeps
+              (coarse_solver = NETWORK), or `coarse_sweeps` sweeps from 0
+              (coarse_solver = JACOBI).
+   A damped-Jacobi sweep is x += omega D^-1 (r - A x); identity rows stay 0. */
+typedef enum SmokeSimulationCoarseSolver {
+    SMOKE_SIMULATION_COARSE_NETWORK = 0,
+    SMOKE_SIMULATION_COARSE_JACOBI = 1,
+} SmokeSimulationCoarseSolver;
+
+typedef struct SmokeSimulationMultilevelOptions {
+    int32_t levels;          /* including the pressure grid, >= 2 */
+    int32_t pre_sweeps;      /* >= 1 */
+    int32_t post_sweeps;     /* >= 1 */
+    float omega;             /* damping of the Jacobi sweeps, (0, 1] */
+    uint32_t coarse_solver;  /* SmokeSimulationCoarseSolver */
+    int32_t coarse_sweeps;   /* >= 0 (NETWORK), >= 1 (JACOBI) */
+    int32_t reserved[2];
+} SmokeSimulationMultilevelOptions;
+
+/* Defaults: 4 levels, 2 pre- and 2 post-sweeps, omega 2/3, network coarse
+   solver with no extra sweeps. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_default_multilevel_options(SmokeSimulationMultilevelOptions* out_options);
+/* Validates and stores the options; the hierarchy is (re)built on the next
+   multilevel solve.  Fails when a periodic axis does not halve evenly down to
+   the coarsest level. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_multilevel_options(SmokeSimulationContext context, const SmokeSimulationMultilevelOptions* options);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_multilevel_options(SmokeSimulationContext context, SmokeSimulationMultilevelOptions* out_options);
+
+/* Builds the hierarchy for the current occupancy if needed and reports the
+   dimensions of each level (levels beyond `levels` are zero). */
+typedef struct SmokeSimulationMultilevelInfo {
+    int32_t levels;
+    int32_t dims[8][3];
+    uint32_t captured;       /* coarsest-level right-hand sides stored since the capture was set */
+} SmokeSimulationMultilevelInfo;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_multilevel_info(SmokeSimulationContext context, SmokeSimulationMultilevelInfo* out_info);
+
+/* One application z = M(r) of the V-cycle on device vectors of the pressure
+   grid (stream-ordered).  r must be 0 on identity rows; z is 0 there. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_apply_multilevel(SmokeSimulationContext context, const float* r, float* z);
+
+/* Copies the flag byte of one level (bits as in the training crops) to a
+   device buffer of dims[0]*dims[1]*dims[2] bytes (stream-ordered). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_level_flags(SmokeSimulationContext context, int32_t level, uint8_t* flags);
+
+/* Training data of the coarse-grid network: while set, every V-cycle copies
+   its coarsest-level right-hand side into the next free slot of `buffer`
+   (device memory, `capacity` volumes of the coarsest level's cells, FP32),
+   until it is full.  NULL stops the capture; setting a buffer restarts the
+   count. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_multilevel_capture(SmokeSimulationContext context, float* buffer, uint32_t capacity);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,20 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_train_step(Sm
                                                                               int32_t batch, const int32_t dims[3], float lr, double* loss,
                                                                               float* host_gradients, void* stream);
 
+/* Training loss of later train_step calls:
+     SMOKE_NN_LOSS_RESIDUAL (default): mean_b ||A z_b - r_b||^2 / ||r_b||^2;
+     SMOKE_NN_LOSS_ENERGY: mean_b (z_b^T A z_b / 2 - z_b^T r_b) / ||r_b||^2, whose
+       minimiser is also z = A^-1 r but which weighs the error e = z - A^-1 r
+       by the energy norm e^T A e (what conjugate gradients minimise) instead
+       of by |A e|^2, so smooth (small-eigenvalue) error is not discounted --
+       the loss for a coarse-grid solver.  Its value is negative at the optimum. */
+typedef enum SmokeNnLoss {
+    SMOKE_NN_LOSS_RESIDUAL = 0,
+    SMOKE_NN_LOSS_ENERGY = 1,
+} SmokeNnLoss;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_set_loss(SmokeNnPreconditioner* net, uint32_t loss);
+
 /* z = M(r) for one volume of dims (W, H, D) (FP32 device arrays, flags uint8). */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_apply(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags,
                                                                          const int32_t dims[3], float* z, void* stream);

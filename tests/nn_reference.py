@@ -129,11 +129,12 @@ def apply_A(v, flags):
     return np.where(ident, 0.0, out)
 
 
-def unet_loss_and_gradients(flat, r, flags, rounding=True):
-    """Loss mean_b ||A z - r||^2 / ||r||^2 and the parameter gradients of the
-    preconditioner network, rounding every stored tensor to bfloat16 where the
-    GPU does (rounding=False: exact float64 throughout, for finite-difference
-    checks).  r, flags: (B, D, H, W)."""
+def unet_loss_and_gradients(flat, r, flags, rounding=True, loss="residual"):
+    """Loss mean_b ||A z - r||^2 / ||r||^2 (loss="residual") or
+    mean_b (z^T A z / 2 - z^T r) / ||r||^2 over the fluid rows (loss="energy")
+    and the parameter gradients of the preconditioner network, rounding every
+    stored tensor to bfloat16 where the GPU does (rounding=False: exact
+    float64 throughout, for finite-difference checks).  r, flags: (B, D, H, W)."""
     bf16_ = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     f32 = (lambda a: np.asarray(a, np.float32).astype(np.float64)) if rounding else (lambda a: np.asarray(a, np.float64))
     P = split_parameters(flat)
@@ -170,9 +171,15 @@ def unet_loss_and_gradients(flat, r, flags, rounding=True):
     z = f32(conv(10, T["c0u"])[..., 0])
     q = np.where(ident, 0.0, apply_A(z, flags) - rn)
     rr = (rn ** 2).reshape(B, -1).sum(1)
-    qq = (q ** 2).reshape(B, -1).sum(1)
-    loss = float(np.mean(qq / rr))
-    g = (2.0 / B) * apply_A(q, flags) / rr[:, None, None, None]
+    if loss == "energy":
+        zf = np.where(ident, 0.0, z)
+        ee = (0.5 * zf * apply_A(zf, flags) - zf * rn).reshape(B, -1).sum(1)
+        loss = float(np.mean(ee / rr))
+        g = (1.0 / B) * q / rr[:, None, None, None]
+    else:
+        qq = (q ** 2).reshape(B, -1).sum(1)
+        loss = float(np.mean(qq / rr))
+        g = (2.0 / B) * apply_A(q, flags) / rr[:, None, None, None]
     gout = np.zeros((B, D, H, W, 16))
     gout[..., 0] = bf16_(g.astype(np.float32) if rounding else g)
 

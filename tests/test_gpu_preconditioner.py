@@ -128,3 +128,49 @@ def test_training_step_matches_float64_network(lib, tmp_path):
         for p in ptrs:
             hip.free(p)
         L.smoke_nn_preconditioner_destroy(net)
+
+
+def test_energy_loss_training_step_matches_float64_network(lib):
+    """The energy loss (smoke_nn_preconditioner_set_loss): loss value and
+    every layer's gradients against the float64 restatement."""
+    L, hip = lib
+    L.smoke_nn_preconditioner_set_loss.argtypes = [C.c_void_p, C.c_uint32]
+    L.smoke_nn_preconditioner_set_loss.restype = C.c_int
+    net = C.c_void_p()
+    assert L.smoke_nn_preconditioner_create(11, C.byref(net)) == 0
+    ptrs = []
+    try:
+        assert L.smoke_nn_preconditioner_set_loss(net, 2) != 0
+        assert L.smoke_nn_preconditioner_set_loss(net, 1) == 0
+        n = L.smoke_nn_preconditioner_parameter_count(net)
+        params = np.empty(n, np.float32)
+        assert L.smoke_nn_preconditioner_get_parameters(net, params.ctypes.data) == 0
+        batch, dims = 2, (16, 12, 8)
+        r, flags = volumes(batch, dims, 5)
+        d_r, d_f = hip.upload(r), hip.upload(flags)
+        ptrs += [d_r, d_f]
+        cd = (C.c_int32 * 3)(*dims)
+        loss = C.c_double()
+        grads = np.empty(n, np.float32)
+        assert L.smoke_nn_preconditioner_train_step(net, d_r, d_f, batch, cd, 0.0, C.byref(loss), grads.ctypes.data, None) == 0
+        ref_loss, ref_grads = ref.unet_loss_and_gradients(params, r, flags, loss="energy")
+        assert abs(loss.value - ref_loss) <= 1e-3 * abs(ref_loss) + 1e-6, (loss.value, ref_loss)
+        off = 0
+        for li, (_, cin, cout, *_) in enumerate(ref.LAYERS):
+            for size in (27 * cin * cout, cout):
+                g, rg = grads[off:off + size], ref_grads[off:off + size]
+                off += size
+                norm = np.linalg.norm(rg)
+                if norm == 0:
+                    assert np.all(g == 0)
+                else:
+                    # 5 %: bfloat16 tie flips between the two summation orders
+                    assert np.linalg.norm(g - rg) <= 5e-2 * norm, (li, size, np.linalg.norm(g - rg) / norm)
+        first = loss.value
+        for _ in range(30):
+            assert L.smoke_nn_preconditioner_train_step(net, d_r, d_f, batch, cd, 1e-3, C.byref(loss), None, None) == 0
+        assert loss.value < first - 0.1 * abs(first), (first, loss.value)
+    finally:
+        for p in ptrs:
+            hip.free(p)
+        L.smoke_nn_preconditioner_destroy(net)

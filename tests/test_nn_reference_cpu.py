@@ -53,7 +53,8 @@ def test_forward_matches_brute_force_and_adjoints_hold(mode, size):
     np.testing.assert_allclose(db, dy.sum(axis=(0, 1, 2, 3)))
 
 
-def test_network_gradients_match_finite_differences():
+@pytest.mark.parametrize("loss_kind", ["residual", "energy"])
+def test_network_gradients_match_finite_differences(loss_kind):
     """The float64 network restatement (rounding off) against central
     differences of its own loss, for parameters of every layer."""
     import sys
@@ -68,7 +69,7 @@ def test_network_gradients_match_finite_differences():
     for (_, ci, co, *_), n in zip(ref.LAYERS, sizes):
         params[off:off + n] = rng.normal(size=n) * np.sqrt(2.0 / (27 * ci))
         off += n
-    loss, grads = ref.unet_loss_and_gradients(params, r, flags, rounding=False)
+    loss, grads = ref.unet_loss_and_gradients(params, r, flags, rounding=False, loss=loss_kind)
     off = 0
     for li, n in enumerate(sizes):
         # the largest-gradient weight of the layer and its first bias
@@ -79,6 +80,39 @@ def test_network_gradients_match_finite_differences():
             p1, p2 = params.copy(), params.copy()
             p1[k] += h
             p2[k] -= h
-            fd = (ref.unet_loss_and_gradients(p1, r, flags, rounding=False)[0] - ref.unet_loss_and_gradients(p2, r, flags, rounding=False)[0]) / (2 * h)
+            fd = (ref.unet_loss_and_gradients(p1, r, flags, rounding=False, loss=loss_kind)[0] -
+                  ref.unet_loss_and_gradients(p2, r, flags, rounding=False, loss=loss_kind)[0]) / (2 * h)
             assert abs(fd - grads[k]) <= 1e-4 * max(abs(fd), 1e-3 * abs(loss)), (li, k, fd, grads[k])
         off += n
+
+
+def test_energy_loss_is_minimised_by_the_exact_solve():
+    """The energy loss of z = A^-1 r is -r^T A^-1 r / 2 / ||r||^2 and any other
+    z scores higher by ||z - A^-1 r||_A^2 / 2 / ||r||^2 (closed form of the
+    quadratic), checked on the operator restatement with a dense solve."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent))
+    from test_gpu_preconditioner import volumes
+    r, flags = volumes(1, (5, 4, 3), 2)
+    r = r.astype(np.float64)
+    fluid = (flags[0] & ref.IDENTITY) == 0
+    n = r[0].size
+    A = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        A[:, k] = ref.apply_A(e.reshape(1, *r.shape[1:]), flags).ravel()
+    idx = np.flatnonzero(fluid.ravel())
+    Af = A[np.ix_(idx, idx)]
+    np.testing.assert_allclose(Af, Af.T)
+    rf = r.ravel()[idx]
+    zs = np.linalg.solve(Af, rf)
+
+    def energy(zf):
+        return 0.5 * zf @ Af @ zf - zf @ rf
+
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=zs.shape)
+    assert abs(energy(zs) + 0.5 * rf @ zs) <= 1e-12 * abs(rf @ zs)
+    assert abs(energy(zs + d) - energy(zs) - 0.5 * d @ Af @ d) <= 1e-10 * abs(d @ Af @ d)

@@ -64,6 +64,13 @@ void release(T*& p) {
     p = nullptr;
 }
 
+// the preconditioned residual of flexible PCG (network or multilevel cycle)
+void ensure_pcg_z(Context& c) {
+    if (c.pcg_z) return;
+    alloc(c.pcg_z, size_t(c.grid.cells));
+    enqueue_fill(c, c.pcg_z, 0.0f, c.grid.cells);
+}
+
 void destroy_resources(Context& c) {
     if (c.graph_exec) hipGraphExecDestroy(c.graph_exec);
     if (c.graph) hipGraphDestroy(c.graph);
@@ -94,6 +101,7 @@ void destroy_resources(Context& c) {
     release(c.scalars);
     release(c.partials);
     release(c.max_speed_bits);
+    multilevel_release(c);
     if (c.host_status) hipHostFree(c.host_status);
     c.host_status = nullptr;
     for (hipEvent_t& e : c.status_event)
@@ -391,6 +399,7 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         smoke::enqueue_fill(c, c.max_speed, 0.0f, 1);
         smoke::enqueue_occupancy_derived(c);
         c.options = smoke::default_options(c.config);
+        c.mg.options = smoke::default_multilevel_options();
         smoke::upload_solver_constants(c);
         smoke::rebuild_graph(c);
         hip_check(hipStreamSynchronize(c.stream), "create sync");
@@ -533,12 +542,15 @@ SmokeSimulationResult smoke_simulation_ext_default_solver_options(SmokeSimulatio
 
 SmokeSimulationResult smoke_simulation_ext_set_solver_options(SmokeSimulationContext context, const SmokeSimulationSolverOptions* o) {
     if (!o) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
-    if (o->pressure_mode > 1u || o->preconditioner > 2u || !(o->relative_tolerance > 0.0) || o->max_iterations <= 0 || o->max_refinements < 0 ||
+    if (o->pressure_mode > 1u || o->preconditioner > 3u || !(o->relative_tolerance > 0.0) || o->max_iterations <= 0 || o->max_refinements < 0 ||
         (o->adaptive_dt && (!(o->cfl > 0.0f) || !(o->dt_max > 0.0f))))
         return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) {
+        if (o->preconditioner == SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL && !smoke::multilevel_options_valid(c, c.mg.options))
+            throw smoke::HipError("multilevel options do not fit this grid");
         c.options = *o;
         if (c.options.check_interval <= 0) c.options.check_interval = 16;
+        if (c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL) smoke::ensure_pcg_z(c);
         smoke::upload_solver_constants(c);
         smoke::rebuild_graph(c);
     });
@@ -768,11 +780,67 @@ SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationC
 
 SmokeSimulationResult smoke_simulation_ext_set_learned_preconditioner(SmokeSimulationContext context, void* network) {
     return smoke::guarded_call(context, [&](Context& c) {
-        if (network && !c.pcg_z) {
-            smoke::alloc(c.pcg_z, size_t(c.grid.cells));
-            smoke::enqueue_fill(c, c.pcg_z, 0.0f, c.grid.cells);
-        }
+        if (network) smoke::ensure_pcg_z(c);
         c.learned = static_cast<SmokeNnPreconditioner*>(network);
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_default_multilevel_options(SmokeSimulationMultilevelOptions* out) {
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    *out = smoke::default_multilevel_options();
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_ext_set_multilevel_options(SmokeSimulationContext context, const SmokeSimulationMultilevelOptions* o) {
+    if (!o || !context) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    if (!smoke::multilevel_options_valid(*context, *o)) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        hip_check(hipStreamSynchronize(c.stream), "sync");   // buffers of the old hierarchy may be in use
+        if (o->levels != c.mg.options.levels) smoke::multilevel_release(c);
+        c.mg.options = *o;
+        c.mg.built = 0;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_multilevel_options(SmokeSimulationContext context, SmokeSimulationMultilevelOptions* out) {
+    if (!context || !out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    *out = context->mg.options;
+    return SMOKE_SIMULATION_RESULT_OK;
+}
+
+SmokeSimulationResult smoke_simulation_ext_get_multilevel_info(SmokeSimulationContext context, SmokeSimulationMultilevelInfo* out) {
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::multilevel_build(c);
+        SmokeSimulationMultilevelInfo info{};
+        info.levels = c.mg.built;
+        for (int l = 0; l < c.mg.built; ++l)
+            for (int k = 0; k < 3; ++k) info.dims[l][k] = c.mg.level[l].n[k];
+        info.captured = c.mg.captured;
+        *out = info;
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_apply_multilevel(SmokeSimulationContext context, const float* r, float* z) {
+    if (!r || !z || r == z) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) { smoke::multilevel_apply(c, r, z); });
+}
+
+SmokeSimulationResult smoke_simulation_ext_export_level_flags(SmokeSimulationContext context, int32_t level, uint8_t* flags) {
+    if (!flags) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::multilevel_build(c);
+        if (level < 0 || level >= c.mg.built) throw smoke::HipError("no such level");
+        const smoke::MgLevel& L = c.mg.level[level];
+        hip_check(hipMemcpyAsync(flags, L.flags, size_t(L.cells), hipMemcpyDeviceToDevice, c.stream), "level flags");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_set_multilevel_capture(SmokeSimulationContext context, float* buffer, uint32_t capacity) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        c.mg.capture = capacity > 0 ? buffer : nullptr;
+        c.mg.capacity = buffer ? capacity : 0;
+        c.mg.captured = 0;
     });
 }
 

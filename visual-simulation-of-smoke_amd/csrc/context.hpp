@@ -49,6 +49,33 @@ struct PcgStatus {      // what the host polls between chunks
     int done;
 };
 
+// One level of the multilevel preconditioner's hierarchy.  Level 0 borrows
+// the pressure grid's occupancy, flags and anchor; its r and z are the
+// vectors of the current application.
+struct MgLevel {
+    int n[3] = {0, 0, 0};
+    int64_t cells = 0;
+    uint8_t* occ = nullptr;
+    uint8_t* flags = nullptr;
+    int* anchor = nullptr;
+    float* r = nullptr;
+    float* z = nullptr;
+    float* ta = nullptr;     // sweep results (ping-pong)
+    float* tb = nullptr;
+};
+
+constexpr int kMaxLevels = 8;
+
+struct Multilevel {
+    SmokeSimulationMultilevelOptions options{};
+    int allocated = 0;           // levels whose buffers exist
+    int built = 0;               // levels whose occupancy / flags match the pressure grid's
+    MgLevel level[kMaxLevels];
+    float* capture = nullptr;    // coarsest-level right-hand sides (training data), not owned
+    uint32_t capacity = 0;
+    uint32_t captured = 0;
+};
+
 struct Context {
     SmokeSimulationConfig config{};
     Grid grid{};
@@ -121,6 +148,8 @@ struct Context {
     int pcg_shape[2] = {0, 0};
     int advect_block[4] = {0, 0, 0, 0};   // set from the table at creation
 
+    Multilevel mg;
+
     bool timing = false;
     SmokeSimulationKernelTiming timing_totals{};
     hipEvent_t timing_events[2][6] = {};
@@ -157,6 +186,13 @@ void table_pcg_shapes(Context& c);
 void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out);
 int tune_pcg_shapes(Context& c, SmokeSimulationLaunchReport& report);   // returns candidates timed
 void table_advect_blocks(Context& c);
+
+// Multilevel preconditioner (multigrid.hip).
+SmokeSimulationMultilevelOptions default_multilevel_options();
+bool multilevel_options_valid(const Context& c, const SmokeSimulationMultilevelOptions& o);
+void multilevel_build(Context& c);                          // hierarchy of the current occupancy (if stale)
+void multilevel_apply(Context& c, const float* r, float* z);   // z = M(r), one V-cycle
+void multilevel_release(Context& c);
 
 // Training-pair crops of the last pressure solve (training_data.hip).
 void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& crops);

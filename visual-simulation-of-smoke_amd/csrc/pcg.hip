@@ -828,6 +828,10 @@ void pass_finalize(Context& c, int mode) {
 // ---- flexible PCG with the learned preconditioner
 // z = M(r)
 void learned_apply(Context& c) {
+    if (c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL) {
+        multilevel_apply(c, c.pcg_r, c.pcg_z);
+        return;
+    }
     const int dims[3] = {c.grid.n[0], c.grid.n[1], c.grid.n[2]};
     preconditioner_apply(*c.learned, c.pcg_r, c.stencil_flags, dims, c.pcg_z, c.stream);
 }
@@ -1017,12 +1021,15 @@ void run_tolerance_pcg(Context& c) {
         host.done = 1;
     }
     int observed = host.iteration;
-    bool learned = o.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_LEARNED && c.learned && c.pcg_z;
+    // flexible PCG: the network on the pressure grid, or the multilevel cycle
+    // (which fails loudly when its coarse solver needs a network and none is attached)
+    const bool multilevel = o.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL && c.pcg_z;
+    bool learned = (o.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_LEARNED && c.learned && c.pcg_z) || multilevel;
     int learned_iterations = 0;
     bool fell_back = false;
-    // Inner solve with the learned preconditioner: flexible PCG, convergence
-    // checked after every iteration (an application of the network costs far
-    // more than the host round trip).  Returns false when the residual fell
+    // Inner solve with the learned (or multilevel) preconditioner: flexible
+    // PCG, convergence checked after every iteration (an application of the
+    // preconditioner costs more than the host round trip).  Returns false when the residual fell
     // by less than 10 % over 5 iterations or turned non-finite.
     auto learned_inner = [&]() -> bool {
         learned_apply(c);
@@ -1036,7 +1043,19 @@ void run_tolerance_pcg(Context& c) {
             rr.push_back(host.rr);
             if (host.done) return true;
             const size_t k = rr.size() - 1;
-            if (k >= 5 && rr[k] > 0.81 * rr[k - 5]) return false;
+            if (!multilevel) {
+                if (k >= 5 && rr[k] > 0.81 * rr[k - 5]) return false;
+            } else if (k >= 20) {
+                // The V-cycle is a convergent linear map, but ||r|| under it
+                // is not monotone: the near-null mode of the anchored
+                // operator (the pressure level) is built over the first
+                // iterations while ||r|| grows.  Stagnation here means the
+                // best residual of the last 10 iterations is not 10 % below
+                // the best one before them.
+                const double recent = *std::min_element(rr.end() - 10, rr.end());
+                const double before = *std::min_element(rr.begin(), rr.end() - 10);
+                if (recent > 0.81 * before) return false;
+            }
             learned_apply(c);
             learned_rz(c, false);
         }

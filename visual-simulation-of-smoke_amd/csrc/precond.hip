@@ -31,6 +31,7 @@ struct SmokeNnPreconditioner {
     float* adam_v = nullptr;
     int adam_t = 0;
     bool packed_dirty = true;
+    int loss = SMOKE_NN_LOSS_RESIDUAL;
 
     // workspace for one (batch, dims); grads only when trained
     int ws_batch = 0, ws_dims[3] = {0, 0, 0};
@@ -211,9 +212,11 @@ __device__ __forceinline__ float apply_A(const float* __restrict__ v, const uint
     return s;
 }
 
-// q = A z - rn on fluid rows (0 on identity rows), ||q||^2 per volume
+// q = A z - rn on fluid rows (0 on identity rows); per volume ||q||^2
+// (residual loss) or z.(q - rn) = z^T A z - 2 z^T rn (energy loss)
 __global__ __launch_bounds__(256) void loss_residual_kernel(const float* __restrict__ z, const uint8_t* __restrict__ flags, int W, int H, int D,
-                                                            const float* __restrict__ rn, float* __restrict__ q, double* __restrict__ sums) {
+                                                            const float* __restrict__ rn, float* __restrict__ q, double* __restrict__ sums,
+                                                            int energy) {
     const int b = blockIdx.y;
     const int64_t cells = int64_t(W) * H * D, base = int64_t(b) * cells;
     double acc = 0.0;
@@ -222,23 +225,25 @@ __global__ __launch_bounds__(256) void loss_residual_kernel(const float* __restr
         const bool ident = (flags[base + i] & kIdentity) != 0;
         const float qi = ident ? 0.0f : apply_A(z, flags, W, H, D, base, x, y, zz) - rn[base + i];
         q[base + i] = qi;
-        acc += double(qi) * double(qi);
+        if (energy) acc += ident ? 0.0 : double(z[base + i]) * (double(qi) - double(rn[base + i]));
+        else acc += double(qi) * double(qi);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&sums[3 * b + 2], acc);
 }
 
-// d loss / d z = (2 / batch) A q / ||r||^2 on fluid rows, written as the
-// bfloat16 gradient of the output layer's 16 channels (channel 0 only)
+// d loss / d z on fluid rows, written as the bfloat16 gradient of the output
+// layer's 16 channels (channel 0 only): (2 / batch) A q / ||r||^2 for the
+// residual loss, q / (batch ||r||^2) for the energy loss (A is symmetric)
 __global__ __launch_bounds__(256) void loss_grad_kernel(const float* __restrict__ q, const uint8_t* __restrict__ flags, int W, int H, int D, int batch,
-                                                        const double* __restrict__ sums, uint16_t* __restrict__ gout) {
+                                                        const double* __restrict__ sums, uint16_t* __restrict__ gout, int energy) {
     const int b = blockIdx.y;
     const int64_t cells = int64_t(W) * H * D, base = int64_t(b) * cells;
     const double rr = sums[3 * b + 1];
-    const float scale = rr > 0.0 ? float(2.0 / (double(batch) * rr)) : 0.0f;
+    const float scale = rr > 0.0 ? float((energy ? 1.0 : 2.0) / (double(batch) * rr)) : 0.0f;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cells; i += int64_t(gridDim.x) * 256) {
         const int x = int(i % W), y = int((i / W) % H), zz = int(i / (int64_t(W) * H));
-        const float g = scale * apply_A(q, flags, W, H, D, base, x, y, zz);
+        const float g = energy ? scale * q[base + i] : scale * apply_A(q, flags, W, H, D, base, x, y, zz);
         uint16_t* o = gout + (base + i) * 16;
         o[0] = to_bf16(g);
 #pragma unroll
@@ -432,8 +437,9 @@ void preconditioner_train_step(SmokeNnPreconditioner& net, const float* r, const
     prepare_input(net, r, flags, batch, cells, s);
     forward(net, batch, d, s);
     const dim3 grid(grid_for(cells), batch);
-    loss_residual_kernel<<<grid, 256, 0, s>>>(net.z, flags, dims[0], dims[1], dims[2], net.rn, net.q, net.sums);
-    loss_grad_kernel<<<grid, 256, 0, s>>>(net.q, flags, dims[0], dims[1], dims[2], batch, net.sums, net.G[kGOut]);
+    const int energy = net.loss == SMOKE_NN_LOSS_ENERGY ? 1 : 0;
+    loss_residual_kernel<<<grid, 256, 0, s>>>(net.z, flags, dims[0], dims[1], dims[2], net.rn, net.q, net.sums, energy);
+    loss_grad_kernel<<<grid, 256, 0, s>>>(net.q, flags, dims[0], dims[1], dims[2], batch, net.sums, net.G[kGOut], energy);
     hip_check(hipGetLastError(), "preconditioner loss");
     hip_check(hipMemsetAsync(net.grads, 0, size_t(net.nparams) * sizeof(float), s), "zero gradients");
     backward(net, batch, d, s);
@@ -449,7 +455,9 @@ void preconditioner_train_step(SmokeNnPreconditioner& net, const float* r, const
     }
     hip_check(hipStreamSynchronize(s), "train step");
     double total = 0.0;
-    for (int b = 0; b < batch; ++b) total += sums[3 * b + 1] > 0.0 ? sums[3 * b + 2] / sums[3 * b + 1] : 0.0;
+    // energy: sums[2] = z^T A z - 2 z^T rn, half of it is the loss
+    const double f = energy ? 0.5 : 1.0;
+    for (int b = 0; b < batch; ++b) total += sums[3 * b + 1] > 0.0 ? f * sums[3 * b + 2] / sums[3 * b + 1] : 0.0;
     if (loss) *loss = total / batch;
 }
 
@@ -528,6 +536,12 @@ SmokeSimulationResult smoke_nn_preconditioner_train_step(SmokeNnPreconditioner* 
                                                          const int32_t dims[3], float lr, double* loss, float* host_grads, void* stream) {
     if (!net || !r || !flags || batch <= 0 || !dims || dims[0] < 4 || dims[1] < 4 || dims[2] < 4) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     SMOKE_NN_GUARD(smoke::preconditioner_train_step(*net, r, flags, batch, dims, lr, loss, host_grads, static_cast<hipStream_t>(stream)))
+}
+
+SmokeSimulationResult smoke_nn_preconditioner_set_loss(SmokeNnPreconditioner* net, uint32_t loss) {
+    if (!net || loss > 1u) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    net->loss = int(loss);
+    return SMOKE_SIMULATION_RESULT_OK;
 }
 
 SmokeSimulationResult smoke_nn_preconditioner_apply(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags, const int32_t dims[3], float* z,

@@ -507,6 +507,7 @@ void enqueue_fill_u8(Context& c, uint8_t* p, uint8_t value, int64_t n) {
 
 void enqueue_occupancy_derived(Context& c) {
     c.solves_done = std::min(c.solves_done, 1);   // the warm-start history predates this occupancy
+    c.mg.built = 0;                               // the multilevel hierarchy is rebuilt from the new occupancy
     const Grid& g = c.grid;
     occupancy_float_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.occupancy, c.occupancy_float, g.cells);
     anchor_reset_kernel<<<1, 1, 0, c.stream>>>(c.anchor, int(g.cells), c.solid_count);

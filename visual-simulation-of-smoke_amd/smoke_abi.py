@@ -33,7 +33,9 @@ VIEW_KINDS = {
 LAYOUT_F32_3D, LAYOUT_F32_3D_SOA3 = 0, 1
 
 PRESSURE_FIXED_ITERATIONS, PRESSURE_TOLERANCE = 0, 1
-PRECONDITIONER_NONE, PRECONDITIONER_JACOBI, PRECONDITIONER_LEARNED = 0, 1, 2
+PRECONDITIONER_NONE, PRECONDITIONER_JACOBI, PRECONDITIONER_LEARNED, PRECONDITIONER_MULTILEVEL = 0, 1, 2, 3
+COARSE_NETWORK, COARSE_JACOBI = 0, 1
+NN_LOSS_RESIDUAL, NN_LOSS_ENERGY = 0, 1
 STAGE_FORCES, STAGE_ADVECT_VELOCITY, STAGE_PRESSURE, STAGE_PROJECT, STAGE_SCALARS = range(5)
 
 
@@ -125,6 +127,15 @@ class LaunchReport(C.Structure):
 
 class TrainingCrops(C.Structure):
     _fields_ = [("path", C.c_char_p), ("origins", C.c_void_p), ("count", C.c_uint32), ("size", C.c_uint32 * 3), ("step", C.c_uint32)]
+
+
+class MultilevelOptions(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("omega", C.c_float),
+                ("coarse_solver", C.c_uint32), ("coarse_sweeps", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
+class MultilevelInfo(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("dims", (C.c_int32 * 3) * 8), ("captured", C.c_uint32)]
 
 
 class PlumeSource(C.Structure):
@@ -279,6 +290,13 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
         "smoke_simulation_ext_set_learned_preconditioner": [C.c_void_p, C.c_void_p],
+        "smoke_simulation_ext_default_multilevel_options": [C.POINTER(MultilevelOptions)],
+        "smoke_simulation_ext_set_multilevel_options": [C.c_void_p, C.POINTER(MultilevelOptions)],
+        "smoke_simulation_ext_get_multilevel_options": [C.c_void_p, C.POINTER(MultilevelOptions)],
+        "smoke_simulation_ext_get_multilevel_info": [C.c_void_p, C.POINTER(MultilevelInfo)],
+        "smoke_simulation_ext_apply_multilevel": [C.c_void_p, C.c_void_p, C.c_void_p],
+        "smoke_simulation_ext_export_level_flags": [C.c_void_p, C.c_int32, C.c_void_p],
+        "smoke_simulation_ext_set_multilevel_capture": [C.c_void_p, C.c_void_p, C.c_uint32],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -463,6 +481,43 @@ class SmokeContext:
         check(self.lib.smoke_simulation_ext_get_launches(self.handle, C.byref(r)), "get_launches")
         return r
 
+    def set_multilevel_options(self, **kw):
+        o = MultilevelOptions()
+        check(self.lib.smoke_simulation_ext_get_multilevel_options(self.handle, C.byref(o)), "get_multilevel_options")
+        for k, val in kw.items():
+            setattr(o, k, val)
+        check(self.lib.smoke_simulation_ext_set_multilevel_options(self.handle, C.byref(o)), "set_multilevel_options")
+
+    def multilevel_options(self) -> MultilevelOptions:
+        o = MultilevelOptions()
+        check(self.lib.smoke_simulation_ext_get_multilevel_options(self.handle, C.byref(o)), "get_multilevel_options")
+        return o
+
+    def multilevel_info(self) -> tuple[list[tuple[int, int, int]], int]:
+        """(dims (nx, ny, nz) of each level, coarsest right-hand sides captured)"""
+        i = MultilevelInfo()
+        check(self.lib.smoke_simulation_ext_get_multilevel_info(self.handle, C.byref(i)), "get_multilevel_info")
+        return [tuple(i.dims[l]) for l in range(i.levels)], int(i.captured)
+
+    def apply_multilevel(self, d_r: int, d_z: int):
+        """z = M(r) on device vectors of the pressure grid."""
+        check(self.lib.smoke_simulation_ext_apply_multilevel(self.handle, d_r, d_z), "apply_multilevel")
+
+    def level_flags(self, level: int) -> np.ndarray:
+        """The flag byte of one level, as a (nz, ny, nx) array."""
+        dims, _ = self.multilevel_info()
+        nx, ny, nz = dims[level]
+        d = self.hip.malloc(nx * ny * nz)
+        try:
+            check(self.lib.smoke_simulation_ext_export_level_flags(self.handle, level, d), "export_level_flags")
+            self.synchronize()
+            return self.hip.download(d, (nz, ny, nx), np.uint8)
+        finally:
+            self.hip.free(d)
+
+    def set_multilevel_capture(self, d_buffer: int | None, capacity: int):
+        check(self.lib.smoke_simulation_ext_set_multilevel_capture(self.handle, d_buffer, capacity), "set_multilevel_capture")
+
 
 class LearnedPreconditioner:
     """The learned pressure preconditioner (visual-simulation-of-smoke-3d-nn.h):
@@ -481,6 +536,7 @@ class LearnedPreconditioner:
         L.smoke_nn_preconditioner_load.argtypes = [V, C.c_char_p]
         L.smoke_nn_preconditioner_train_step.argtypes = [V, V, V, C.c_int32, C.POINTER(C.c_int32), C.c_float, C.POINTER(C.c_double), V, V]
         L.smoke_nn_preconditioner_apply.argtypes = [V, V, V, C.POINTER(C.c_int32), V, V]
+        L.smoke_nn_preconditioner_set_loss.argtypes = [V, C.c_uint32]
         self.handle = V()
         check(L.smoke_nn_preconditioner_create(seed, C.byref(self.handle)), "preconditioner create")
         if path is not None:
@@ -510,6 +566,14 @@ class LearnedPreconditioner:
 
     def load(self, path):
         check(self.lib.smoke_nn_preconditioner_load(self.handle, str(path).encode()), "load weights")
+
+    def set_loss(self, loss: int):
+        """NN_LOSS_RESIDUAL or NN_LOSS_ENERGY for later train_step calls."""
+        check(self.lib.smoke_nn_preconditioner_set_loss(self.handle, loss), "set loss")
+
+    def apply(self, d_r: int, d_flags: int, dims, d_z: int, stream=None):
+        """z = M(r) on one device volume of dims (W, H, D)."""
+        check(self.lib.smoke_nn_preconditioner_apply(self.handle, d_r, d_flags, (C.c_int32 * 3)(*dims), d_z, stream), "apply")
 
     def train_step(self, d_r: int, d_flags: int, batch: int, dims, lr: float) -> float:
         """One Adam step on `batch` device volumes (dims = (W, H, D)); returns the loss."""

@@ -12,7 +12,16 @@ FP32 Adam) on one GPU.
    printing the loss ||A z - r||^2 / ||r||^2 as it goes;
 3. saves the weights (SMOKENN1) to `--out`.
 
+With `--coarse` the network is trained as the coarse-grid solver of the
+multilevel preconditioner instead: the plume (and, with `--spheres`, the
+obstacle scene) runs with flexible PCG preconditioned by the V-cycle whose
+coarsest level is smoothed only, every V-cycle's coarsest right-hand side is
+captured on the device (smoke_simulation_ext_set_multilevel_capture), and the
+network learns z = A_c^-1 r_c on those whole coarse volumes with the energy
+loss (z^T A z / 2 - z^T r) / ||r||^2.
+
 Usage: python train_preconditioner.py --out weights.bin [--grid 256 --steps 40 ...]
+       python train_preconditioner.py --coarse --out coarse.bin [--grid 256 --levels 4 --steps 30 --spheres]
 """
 from __future__ import annotations
 
@@ -60,6 +69,62 @@ def generate(path: Path, n: int, steps: int, crop: int, per_step: int, seed: int
             ctx.write_training_crops(path, origins, (crop, crop, crop), step=k)
             if k % 10 == 0:
                 log(f"data: step {k}, {ctx.stats().iterations} PCG iterations", flush=True)
+
+
+def spheres_occupancy(n: int, count: int = 12, seed: int = 5) -> np.ndarray:
+    """`count` seeded solid spheres of radius U(8, 24) * n / 256 cells,
+    centred above the plume source (y in [0.3 n, 0.9 n])."""
+    rng = np.random.default_rng(seed)
+    s = n / 256.0
+    z, y, x = np.meshgrid(np.arange(n) + 0.5, np.arange(n) + 0.5, np.arange(n) + 0.5, indexing="ij")
+    occ = np.zeros((n, n, n), np.uint8)
+    for _ in range(count):
+        r = rng.uniform(8.0, 24.0) * s
+        cx, cz = rng.uniform(r, n - r, size=2)
+        cy = rng.uniform(0.3 * n, 0.9 * n - r)
+        occ[(x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= r * r] = 1
+    return occ
+
+
+def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plume",), coarse_sweeps: int = 16, log=print):
+    """Coarsest-level right-hand sides of the V-cycles of `steps` multilevel
+    solves per scene; returns {"r": (count, D, H, W) float32, "flags": uint8,
+    "crop": (W, H, D)}."""
+    hip = abi.Hip()
+    rs, fs, dims = [], [], None
+    for scene in scenes:
+        cfg, src = plume_config(n)
+        if scene == "spheres":
+            cfg.vorticity_confinement = 2.5
+        with abi.SmokeContext(cfg, hip=hip) as ctx:
+            if scene == "spheres":
+                ctx.set_occupancy(spheres_occupancy(n))
+            ctx.set_multilevel_options(levels=levels, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+                                       coarse_sweeps=coarse_sweeps)
+            ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_MULTILEVEL,
+                                   relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1,
+                                   cfl=0.5, dt_max=0.01, check_interval=16)
+            level_dims, _ = ctx.multilevel_info()
+            dims = level_dims[-1]
+            cells = dims[0] * dims[1] * dims[2]
+            buf = hip.malloc(4 * cells * capacity)
+            try:
+                ctx.set_multilevel_capture(buf, capacity)
+                for k in range(steps):
+                    ctx.plume_sources(src, k)
+                    ctx.step()
+                    if k % 10 == 0:
+                        log(f"data: {scene} step {k}, {ctx.stats().iterations} iterations, "
+                            f"{ctx.multilevel_info()[1]} coarse volumes", flush=True)
+                ctx.synchronize()
+                count = ctx.multilevel_info()[1]
+                ctx.set_multilevel_capture(None, 0)
+                flags = ctx.level_flags(len(level_dims) - 1)
+                rs.append(hip.download(buf, (count, dims[2], dims[1], dims[0])))
+                fs.append(np.broadcast_to(flags, (count, *flags.shape)))
+            finally:
+                hip.free(buf)
+    return {"r": np.concatenate(rs), "flags": np.ascontiguousarray(np.concatenate(fs)), "crop": dims}
 
 
 def train(net: abi.LearnedPreconditioner, data: dict, batch: int, lr: float, epochs: int, time_limit: float, seed: int = 0, log=print):
@@ -113,7 +178,22 @@ def main():
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--reuse-data", action="store_true")
+    ap.add_argument("--coarse", action="store_true", help="train the multilevel preconditioner's coarse-grid solver")
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--capacity", type=int, default=1500, help="coarse volumes captured per scene")
+    ap.add_argument("--spheres", action="store_true", help="also capture the obstacle scene's solves")
     args = ap.parse_args()
+    if args.coarse:
+        scenes = ("plume", "spheres") if args.spheres else ("plume",)
+        data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes)
+        print(f"data: {data['r'].shape[0]} coarse volumes of {data['crop']}", flush=True)
+        with abi.LearnedPreconditioner(seed=args.seed) as net:
+            net.set_loss(abi.NN_LOSS_ENERGY)
+            hist = train(net, data, args.batch, args.lr, args.epochs, args.time_limit, args.seed)
+            net.save(args.out)
+        print(f"trained: {len(hist)} Adam steps, energy loss {np.mean(hist[:10]):.4f} -> {np.mean(hist[-10:]):.4f}; weights in {args.out}",
+              flush=True)
+        return
     path = Path(args.data)
     if not (args.reuse_data and path.exists()):
         generate(path, args.grid, args.steps, args.crop, args.crops_per_step, args.seed)
