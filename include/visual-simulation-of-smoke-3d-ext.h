@@ -258,10 +258,15 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_learned_prec
               z_(l+1) = M_(l+1)(r_(l+1));
               x += z_(l+1) of the parent cell                   (prolongation);
               z = `post_sweeps` damped-Jacobi sweeps from x;
-     l = L-1: z = network(r) followed by `coarse_sweeps` sweeps
-              (coarse_solver = NETWORK), or `coarse_sweeps` sweeps from 0
-              (coarse_solver = JACOBI).
-   A damped-Jacobi sweep is x += omega D^-1 (r - A x); identity rows stay 0. */
+     l = L-1: z = c v + P^T S(P r), the exact deflation of v = 1 on the
+              fluid rows (c = v.r / v.Av, P r = r - c A v,
+              P^T y = y - v (Av.y) / v.Av), where S is the network followed by
+              `coarse_sweeps` sweeps (coarse_solver = NETWORK) or
+              `coarse_sweeps` sweeps from 0 (coarse_solver = JACOBI).
+   A damped-Jacobi sweep is x += omega D^-1 (r - A x); identity rows stay 0.
+   The deflation is exact for an exact S and takes the near-null mode of the
+   anchored operator (nearly constant, set by the global sum of r) out of S's
+   task. */
 typedef enum SmokeSimulationCoarseSolver {
     SMOKE_SIMULATION_COARSE_NETWORK = 0,
     SMOKE_SIMULATION_COARSE_JACOBI = 1,
@@ -291,7 +296,7 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_multilevel_o
 typedef struct SmokeSimulationMultilevelInfo {
     int32_t levels;
     int32_t dims[8][3];
-    uint32_t captured;       /* coarsest-level right-hand sides stored since the capture was set */
+    uint32_t captured;       /* deflated coarsest-level right-hand sides stored since the capture was set */
 } SmokeSimulationMultilevelInfo;
 
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_multilevel_info(SmokeSimulationContext context, SmokeSimulationMultilevelInfo* out_info);
@@ -305,7 +310,8 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_apply_multilevel
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_level_flags(SmokeSimulationContext context, int32_t level, uint8_t* flags);
 
 /* Training data of the coarse-grid network: while set, every V-cycle copies
-   its coarsest-level right-hand side into the next free slot of `buffer`
+   its deflated coarsest-level right-hand side P r (the input of S) into the
+   next free slot of `buffer`
    (device memory, `capacity` volumes of the coarsest level's cells, FP32),
    until it is full.  NULL stops the capture; setting a buffer restarts the
    count. */

@@ -87,6 +87,26 @@ def hierarchy(occupancy: np.ndarray, levels: int, periodic=(False, False, False)
     return out
 
 
+def deflation_vector(L: Level):
+    """v = 1 on the fluid rows and A v (nonzero only next to the anchor and on
+    OUTFLOW faces); v^T A v."""
+    v = np.where(L.ident, 0.0, 1.0)
+    av = L.apply(v)
+    return v, av, float(np.vdot(v, av))
+
+
+def deflated_solve(L: Level, r, solve):
+    """z = Q r + P^T solve(P r) with Q = v v^T / (v^T A v), P = I - A Q: exact
+    when `solve` is A^-1, and the near-null mode of the anchored operator
+    (v, nearly constant) never reaches `solve`."""
+    v, av, vav = deflation_vector(L)
+    if vav <= 0.0:
+        return solve(r)
+    c = np.vdot(v, r) / vav
+    zp = solve(np.where(L.ident, 0.0, r - c * av))
+    return np.where(L.ident, 0.0, zp + (c - np.vdot(av, zp) / vav) * v)
+
+
 def restrict(res: np.ndarray, coarse_shape) -> np.ndarray:
     cz, cy, cx = coarse_shape
     pad = np.zeros((2 * cz, 2 * cy, 2 * cx))
@@ -99,21 +119,26 @@ def prolong(e: np.ndarray, fine_shape) -> np.ndarray:
     return up[:fine_shape[0], :fine_shape[1], :fine_shape[2]]
 
 
-def vcycle(levels, r, pre=2, post=2, omega=2.0 / 3.0, coarse_sweeps=8, l=0):
-    """z = M(r) with damped-Jacobi sweeps as the coarsest solver."""
+def vcycle(levels, r, pre=2, post=2, omega=2.0 / 3.0, coarse_sweeps=8, l=0, coarse=None):
+    """z = M(r); the coarsest level is solved by `coarse(level, r)` (default:
+    `coarse_sweeps` damped-Jacobi sweeps from 0) inside the deflation of
+    deflated_solve."""
     L = levels[l]
     r = np.where(L.ident, 0.0, r)
     if l == len(levels) - 1:
-        x = np.zeros_like(r)
-        for _ in range(coarse_sweeps):
-            x = L.sweep(r, x, omega)
-        return x
+        def jacobi(rr):
+            x = np.zeros_like(rr)
+            for _ in range(coarse_sweeps):
+                x = L.sweep(rr, x, omega)
+            return x
+        solve = (lambda rr: coarse(L, rr)) if coarse is not None else jacobi
+        return deflated_solve(L, r, solve) if l > 0 else solve(r)
     x = np.zeros_like(r)
     for _ in range(pre):
         x = L.sweep(r, x, omega)
     C = levels[l + 1]
     rc = np.where(C.ident, 0.0, restrict(np.where(L.ident, 0.0, r - L.apply(x)), C.solid.shape))
-    ec = vcycle(levels, rc, pre, post, omega, coarse_sweeps, l + 1)
+    ec = vcycle(levels, rc, pre, post, omega, coarse_sweeps, l + 1, coarse)
     x = np.where(L.ident, 0.0, x + prolong(ec, L.solid.shape))
     for _ in range(post):
         x = L.sweep(r, x, omega)

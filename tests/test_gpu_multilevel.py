@@ -55,9 +55,12 @@ def _tolerance(ctx, precond, **kw):
     ctx.set_solver_options(**o)
 
 
-@pytest.mark.parametrize("pre,post,levels", [(2, 2, 3), (1, 1, 4), (3, 2, 3)])
-def test_vcycle_matches_restatement(pre, post, levels):
-    shape = (21, 18, 24)                       # (nz, ny, nx); x periodic, odd z
+@pytest.mark.parametrize("pre,post,levels,shape", [(2, 2, 3, (21, 18, 24)), (1, 1, 4, (21, 18, 24)), (3, 2, 3, (21, 18, 24)),
+                                                    (2, 2, 4, (97, 96, 128)), (1, 3, 3, (97, 96, 128))])
+def test_vcycle_matches_restatement(pre, post, levels, shape):
+    """(nz, ny, nx) shapes with x periodic and odd z; the larger one (over 2^20
+    cells) runs the fine level through the element-wise first-sweep and
+    prolongation passes."""
     flow = [P, P, W, W, W, W]
     cfg = abi.make_config(shape[2], shape[1], shape[0], flow=flow)
     occ = _spheres(shape, 5, 3)

@@ -2,8 +2,8 @@
 nn_reference.py: forward (FP32 output, relative L2 <= 1e-5 -- the inputs are
 bfloat16 and the products exact, so only the FP32 summation differs), the data
 gradient (bfloat16 output, <= 8e-3: one bfloat16 rounding) and the weight and
-bias gradients (FP32, <= 1e-5), for every mode, odd and even sizes and
-partial 16-voxel tiles."""
+bias gradients (FP32, <= 1e-5), for every mode, odd and even sizes,
+partial 16-voxel tiles, and volumes on both sides of the split-K threshold."""
 import ctypes as C
 
 import numpy as np
@@ -55,6 +55,9 @@ CASES = [
     (ref.DOWN, (16, 6, 6), 32, 64),
     (ref.UP, (10, 4, 3), 64, 32),
     (ref.UP, (9, 5, 4), 32, 16),
+    # 8192+ output tiles: one wave per tile (smaller volumes split K over a block)
+    (ref.SAME, (64, 64, 32), 16, 32),
+    (ref.UP, (32, 32, 16), 32, 16),
 ]
 
 

@@ -107,3 +107,33 @@ def test_multilevel_fcg_needs_far_fewer_iterations_than_jacobi():
     for x in (x_j, x_m):
         assert np.linalg.norm(b - L.apply(x)) <= 1e-6 * np.linalg.norm(b)
     assert it_m * 5 < it_j, (it_m, it_j)
+
+
+def test_deflated_coarse_solve_is_exact_for_an_exact_solver():
+    """z = c v + P^T S(P r) equals A^-1 r when S = A^-1 (the identity
+    A^-1 = Q + P^T A^-1 P), for an anchored level with solids; and with S = 0
+    it is the v-component Q r alone."""
+    occ = _spheres(10, 3, 5)
+    L = ml.hierarchy(occ, 2)[1]
+    fluid = ~L.ident
+    n = L.solid.size
+    cols = []
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        cols.append(L.apply(np.where(L.ident, 0.0, e.reshape(L.solid.shape))).ravel())
+    A = np.array(cols).T
+    idx = np.flatnonzero(fluid.ravel())
+    Af = A[np.ix_(idx, idx)]
+
+    def exact(r):
+        out = np.zeros(n)
+        out[idx] = np.linalg.solve(Af, r.ravel()[idx])
+        return out.reshape(r.shape)
+
+    rng = np.random.default_rng(9)
+    r = np.where(fluid, rng.normal(size=L.solid.shape) + 0.3, 0.0)
+    np.testing.assert_allclose(ml.deflated_solve(L, r, exact), exact(r), rtol=1e-10, atol=1e-10 * np.abs(exact(r)).max())
+    v, av, vav = ml.deflation_vector(L)
+    assert vav > 0
+    np.testing.assert_allclose(ml.deflated_solve(L, r, np.zeros_like), v * (np.vdot(v, r) / vav), rtol=1e-12)

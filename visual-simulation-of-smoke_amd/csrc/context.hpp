@@ -62,6 +62,10 @@ struct MgLevel {
     float* z = nullptr;
     float* ta = nullptr;     // sweep results (ping-pong)
     float* tb = nullptr;
+    // coarsest level: deflation of v = 1 on the fluid rows
+    float* av = nullptr;     // A v
+    float* rd = nullptr;     // deflated right-hand side r - c A v
+    double* defl = nullptr;  // v^T A v, c = v^T r / v^T A v
 };
 
 constexpr int kMaxLevels = 8;

@@ -15,6 +15,14 @@
 // operator -- and the prolongation adds the parent's correction to each fluid
 // child.
 //
+// The coarsest level is solved inside an exact deflation of v = 1 on its
+// fluid rows: z = c v + P^T S(P r), c = v.r / v.Av, P = I - A v v^T / v.Av,
+// with S the network (or Jacobi sweeps).  The identity
+// A^-1 = Q + P^T A^-1 P (Q = v v^T / v.Av) makes this exact for an exact S;
+// it keeps the near-null mode of the anchored operator -- nearly constant,
+// with an eigenvalue far below the rest -- away from S, which a network with a
+// local receptive field cannot resolve (it needs the global sum of r).
+//
 // Kernels are one thread per cell (per coarse cell for the restriction)
 // reading their neighbours through the cache.  The first sweep from zero
 // (x = omega D^-1 r) is never stored: the next kernel recomputes it at every
@@ -36,6 +44,7 @@ struct LevelArgs {
     int periodic[3];
     int outflow[6];
     int any_outflow;
+    int sy, sz;          // strides of y and z (n0, n0 n1)
     const uint8_t* flags;
     float omega;
 };
@@ -47,22 +56,47 @@ __device__ __forceinline__ int outflow_faces(const LevelArgs& a, int x, int y, i
 }
 
 // diagonal of a fluid (non-identity) row: fluid neighbours + OUTFLOW faces, at least 1
-__device__ __forceinline__ float row_diag(const LevelArgs& a, unsigned f, int x, int y, int z) {
+__device__ __forceinline__ int row_diag(const LevelArgs& a, unsigned f, int x, int y, int z) {
     const int d = __popc(f & 0x3fu) + outflow_faces(a, x, y, z);
-    return float(d > 1 ? d : 1);
+    return d > 1 ? d : 1;
 }
 
-__device__ __forceinline__ int64_t lin(const LevelArgs& a, int x, int y, int z) { return (int64_t(z) * a.n[1] + y) * a.n[0] + x; }
+// 1 / d for the small integer diagonal (the hardware reciprocal, within one
+// ulp; the preconditioner does not need the correctly rounded quotient)
+__device__ __forceinline__ float recip(int d) { return __builtin_amdgcn_rcpf(float(d)); }
 
-// neighbour in direction d (0..5: x-, x+, y-, y+, z-, z+); only called for
-// set flag bits, so it exists (periodic axes wrap by one period)
-__device__ __forceinline__ void neighbour(const LevelArgs& a, int d, int& x, int& y, int& z) {
-    const int ax = d >> 1, s = (d & 1) ? 1 : -1;
-    int* c[3] = {&x, &y, &z};
-    int v = *c[ax] + s;
-    if (v < 0) v += a.n[ax];
-    else if (v >= a.n[ax]) v -= a.n[ax];
-    *c[ax] = v;
+__device__ __forceinline__ int lin(const LevelArgs& a, int x, int y, int z) { return z * a.sz + y * a.sy + x; }
+
+// Neighbour D (0..5: x-, x+, y-, y+, z-, z+) of cell i at (x, y, z): its index
+// and coordinates.  Only used for set flag bits, so the neighbour exists; at
+// the domain edge it is the periodic image.
+template <int D>
+__device__ __forceinline__ int neighbour(const LevelArgs& a, int i, int& x, int& y, int& z) {
+    if constexpr (D == 0) {
+        if (x > 0) return --x, i - 1;
+        x = a.n[0] - 1;
+        return i + a.n[0] - 1;
+    } else if constexpr (D == 1) {
+        if (x < a.n[0] - 1) return ++x, i + 1;
+        x = 0;
+        return i - (a.n[0] - 1);
+    } else if constexpr (D == 2) {
+        if (y > 0) return --y, i - a.sy;
+        y = a.n[1] - 1;
+        return i + (a.n[1] - 1) * a.sy;
+    } else if constexpr (D == 3) {
+        if (y < a.n[1] - 1) return ++y, i + a.sy;
+        y = 0;
+        return i - (a.n[1] - 1) * a.sy;
+    } else if constexpr (D == 4) {
+        if (z > 0) return --z, i - a.sz;
+        z = a.n[2] - 1;
+        return i + (a.n[2] - 1) * a.sz;
+    } else {
+        if (z < a.n[2] - 1) return ++z, i + a.sz;
+        z = 0;
+        return i - (a.n[2] - 1) * a.sz;
+    }
 }
 
 // What the sweep and restriction kernels take as the current iterate x:
@@ -76,38 +110,55 @@ enum XIn {
 
 struct Coarse {
     const float* e;   // correction on the next level
-    int n0, n1;       // its x and y extent
+    int n0;           // its x extent (y stride)
+    int s2;           // its z stride
 };
 
+__device__ __forceinline__ int parent(const Coarse& cc, int x, int y, int z) { return (z >> 1) * cc.s2 + (y >> 1) * cc.n0 + (x >> 1); }
+
+// x at cell i (flag byte f): identity rows hold 0
 template <int XIN>
 __device__ __forceinline__ float x_at(const LevelArgs& a, const float* __restrict__ r, const float* __restrict__ xb, const Coarse& cc, int x, int y,
-                                      int z, int64_t i, unsigned f) {
-    if (XIN == kZero || (f & kIdentity)) return 0.0f;
+                                      int z, int i, unsigned f) {
+    if (XIN == kZero) return 0.0f;
+    if constexpr (XIN == kBuffer) return xb[i];   // already 0 on identity rows
+    if (f & kIdentity) return 0.0f;
     float v = 0.0f;
-    if constexpr (XIN == kFromR || XIN == kFromRPlusC) v = a.omega * r[i] / row_diag(a, f, x, y, z);
-    if constexpr (XIN == kBuffer || XIN == kBufferPlusC) v = xb[i];
-    if constexpr (XIN == kBufferPlusC || XIN == kFromRPlusC) v += cc.e[(int64_t(z >> 1) * cc.n1 + (y >> 1)) * cc.n0 + (x >> 1)];
+    if constexpr (XIN == kFromR || XIN == kFromRPlusC) v = a.omega * r[i] * recip(row_diag(a, f, x, y, z));
+    if constexpr (XIN == kBufferPlusC) v = xb[i];
+    if constexpr (XIN == kBufferPlusC || XIN == kFromRPlusC) v += cc.e[parent(cc, x, y, z)];
     return v;
 }
 
-// (r - A x)_i of fluid row i with flag byte f
+template <int XIN, int D>
+__device__ __forceinline__ float tap(const LevelArgs& a, const float* __restrict__ r, const float* __restrict__ xb, const Coarse& cc, int x, int y,
+                                     int z, int i, unsigned f) {
+    if (!(f & (1u << D))) return 0.0f;
+    int xx = x, yy = y, zz = z;
+    const int j = neighbour<D>(a, i, xx, yy, zz);
+    const unsigned fj = (XIN == kBuffer) ? 0u : unsigned(a.flags[j]);
+    return x_at<XIN>(a, r, xb, cc, xx, yy, zz, j, fj);
+}
+
+// (r - A x)_i of fluid row i with flag byte f; also returns x_i and 1 / diag_i
 template <int XIN>
 __device__ __forceinline__ float residual_at(const LevelArgs& a, const float* __restrict__ r, const float* __restrict__ xb, const Coarse& cc, int x,
-                                             int y, int z, int64_t i, unsigned f, float& xi, float& diag) {
-    diag = row_diag(a, f, x, y, z);
-    xi = x_at<XIN>(a, r, xb, cc, x, y, z, i, f);
-    float ax = diag * xi;
-    if constexpr (XIN != kZero) {
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            if (!(f & (1u << d))) continue;
-            int xx = x, yy = y, zz = z;
-            neighbour(a, d, xx, yy, zz);
-            const int64_t j = lin(a, xx, yy, zz);
-            ax -= x_at<XIN>(a, r, xb, cc, xx, yy, zz, j, a.flags[j]);
-        }
+                                             int y, int z, int i, unsigned f, float& xi, float& inv) {
+    const int diag = row_diag(a, f, x, y, z);
+    inv = recip(diag);
+    if constexpr (XIN == kZero) {
+        xi = 0.0f;
+        return r[i];
+    } else {
+        float ri = r[i];
+        if constexpr (XIN == kFromR || XIN == kFromRPlusC) xi = a.omega * ri * inv;
+        else xi = xb[i];
+        if constexpr (XIN == kBufferPlusC || XIN == kFromRPlusC) xi += cc.e[parent(cc, x, y, z)];
+        const float s = tap<XIN, 0>(a, r, xb, cc, x, y, z, i, f) + tap<XIN, 1>(a, r, xb, cc, x, y, z, i, f) +
+                        tap<XIN, 2>(a, r, xb, cc, x, y, z, i, f) + tap<XIN, 3>(a, r, xb, cc, x, y, z, i, f) +
+                        tap<XIN, 4>(a, r, xb, cc, x, y, z, i, f) + tap<XIN, 5>(a, r, xb, cc, x, y, z, i, f);
+        return ri - (float(diag) * xi - s);
     }
-    return r[i] - ax;
 }
 
 // out = x + omega D^-1 (r - A x) on fluid rows, 0 on identity rows
@@ -118,43 +169,72 @@ __global__ __launch_bounds__(256) void mg_sweep_kernel(LevelArgs a, const float*
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
     if (x >= a.n[0] || y >= a.n[1]) return;
-    const int64_t i = lin(a, x, y, z);
+    const int i = lin(a, x, y, z);
     const unsigned f = a.flags[i];
     if (f & kIdentity) {
         out[i] = 0.0f;
         return;
     }
-    float xi, diag;
-    const float res = residual_at<XIN>(a, r, xb, cc, x, y, z, i, f, xi, diag);
-    out[i] = xi + a.omega * res / diag;
+    float xi, inv;
+    const float res = residual_at<XIN>(a, r, xb, cc, x, y, z, i, f, xi, inv);
+    out[i] = xi + a.omega * res * inv;
 }
 
-// r_c = 1/2 * sum over the children of (r - A x); 0 on coarse identity rows
+// x = omega D^-1 r (one sweep from zero) on fluid rows, 0 on identity rows
+__global__ __launch_bounds__(256) void mg_scale_kernel(LevelArgs a, const float* __restrict__ r, float* __restrict__ out) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= a.n[0] || y >= a.n[1]) return;
+    const int i = lin(a, x, y, z);
+    const unsigned f = a.flags[i];
+    out[i] = (f & kIdentity) ? 0.0f : a.omega * r[i] * recip(row_diag(a, f, x, y, z));
+}
+
+// x += correction of the parent cell, on fluid rows (identity rows stay 0)
+__global__ __launch_bounds__(256) void mg_prolong_kernel(LevelArgs a, Coarse cc, float* __restrict__ xb) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= a.n[0] || y >= a.n[1]) return;
+    const int i = lin(a, x, y, z);
+    if (!(a.flags[i] & kIdentity)) xb[i] += cc.e[parent(cc, x, y, z)];
+}
+
+// r_c = 1/2 * sum over the children of (r - A x); 0 on coarse identity rows.
+// Threads own fine cells: a 64 x 4 block covers two z planes (each thread a
+// column of two cells); the x pair is summed with a lane shuffle, the y pair
+// through LDS, and the thread of the even (x, y) child writes.
 template <int XIN>
 __global__ __launch_bounds__(256) void mg_restrict_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb, int nc0, int nc1,
-                                                          int nc2, const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
-    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+                                                          const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
+    __shared__ float rows[4][64];
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const int y = blockIdx.y * 4 + row;
     const int Z = blockIdx.z;
-    if (X >= nc0 || Y >= nc1 || Z >= nc2) return;
-    const int64_t ci = (int64_t(Z) * nc1 + Y) * nc0 + X;
-    if (cflags[ci] & kIdentity) {
-        rc[ci] = 0.0f;
-        return;
-    }
     const Coarse none{nullptr, 0, 0};
     float s = 0.0f;
+    if (x < a.n[0] && y < a.n[1]) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int x = 2 * X + (c & 1), y = 2 * Y + ((c >> 1) & 1), z = 2 * Z + (c >> 2);
-        if (x >= a.n[0] || y >= a.n[1] || z >= a.n[2]) continue;
-        const int64_t i = lin(a, x, y, z);
-        const unsigned f = a.flags[i];
-        if (f & kIdentity) continue;
-        float xi, diag;
-        s += residual_at<XIN>(a, r, xb, none, x, y, z, i, f, xi, diag);
+        for (int k = 0; k < 2; ++k) {
+            const int z = 2 * Z + k;
+            if (z >= a.n[2]) break;
+            const int i = lin(a, x, y, z);
+            const unsigned f = a.flags[i];
+            if (f & kIdentity) continue;
+            float xi, inv;
+            s += residual_at<XIN>(a, r, xb, none, x, y, z, i, f, xi, inv);
+        }
     }
-    rc[ci] = 0.5f * s;
+    s += __shfl_xor(s, 1, 64);
+    rows[row][lane] = s;
+    __syncthreads();
+    if ((row & 1) || (lane & 1)) return;
+    const int X = x >> 1, Y = y >> 1;
+    if (X >= nc0 || Y >= nc1) return;
+    const int ci = (Z * nc1 + Y) * nc0 + X;
+    rc[ci] = (cflags[ci] & kIdentity) ? 0.0f : 0.5f * (s + rows[row + 1][lane]);
 }
 
 // coarse occupancy: solid when every existing child is solid
@@ -210,6 +290,91 @@ __global__ void mg_flags_kernel(LevelArgs a, const uint8_t* __restrict__ occ, co
     flags[i] = uint8_t(m);
 }
 
+constexpr int kDeflThreads = 1024;
+
+// one block: sum over the block's threads in FP64
+__device__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kDeflThreads / 64; ++w) t += red[w];
+        red[kDeflThreads / 64] = t;
+    }
+    __syncthreads();
+    return red[kDeflThreads / 64];
+}
+
+// A v for v = 1 on the fluid rows (the anchor's fluid neighbours and the
+// OUTFLOW faces carry it) and v^T A v (one block)
+__global__ __launch_bounds__(kDeflThreads) void mg_deflation_setup_kernel(LevelArgs a, float* __restrict__ av, double* __restrict__ defl) {
+    __shared__ double red[kDeflThreads / 64 + 1];
+    const int64_t cells = int64_t(a.n[0]) * a.n[1] * a.n[2];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < cells; i += kDeflThreads) {
+        const unsigned f = a.flags[i];
+        float v = 0.0f;
+        if (!(f & kIdentity)) {
+            const int x = int(i % a.n[0]), y = int((i / a.n[0]) % a.n[1]), z = int(i / (int64_t(a.n[0]) * a.n[1]));
+            int k = outflow_faces(a, x, y, z);
+            for (int d = 0; d < 6; ++d) {
+                if (!(f & (1u << d))) continue;
+                int c[3] = {x, y, z};
+                const int ax = d >> 1;
+                c[ax] += (d & 1) ? 1 : -1;
+                if (c[ax] < 0) c[ax] += a.n[ax];
+                else if (c[ax] >= a.n[ax]) c[ax] -= a.n[ax];
+                k += (a.flags[lin(a, c[0], c[1], c[2])] & kIdentity) ? 1 : 0;
+            }
+            v = float(k);
+        }
+        av[i] = v;
+        s += v;
+    }
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) defl[0] = t;
+}
+
+// sum over i of f(i) by one block, eight independent loads in flight per thread
+template <class F>
+__device__ __forceinline__ double block_sum_of(int64_t cells, F f, double* red) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = threadIdx.x;
+    for (; i + 7 * kDeflThreads < cells; i += 8 * kDeflThreads) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += f(i + u * kDeflThreads);
+    }
+    for (; i < cells; i += kDeflThreads) acc[0] += f(i);
+    return block_sum(((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])), red);
+}
+
+// c = v.r / v.Av, rd = r - c A v (one block)
+__global__ __launch_bounds__(kDeflThreads) void mg_deflate_rhs_kernel(int64_t cells, const uint8_t* __restrict__ flags, const float* __restrict__ r,
+                                                                       const float* __restrict__ av, double* __restrict__ defl, float* __restrict__ rd) {
+    __shared__ double red[kDeflThreads / 64 + 1];
+    const double t = block_sum_of(cells, [&](int64_t i) { return (flags[i] & kIdentity) ? 0.0 : double(r[i]); }, red);
+    const double vav = defl[0];
+    const double c = vav > 0.0 ? t / vav : 0.0;
+    if (threadIdx.x == 0) defl[1] = c;
+    const float cf = float(c);
+#pragma unroll 8
+    for (int64_t i = threadIdx.x; i < cells; i += kDeflThreads) rd[i] = (flags[i] & kIdentity) ? 0.0f : r[i] - cf * av[i];
+}
+
+// z = z' + (c - Av.z' / v.Av) v (one block, in place)
+__global__ __launch_bounds__(kDeflThreads) void mg_deflate_solution_kernel(int64_t cells, const uint8_t* __restrict__ flags,
+                                                                            const float* __restrict__ av, const double* __restrict__ defl,
+                                                                            float* __restrict__ z) {
+    __shared__ double red[kDeflThreads / 64 + 1];
+    const double t = block_sum_of(cells, [&](int64_t i) { return double(av[i]) * double(z[i]); }, red);
+    const double vav = defl[0];
+    const float coef = vav > 0.0 ? float(defl[1] - t / vav) : 0.0f;
+#pragma unroll 8
+    for (int64_t i = threadIdx.x; i < cells; i += kDeflThreads)
+        if (!(flags[i] & kIdentity)) z[i] += coef;
+}
+
 dim3 cell_grid(const int n[3]) { return dim3(unsigned((n[0] + 63) / 64), unsigned((n[1] + 3) / 4), unsigned(n[2])); }
 
 LevelArgs level_args(const Context& c, const MgLevel& L) {
@@ -220,6 +385,8 @@ LevelArgs level_args(const Context& c, const MgLevel& L) {
     }
     for (int f = 0; f < 6; ++f) a.outflow[f] = c.grid.flow_type[f] == kOutflow;
     a.any_outflow = a.outflow[0] | a.outflow[1] | a.outflow[2] | a.outflow[3] | a.outflow[4] | a.outflow[5];
+    a.sy = L.n[0];
+    a.sz = L.n[0] * L.n[1];
     a.flags = L.flags;
     a.omega = c.mg.options.omega;
     return a;
@@ -277,29 +444,57 @@ void vcycle(Context& c, int l) {
     const Coarse none{nullptr, 0, 0};
     const auto& o = m.options;
     if (l == m.built - 1) {
+        // deflated right-hand side (what the coarse solver sees, and what the
+        // capture stores as training data)
+        mg_deflate_rhs_kernel<<<1, kDeflThreads, 0, c.stream>>>(L.cells, L.flags, L.r, L.av, L.defl, L.rd);
+        hip_check(hipGetLastError(), "mg_deflate_rhs_kernel");
+        if (m.capture && m.captured < m.capacity) {
+            hip_check(hipMemcpyAsync(m.capture + size_t(m.captured) * size_t(L.cells), L.rd, size_t(L.cells) * sizeof(float),
+                                     hipMemcpyDeviceToDevice, c.stream),
+                      "multilevel capture");
+            ++m.captured;
+        }
         if (o.coarse_solver == SMOKE_SIMULATION_COARSE_NETWORK) {
             if (!c.learned) throw HipError("multilevel preconditioner: no network attached");
-            preconditioner_apply(*c.learned, L.r, L.flags, L.n, L.z, c.stream);
+            preconditioner_apply(*c.learned, L.rd, L.flags, L.n, L.z, c.stream);
             // optional sweeps after the network's correction
             float* bufs[2] = {L.ta, L.tb};
             const float* cur = L.z;
             for (int k = 1; k <= o.coarse_sweeps; ++k) {
                 float* dst = bufs[(k - 1) & 1];
-                sweep(c, L, kBuffer, L.r, cur, none, dst);
+                sweep(c, L, kBuffer, L.rd, cur, none, dst);
                 cur = dst;
             }
             if (cur != L.z) hip_check(hipMemcpyAsync(L.z, cur, size_t(L.cells) * sizeof(float), hipMemcpyDeviceToDevice, c.stream), "coarse copy");
         } else {
-            sweeps_from_zero(c, L, std::max(1, o.coarse_sweeps), L.r, L.z);
+            sweeps_from_zero(c, L, std::max(1, o.coarse_sweeps), L.rd, L.z);
         }
+        mg_deflate_solution_kernel<<<1, kDeflThreads, 0, c.stream>>>(L.cells, L.flags, L.av, L.defl, L.z);
+        hip_check(hipGetLastError(), "mg_deflate_solution_kernel");
         return;
     }
     MgLevel& C = m.level[l + 1];
     const LevelArgs a = level_args(c, L);
-    // pre-smoothing: x on the fly (one sweep) or in ta
-    const float* xb = nullptr;
-    if (o.pre_sweeps >= 2) {
-        float* bufs[2] = {L.ta, L.tb};
+    const dim3 g = cell_grid(L.n);
+    const Coarse cc{C.z, C.n[0], C.n[0] * C.n[1]};
+    float* bufs[2] = {L.ta, L.tb};
+    // Large levels run every sweep from a stored iterate: the first sweep
+    // and the prolongation are element-wise passes, so no stencil tap needs
+    // its neighbour's flag byte (identity rows hold 0 in every stored
+    // iterate).  Small levels are bound by launches instead and recompute
+    // the first sweep and the prolongation at every tap.
+    const bool split = L.cells >= (int64_t(1) << 20);
+    float* xb = nullptr;   // x in a buffer; nullptr: x = omega D^-1 r on the fly
+    if (split) {
+        mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, L.r, L.ta);
+        hip_check(hipGetLastError(), "mg_scale_kernel");
+        xb = L.ta;
+        for (int k = 2; k <= o.pre_sweeps; ++k) {
+            float* dst = xb == L.ta ? L.tb : L.ta;
+            sweep(c, L, kBuffer, L.r, xb, none, dst);
+            xb = dst;
+        }
+    } else {
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = bufs[k & 1];
             if (k == 2) sweep(c, L, kFromR, L.r, nullptr, none, dst);
@@ -307,25 +502,28 @@ void vcycle(Context& c, int l) {
             xb = dst;
         }
     }
-    const dim3 gc = cell_grid(C.n);
-    if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.n[2], C.flags, C.r);
-    else mg_restrict_kernel<kFromR><<<gc, 256, 0, c.stream>>>(a, L.r, nullptr, C.n[0], C.n[1], C.n[2], C.flags, C.r);
+    // fine-cell threads, two z planes per block
+    const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
+    if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
+    else mg_restrict_kernel<kFromR><<<gc, 256, 0, c.stream>>>(a, L.r, nullptr, C.n[0], C.n[1], C.flags, C.r);
     hip_check(hipGetLastError(), "mg_restrict_kernel");
-    if (l + 1 == m.built - 1 && m.capture && m.captured < m.capacity) {
-        hip_check(hipMemcpyAsync(m.capture + size_t(m.captured) * size_t(C.cells), C.r, size_t(C.cells) * sizeof(float), hipMemcpyDeviceToDevice,
-                                 c.stream),
-                  "multilevel capture");
-        ++m.captured;
-    }
     vcycle(c, l + 1);
-    // post-smoothing: the first sweep adds the prolonged correction on the fly
-    const Coarse cc{C.z, C.n[0], C.n[1]};
-    float* other = (xb == L.ta) ? L.tb : L.ta;   // the buffer not holding x
-    float* first_out = o.post_sweeps == 1 ? L.z : other;
-    sweep(c, L, xb ? kBufferPlusC : kFromRPlusC, L.r, xb, cc, first_out);
-    const float* cur = first_out;
-    float* bufs[2] = {L.ta, L.tb};
-    for (int k = 2; k <= o.post_sweeps; ++k) {
+    // post-smoothing from x + P z_(l+1)
+    const float* cur;
+    int done_sweeps;
+    if (split) {
+        mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, xb);
+        hip_check(hipGetLastError(), "mg_prolong_kernel");
+        cur = xb;
+        done_sweeps = 0;
+    } else {
+        float* other = (xb == L.ta) ? L.tb : L.ta;   // the buffer not holding x
+        float* first_out = o.post_sweeps == 1 ? L.z : other;
+        sweep(c, L, xb ? kBufferPlusC : kFromRPlusC, L.r, xb, cc, first_out);
+        cur = first_out;
+        done_sweeps = 1;
+    }
+    for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
         sweep(c, L, kBuffer, L.r, cur, none, dst);
         cur = dst;
@@ -378,6 +576,9 @@ void multilevel_release(Context& c) {
         }
         free_level(L.ta);
         free_level(L.tb);
+        free_level(L.av);
+        free_level(L.rd);
+        free_level(L.defl);
         L = MgLevel{};
     }
     m.allocated = 0;
@@ -404,6 +605,11 @@ void multilevel_build(Context& c) {
             }
             alloc_level(L.ta, L.cells);
             alloc_level(L.tb, L.cells);
+            if (l == levels - 1) {
+                alloc_level(L.av, L.cells);
+                alloc_level(L.rd, L.cells);
+                alloc_level(L.defl, 2);
+            }
             for (int k = 0; k < 3; ++k) n[k] = (n[k] + 1) / 2;
         }
         m.allocated = levels;
@@ -421,6 +627,9 @@ void multilevel_build(Context& c) {
         mg_flags_kernel<<<g, 256, 0, c.stream>>>(level_args(c, L), L.occ, L.anchor, L.flags);
         hip_check(hipGetLastError(), "multilevel hierarchy");
     }
+    MgLevel& B = m.level[levels - 1];
+    mg_deflation_setup_kernel<<<1, kDeflThreads, 0, c.stream>>>(level_args(c, B), B.av, B.defl);
+    hip_check(hipGetLastError(), "mg_deflation_setup_kernel");
     m.built = levels;
 }
 

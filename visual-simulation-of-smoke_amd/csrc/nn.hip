@@ -65,18 +65,23 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* __restrict__ p) {
 
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0}); }
 
-// y (or dx) for one 16-voxel x segment of one output row per wave.
-// Epilogue: + bias, + add, relu, * [mask > 0].
-template <int NT>
+// y (or dx) for one 16-voxel x segment of one output row: per wave
+// (SPLIT = 1), or per block with the K steps dealt round-robin over its four
+// waves and summed through LDS (SPLIT = 4; small volumes, where one wave per
+// tile leaves the chip idle and every wave waits out its chain of gathers).
+// The next K step's loads are issued before the current step's matrix-core
+// work.  Epilogue: + bias, + add, relu, * [mask > 0].
+template <int NT, int SPLIT>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                    const float* __restrict__ bias, const uint16_t* __restrict__ add,
                                                    const uint16_t* __restrict__ mask, int relu, uint16_t* __restrict__ y,
                                                    float* __restrict__ yf, int f32_channels) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int wib = threadIdx.x >> 6;   // wave in block
+    const int64_t wave = SPLIT == 1 ? int64_t(blockIdx.x) * 4 + wib : int64_t(blockIdx.x);
     const int xt = (a.out[0] + 15) / 16;
     const int64_t tiles = int64_t(a.batch) * a.out[2] * a.out[1] * xt;
-    if (wave >= tiles) return;
+    if (wave >= tiles) return;   // uniform per block when SPLIT = 4
     int64_t t = wave;
     const int xtile = int(t % xt);
     t /= xt;
@@ -87,29 +92,59 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, const uint16_t* _
     const int m = lane & 15, g = lane >> 4;
     const int ox = xtile * 16 + m;   // the voxel whose A row this lane feeds
 
-    float4v acc[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[nt] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
-    const int ksteps = a.kpad / kStep;
-    for (int s = 0; s < ksteps; ++s) {
+    auto load_a = [&](int s) -> bf16x8 {
         const int kb = s * kStep + g * 8;
         const int tap = kb / a.cin;
         const int c0 = kb - tap * a.cin;
-        bf16x8 av = zero8();
         if (tap < 27 && ox < a.out[0]) {
             const int tx = tap % 3, ty = (tap / 3) % 3, tz = tap / 9;
             int ix, iy, iz;
             if (tap_source(a.mode, ox, tx, a.in[0], ix) && tap_source(a.mode, oy, ty, a.in[1], iy) &&
                 tap_source(a.mode, oz, tz, a.in[2], iz)) {
                 const int64_t idx = (((int64_t(b) * a.in[2] + iz) * a.in[1] + iy) * a.in[0] + ix) * a.cin + c0;
-                av = load8(x + idx);
+                return load8(x + idx);
             }
         }
+        return zero8();
+    };
+
+    float4v acc[NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const bf16x8 bv = load8(w + int64_t(nt * 16 + m) * a.kpad + kb);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[nt], 0, 0, 0);
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const int ksteps = a.kpad / kStep;
+    const int first = SPLIT == 1 ? 0 : wib;
+    if (first < ksteps) {
+        bf16x8 av = load_a(first);
+        bf16x8 bv[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bv[nt] = load8(w + int64_t(nt * 16 + m) * a.kpad + first * kStep + g * 8);
+        for (int s = first; s < ksteps; s += SPLIT) {
+            const int sn = s + SPLIT;
+            bf16x8 an = zero8();
+            bf16x8 bn[NT];
+            if (sn < ksteps) {
+                an = load_a(sn);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) bn[nt] = load8(w + int64_t(nt * 16 + m) * a.kpad + sn * kStep + g * 8);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nt], acc[nt], 0, 0, 0);
+            av = an;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bv[nt] = bn[nt];
         }
+    }
+    if constexpr (SPLIT > 1) {
+        // waves 1..3 hand their partial sums to wave 0 (fixed order: deterministic)
+        __shared__ float4v part[3][NT][64];
+        if (wib > 0) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) part[wib - 1][nt][lane] = acc[nt];
+        }
+        __syncthreads();
+        if (wib > 0) return;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = ((acc[nt] + part[0][nt][lane]) + part[1][nt][lane]) + part[2][nt][lane];
     }
     // lane holds D[4 g + i][m]: voxel xtile * 16 + 4 g + i, channel nt * 16 + m
 #pragma unroll
@@ -257,17 +292,27 @@ ConvArgs dual(const ConvArgs& a) {
     return d;
 }
 
+// Volumes with fewer tiles than this split each tile's K steps over the four
+// waves of a block.
+constexpr int64_t kSplitTiles = 8192;
+
+template <int SPLIT>
+void launch_conv_split(const ConvArgs& a, unsigned blocks, const uint16_t* x, const uint16_t* w, const float* bias, const uint16_t* add,
+                       const uint16_t* mask, int relu, uint16_t* y, float* yf, int f32_channels, hipStream_t s) {
+    switch (a.cout / 16) {
+        case 1: conv_kernel<1, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 2: conv_kernel<2, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 3: conv_kernel<3, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 4: conv_kernel<4, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        default: throw HipError("conv: cout must be 16..64");
+    }
+}
+
 void launch_conv(const ConvArgs& a, const uint16_t* x, const uint16_t* w, const float* bias, const uint16_t* add, const uint16_t* mask, int relu,
                  uint16_t* y, float* yf, int f32_channels, hipStream_t s) {
     const int64_t tiles = int64_t(a.batch) * a.out[2] * a.out[1] * ((a.out[0] + 15) / 16);
-    const unsigned blocks = unsigned((tiles + 3) / 4);
-    switch (a.cout / 16) {
-        case 1: conv_kernel<1><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 2: conv_kernel<2><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 3: conv_kernel<3><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 4: conv_kernel<4><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        default: throw HipError("conv: cout must be 16..64");
-    }
+    if (tiles < kSplitTiles) launch_conv_split<4>(a, unsigned(tiles), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
+    else launch_conv_split<1>(a, unsigned((tiles + 3) / 4), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
     hip_check(hipGetLastError(), "conv_kernel");
 }
 
