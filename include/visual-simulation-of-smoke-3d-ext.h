@@ -127,6 +127,9 @@ typedef struct SmokeSimulationKernelTiming {
     double pcg_reduction_ms;      /* summed device time of the two one-block finalize kernels (alpha; beta, stop test) */
     double step_ms;               /* summed device time of whole steps */
     uint64_t steps;
+    double mg_sweep_ms;           /* multilevel preconditioner: summed device time of the last damped-Jacobi
+                                     sweep on the pressure grid of every V-cycle (bracketed by events) */
+    uint64_t mg_sweeps;           /* sweeps so timed */
 } SmokeSimulationKernelTiming;
 
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_default_solver_options(SmokeSimulationSolverOptions* out_options);
@@ -308,6 +311,25 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_apply_multilevel
 /* Copies the flag byte of one level (bits as in the training crops) to a
    device buffer of dims[0]*dims[1]*dims[2] bytes (stream-ordered). */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_level_flags(SmokeSimulationContext context, int32_t level, uint8_t* flags);
+
+/* Device time of the V-cycle's kernels on the pressure grid, measured with
+   HIP events on the context's stream: each kernel is launched `repetitions`
+   times back to back on the current hierarchy with the solve's residual as
+   input (solver scratch only; the state is not touched) and the average per
+   launch is reported, together with one whole V-cycle.  Synchronous. */
+typedef struct SmokeSimulationMultilevelTiming {
+    float sweep_us;          /* one damped-Jacobi sweep on the pressure grid */
+    float restrict_us;       /* residual + restriction to level 1 */
+    float scale_us;          /* first sweep from zero (x = omega D^-1 r) */
+    float prolong_us;        /* prolongation of the level-1 correction */
+    float vcycle_us;         /* one whole application z = M(r) */
+    int32_t vector_rows;     /* 1 when the pressure grid runs the float4-row kernels */
+    int32_t repetitions;
+    int32_t reserved;
+} SmokeSimulationMultilevelTiming;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_time_multilevel(SmokeSimulationContext context, int32_t repetitions,
+                                                                              SmokeSimulationMultilevelTiming* out_timing);
 
 /* Training data of the coarse-grid network: while set, every V-cycle copies
    its deflated coarsest-level right-hand side P r (the input of S) into the

@@ -104,6 +104,8 @@ void destroy_resources(Context& c) {
     multilevel_release(c);
     if (c.host_status) hipHostFree(c.host_status);
     c.host_status = nullptr;
+    if (c.host_scalars) hipHostFree(c.host_scalars);
+    c.host_scalars = nullptr;
     for (hipEvent_t& e : c.status_event)
         if (e) hipEventDestroy(e), e = nullptr;
     for (auto& row : c.timing_events)
@@ -370,6 +372,9 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         void* pinned = nullptr;
         hip_check(hipHostMalloc(&pinned, 2 * sizeof(smoke::PcgStatus), hipHostMallocDefault), "hipHostMalloc");
         c.host_status = static_cast<smoke::PcgStatus*>(pinned);
+        pinned = nullptr;
+        hip_check(hipHostMalloc(&pinned, 2 * sizeof(smoke::PcgScalars), hipHostMallocDefault), "hipHostMalloc");
+        c.host_scalars = static_cast<smoke::PcgScalars*>(pinned);
         for (hipEvent_t& e : c.status_event) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         for (auto& row : c.timing_events)
             for (hipEvent_t& e : row) hip_check(hipEventCreate(&e), "event");
@@ -704,15 +709,18 @@ SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContex
 
 SmokeSimulationResult smoke_simulation_ext_set_kernel_timing(SmokeSimulationContext context, int32_t enable) {
     return smoke::guarded_call(context, [&](Context& c) {
+        smoke::multilevel_drain_timing(c);   // pending pairs belong to the old totals
         c.timing = enable != 0;
         c.timing_totals = SmokeSimulationKernelTiming{};
     });
 }
 
 SmokeSimulationResult smoke_simulation_ext_get_kernel_timing(SmokeSimulationContext context, SmokeSimulationKernelTiming* out) {
-    if (!context || !out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
-    *out = context->timing_totals;
-    return SMOKE_SIMULATION_RESULT_OK;
+    if (!out) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::multilevel_drain_timing(c);
+        *out = c.timing_totals;
+    });
 }
 
 SmokeSimulationResult smoke_simulation_ext_plume_sources(SmokeSimulationContext context, const SmokeSimulationPlumeSource* src, uint32_t step) {
@@ -833,6 +841,14 @@ SmokeSimulationResult smoke_simulation_ext_export_level_flags(SmokeSimulationCon
         if (level < 0 || level >= c.mg.built) throw smoke::HipError("no such level");
         const smoke::MgLevel& L = c.mg.level[level];
         hip_check(hipMemcpyAsync(flags, L.flags, size_t(L.cells), hipMemcpyDeviceToDevice, c.stream), "level flags");
+    });
+}
+
+SmokeSimulationResult smoke_simulation_ext_time_multilevel(SmokeSimulationContext context, int32_t repetitions, SmokeSimulationMultilevelTiming* out) {
+    if (!out || repetitions <= 0) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        smoke::ensure_pcg_z(c);
+        *out = smoke::multilevel_timing(c, repetitions);
     });
 }
 

@@ -65,7 +65,7 @@ struct MgLevel {
     // coarsest level: deflation of v = 1 on the fluid rows
     float* av = nullptr;     // A v
     float* rd = nullptr;     // deflated right-hand side r - c A v
-    double* defl = nullptr;  // v^T A v, c = v^T r / v^T A v
+    double* defl = nullptr;  // v^T A v, c = v^T r / v^T A v, then per-block partial sums
 };
 
 constexpr int kMaxLevels = 8;
@@ -78,6 +78,13 @@ struct Multilevel {
     float* capture = nullptr;    // coarsest-level right-hand sides (training data), not owned
     uint32_t capacity = 0;
     uint32_t captured = 0;
+    // kernel timing: a ring of event pairs around the pressure grid's last
+    // sweep of each V-cycle, read back once the ring comes round (by then the
+    // pair has long completed) or when the totals are read
+    static constexpr int kTimingRing = 64;
+    hipEvent_t timing_events[kTimingRing][2] = {};
+    bool timing_pending[kTimingRing] = {};
+    int timing_next = 0;
 };
 
 struct Context {
@@ -133,6 +140,7 @@ struct Context {
     size_t partial_capacity = 0;
     int pcg_stencil_blocks = 0;          // blocks of the last stencil-pass launch (partials to sum)
     PcgStatus* host_status = nullptr;   // pinned, two slots
+    PcgScalars* host_scalars = nullptr; // pinned, two slots (flexible PCG: whole scalar block per iteration)
     hipEvent_t status_event[2] = {};
 
     // time step
@@ -197,6 +205,8 @@ bool multilevel_options_valid(const Context& c, const SmokeSimulationMultilevelO
 void multilevel_build(Context& c);                          // hierarchy of the current occupancy (if stale)
 void multilevel_apply(Context& c, const float* r, float* z);   // z = M(r), one V-cycle
 void multilevel_release(Context& c);
+SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps);
+void multilevel_drain_timing(Context& c);                  // fold pending sweep events into timing_totals
 
 // Training-pair crops of the last pressure solve (training_data.hip).
 void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& crops);

@@ -180,6 +180,154 @@ __global__ __launch_bounds__(256) void mg_sweep_kernel(LevelArgs a, const float*
     out[i] = xi + a.omega * res * inv;
 }
 
+// ---- float4 row kernels for large levels with nx % 4 == 0: a thread owns
+// four consecutive x cells of one row; the rows at y +- 1 and z +- 1 are
+// float4 loads, the cells at x - 1 and x + 4 scalar loads.  A neighbour
+// outside a non-periodic domain has no flag bit, so its (clamped, in-bounds)
+// load is never used.
+struct Row4 {
+    int i0;                 // index of the first cell
+    int x0, y, z;
+    unsigned f[4];          // flag bytes
+};
+
+__device__ __forceinline__ Row4 row4_at(const LevelArgs& a, int x0, int y, int z) {
+    Row4 q;
+    q.x0 = x0;
+    q.y = y;
+    q.z = z;
+    q.i0 = lin(a, x0, y, z);
+    const unsigned w = *reinterpret_cast<const unsigned*>(a.flags + q.i0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q.f[k] = (w >> (8 * k)) & 0xffu;
+    return q;
+}
+
+__device__ __forceinline__ float4 ld4(const float* __restrict__ p, int i) { return *reinterpret_cast<const float4*>(p + i); }
+__device__ __forceinline__ float comp(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// (r - A x) of the four cells (0 where identity) and their 1 / diag; x from a
+// buffer that holds 0 on identity rows
+__device__ __forceinline__ void row4_residual(const LevelArgs& a, const float* __restrict__ r, const float* __restrict__ xb, const Row4& q,
+                                              float res[4], float xi[4], float inv[4]) {
+    const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+    const int row = q.i0 - q.x0;
+    const int ym = q.y > 0 ? q.y - 1 : (a.periodic[1] ? ny - 1 : q.y);
+    const int yp = q.y < ny - 1 ? q.y + 1 : (a.periodic[1] ? 0 : q.y);
+    const int zm = q.z > 0 ? q.z - 1 : (a.periodic[2] ? nz - 1 : q.z);
+    const int zp = q.z < nz - 1 ? q.z + 1 : (a.periodic[2] ? 0 : q.z);
+    const int im1 = q.x0 > 0 ? q.i0 - 1 : (a.periodic[0] ? row + nx - 1 : q.i0);
+    const int ip4 = q.x0 + 4 < nx ? q.i0 + 4 : (a.periodic[0] ? row : q.i0);
+    const float4 rv = ld4(r, q.i0);
+    const float4 xv = ld4(xb, q.i0);
+    const float4 xym = ld4(xb, q.i0 + (ym - q.y) * a.sy);
+    const float4 xyp = ld4(xb, q.i0 + (yp - q.y) * a.sy);
+    const float4 xzm = ld4(xb, q.i0 + (zm - q.z) * a.sz);
+    const float4 xzp = ld4(xb, q.i0 + (zp - q.z) * a.sz);
+    const float xm1 = xb[im1], xp4 = xb[ip4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned f = q.f[k];
+        const float xk = comp(xv, k);
+        const float left = k == 0 ? xm1 : comp(xv, k - 1);
+        const float right = k == 3 ? xp4 : comp(xv, k + 1);
+        float s = 0.0f;
+        s += (f & 1u) ? left : 0.0f;
+        s += (f & 2u) ? right : 0.0f;
+        s += (f & 4u) ? comp(xym, k) : 0.0f;
+        s += (f & 8u) ? comp(xyp, k) : 0.0f;
+        s += (f & 16u) ? comp(xzm, k) : 0.0f;
+        s += (f & 32u) ? comp(xzp, k) : 0.0f;
+        const int d = row_diag(a, f, q.x0 + k, q.y, q.z);
+        inv[k] = recip(d);
+        xi[k] = xk;
+        res[k] = (f & kIdentity) ? 0.0f : comp(rv, k) - (float(d) * xk - s);
+    }
+}
+
+__global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb,
+                                                        float* __restrict__ out) {
+    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x0 >= a.n[0] || y >= a.n[1]) return;
+    const Row4 q = row4_at(a, x0, y, z);
+    float res[4], xi[4], inv[4];
+    row4_residual(a, r, xb, q, res, xi, inv);
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (q.f[k] & kIdentity) ? 0.0f : xi[k] + a.omega * res[k] * inv[k];
+    *reinterpret_cast<float4*>(out + q.i0) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ __launch_bounds__(256) void mg_scale4_kernel(LevelArgs a, const float* __restrict__ r, float* __restrict__ out) {
+    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x0 >= a.n[0] || y >= a.n[1]) return;
+    const Row4 q = row4_at(a, x0, y, z);
+    const float4 rv = ld4(r, q.i0);
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        o[k] = (q.f[k] & kIdentity) ? 0.0f : a.omega * comp(rv, k) * recip(row_diag(a, q.f[k], x0 + k, y, z));
+    *reinterpret_cast<float4*>(out + q.i0) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// x += correction of the parent cell on fluid rows; the four cells have the
+// parents X = x0 / 2 and X + 1 (x0 % 4 == 0, so the pair is one aligned float2)
+__global__ __launch_bounds__(256) void mg_prolong4_kernel(LevelArgs a, Coarse cc, float* __restrict__ xb) {
+    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x0 >= a.n[0] || y >= a.n[1]) return;
+    const Row4 q = row4_at(a, x0, y, z);
+    const float2 e = *reinterpret_cast<const float2*>(cc.e + parent(cc, x0, y, z));
+    float4 v = ld4(xb, q.i0);
+    if (!(q.f[0] & kIdentity)) v.x += e.x;
+    if (!(q.f[1] & kIdentity)) v.y += e.x;
+    if (!(q.f[2] & kIdentity)) v.z += e.y;
+    if (!(q.f[3] & kIdentity)) v.w += e.y;
+    *reinterpret_cast<float4*>(xb + q.i0) = v;
+}
+
+// restriction with float4 rows: a 64 x 4 block covers 256 x cells, 4 rows and
+// two z planes; each thread sums its 2 x 4 cells into two coarse x cells, the
+// y pair is summed through LDS
+__global__ __launch_bounds__(256) void mg_restrict4_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb, int nc0, int nc1,
+                                                           const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
+    __shared__ float2 rows[4][64];
+    const int lane = threadIdx.x & 63, rowi = threadIdx.x >> 6;
+    const int x0 = (blockIdx.x * 64 + lane) * 4;
+    const int y = blockIdx.y * 4 + rowi;
+    const int Z = blockIdx.z;
+    float s0 = 0.0f, s1 = 0.0f;
+    if (x0 < a.n[0] && y < a.n[1]) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int z = 2 * Z + k;
+            if (z >= a.n[2]) break;
+            const Row4 q = row4_at(a, x0, y, z);
+            float res[4], xi[4], inv[4];
+            row4_residual(a, r, xb, q, res, xi, inv);
+            s0 += res[0] + res[1];
+            s1 += res[2] + res[3];
+        }
+    }
+    rows[rowi][lane] = make_float2(s0, s1);
+    __syncthreads();
+    if (rowi & 1) return;
+    const int X = x0 >> 1, Y = y >> 1;
+    if (X >= nc0 || Y >= nc1) return;
+    const float2 o = rows[rowi + 1][lane];
+    const int ci = (Z * nc1 + Y) * nc0 + X;
+    // X is even and nc0 = nx / 2 is even: the pair is one aligned float2
+    const unsigned cf = *reinterpret_cast<const unsigned short*>(cflags + ci);
+    const float v0 = (cf & kIdentity) ? 0.0f : 0.5f * (s0 + o.x);
+    const float v1 = ((cf >> 8) & kIdentity) ? 0.0f : 0.5f * (s1 + o.y);
+    *reinterpret_cast<float2*>(rc + ci) = make_float2(v0, v1);
+}
+
 // x = omega D^-1 r (one sweep from zero) on fluid rows, 0 on identity rows
 __global__ __launch_bounds__(256) void mg_scale_kernel(LevelArgs a, const float* __restrict__ r, float* __restrict__ out) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -336,43 +484,58 @@ __global__ __launch_bounds__(kDeflThreads) void mg_deflation_setup_kernel(LevelA
     if (threadIdx.x == 0) defl[0] = t;
 }
 
-// sum over i of f(i) by one block, eight independent loads in flight per thread
-template <class F>
-__device__ __forceinline__ double block_sum_of(int64_t cells, F f, double* red) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t i = threadIdx.x;
-    for (; i + 7 * kDeflThreads < cells; i += 8 * kDeflThreads) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] += f(i + u * kDeflThreads);
+// The deflation's two global sums (v.r before the coarse solve, Av.z' after
+// it) are per-block partial sums followed by an element-wise kernel whose
+// every block adds the partials in the same order (deterministic).
+constexpr int kDeflBlocks = 64;
+
+__device__ __forceinline__ double wave_block_sum256(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// MODE 0: partial sums of r over fluid rows; MODE 1: of Av . z
+template <int MODE>
+__global__ __launch_bounds__(256) void mg_defl_sum_kernel(int cells, const uint8_t* __restrict__ flags, const float* __restrict__ v,
+                                                          const float* __restrict__ av, double* __restrict__ partials) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < cells; i += kDeflBlocks * 256) {
+        if constexpr (MODE == 0) s += (flags[i] & kIdentity) ? 0.0 : double(v[i]);
+        else s += double(av[i]) * double(v[i]);
     }
-    for (; i < cells; i += kDeflThreads) acc[0] += f(i);
-    return block_sum(((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])), red);
+    const double t = wave_block_sum256(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// c = v.r / v.Av, rd = r - c A v (one block)
-__global__ __launch_bounds__(kDeflThreads) void mg_deflate_rhs_kernel(int64_t cells, const uint8_t* __restrict__ flags, const float* __restrict__ r,
-                                                                       const float* __restrict__ av, double* __restrict__ defl, float* __restrict__ rd) {
-    __shared__ double red[kDeflThreads / 64 + 1];
-    const double t = block_sum_of(cells, [&](int64_t i) { return (flags[i] & kIdentity) ? 0.0 : double(r[i]); }, red);
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partials) {
+    double t = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < kDeflBlocks; ++b) t += partials[b];
+    return t;
+}
+
+// rd = r - c A v with c = v.r / v.Av (block 0 stores c)
+__global__ __launch_bounds__(256) void mg_deflate_rhs_kernel(int cells, const uint8_t* __restrict__ flags, const float* __restrict__ r,
+                                                             const float* __restrict__ av, double* __restrict__ defl,
+                                                             const double* __restrict__ partials, float* __restrict__ rd) {
     const double vav = defl[0];
-    const double c = vav > 0.0 ? t / vav : 0.0;
-    if (threadIdx.x == 0) defl[1] = c;
+    const double c = vav > 0.0 ? sum_partials(partials) / vav : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) defl[1] = c;
     const float cf = float(c);
-#pragma unroll 8
-    for (int64_t i = threadIdx.x; i < cells; i += kDeflThreads) rd[i] = (flags[i] & kIdentity) ? 0.0f : r[i] - cf * av[i];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < cells) rd[i] = (flags[i] & kIdentity) ? 0.0f : r[i] - cf * av[i];
 }
 
-// z = z' + (c - Av.z' / v.Av) v (one block, in place)
-__global__ __launch_bounds__(kDeflThreads) void mg_deflate_solution_kernel(int64_t cells, const uint8_t* __restrict__ flags,
-                                                                            const float* __restrict__ av, const double* __restrict__ defl,
-                                                                            float* __restrict__ z) {
-    __shared__ double red[kDeflThreads / 64 + 1];
-    const double t = block_sum_of(cells, [&](int64_t i) { return double(av[i]) * double(z[i]); }, red);
+// z = z' + (c - Av.z' / v.Av) v
+__global__ __launch_bounds__(256) void mg_deflate_solution_kernel(int cells, const uint8_t* __restrict__ flags, const double* __restrict__ defl,
+                                                                  const double* __restrict__ partials, float* __restrict__ z) {
     const double vav = defl[0];
-    const float coef = vav > 0.0 ? float(defl[1] - t / vav) : 0.0f;
-#pragma unroll 8
-    for (int64_t i = threadIdx.x; i < cells; i += kDeflThreads)
-        if (!(flags[i] & kIdentity)) z[i] += coef;
+    const float coef = vav > 0.0 ? float(defl[1] - sum_partials(partials) / vav) : 0.0f;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < cells && !(flags[i] & kIdentity)) z[i] += coef;
 }
 
 dim3 cell_grid(const int n[3]) { return dim3(unsigned((n[0] + 63) / 64), unsigned((n[1] + 3) / 4), unsigned(n[2])); }
@@ -438,6 +601,31 @@ void sweeps_from_zero(Context& c, const MgLevel& L, int count, const float* r, f
     }
 }
 
+void fold_timing_pair(Context& c, int k) {
+    Multilevel& m = c.mg;
+    if (!m.timing_pending[k]) return;
+    hip_check(hipEventSynchronize(m.timing_events[k][1]), "timing event");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, m.timing_events[k][0], m.timing_events[k][1]), "elapsed");
+    c.timing_totals.mg_sweep_ms += ms;
+    c.timing_totals.mg_sweeps += 1;
+    m.timing_pending[k] = false;
+}
+
+// the event pair for the next timed launch (created on first use)
+hipEvent_t* next_timing_pair(Context& c) {
+    Multilevel& m = c.mg;
+    const int k = m.timing_next;
+    m.timing_next = (k + 1) % Multilevel::kTimingRing;
+    fold_timing_pair(c, k);
+    if (!m.timing_events[k][0]) {
+        hip_check(hipEventCreate(&m.timing_events[k][0]), "event");
+        hip_check(hipEventCreate(&m.timing_events[k][1]), "event");
+    }
+    m.timing_pending[k] = true;
+    return m.timing_events[k];
+}
+
 void vcycle(Context& c, int l) {
     Multilevel& m = c.mg;
     MgLevel& L = m.level[l];
@@ -446,7 +634,10 @@ void vcycle(Context& c, int l) {
     if (l == m.built - 1) {
         // deflated right-hand side (what the coarse solver sees, and what the
         // capture stores as training data)
-        mg_deflate_rhs_kernel<<<1, kDeflThreads, 0, c.stream>>>(L.cells, L.flags, L.r, L.av, L.defl, L.rd);
+        const int cells = int(L.cells);
+        const unsigned eb = unsigned((cells + 255) / 256);
+        mg_defl_sum_kernel<0><<<kDeflBlocks, 256, 0, c.stream>>>(cells, L.flags, L.r, nullptr, L.defl + 2);
+        mg_deflate_rhs_kernel<<<eb, 256, 0, c.stream>>>(cells, L.flags, L.r, L.av, L.defl, L.defl + 2, L.rd);
         hip_check(hipGetLastError(), "mg_deflate_rhs_kernel");
         if (m.capture && m.captured < m.capacity) {
             hip_check(hipMemcpyAsync(m.capture + size_t(m.captured) * size_t(L.cells), L.rd, size_t(L.cells) * sizeof(float),
@@ -469,7 +660,8 @@ void vcycle(Context& c, int l) {
         } else {
             sweeps_from_zero(c, L, std::max(1, o.coarse_sweeps), L.rd, L.z);
         }
-        mg_deflate_solution_kernel<<<1, kDeflThreads, 0, c.stream>>>(L.cells, L.flags, L.av, L.defl, L.z);
+        mg_defl_sum_kernel<1><<<kDeflBlocks, 256, 0, c.stream>>>(cells, L.flags, L.z, L.av, L.defl + 2);
+        mg_deflate_solution_kernel<<<eb, 256, 0, c.stream>>>(cells, L.flags, L.defl, L.defl + 2, L.z);
         hip_check(hipGetLastError(), "mg_deflate_solution_kernel");
         return;
     }
@@ -484,14 +676,20 @@ void vcycle(Context& c, int l) {
     // iterate).  Small levels are bound by launches instead and recompute
     // the first sweep and the prolongation at every tap.
     const bool split = L.cells >= (int64_t(1) << 20);
+    // float4 rows: x extent a multiple of 4 (and of 8 when the coarse x pair of
+    // the restriction must be a float2-aligned pair: nc0 even)
+    const bool vec = split && L.n[0] % 8 == 0;
+    const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
     float* xb = nullptr;   // x in a buffer; nullptr: x = omega D^-1 r on the fly
     if (split) {
-        mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, L.r, L.ta);
+        if (vec) mg_scale4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, L.ta);
+        else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, L.r, L.ta);
         hip_check(hipGetLastError(), "mg_scale_kernel");
         xb = L.ta;
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = xb == L.ta ? L.tb : L.ta;
-            sweep(c, L, kBuffer, L.r, xb, none, dst);
+            if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, xb, dst);
+            else sweep(c, L, kBuffer, L.r, xb, none, dst);
             xb = dst;
         }
     } else {
@@ -504,7 +702,9 @@ void vcycle(Context& c, int l) {
     }
     // fine-cell threads, two z planes per block
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
+    const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
+    if (vec) mg_restrict4_kernel<<<gc4, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
+    else if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
     else mg_restrict_kernel<kFromR><<<gc, 256, 0, c.stream>>>(a, L.r, nullptr, C.n[0], C.n[1], C.flags, C.r);
     hip_check(hipGetLastError(), "mg_restrict_kernel");
     vcycle(c, l + 1);
@@ -512,7 +712,8 @@ void vcycle(Context& c, int l) {
     const float* cur;
     int done_sweeps;
     if (split) {
-        mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, xb);
+        if (vec) mg_prolong4_kernel<<<g4, 256, 0, c.stream>>>(a, cc, xb);
+        else mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, xb);
         hip_check(hipGetLastError(), "mg_prolong_kernel");
         cur = xb;
         done_sweeps = 0;
@@ -525,7 +726,12 @@ void vcycle(Context& c, int l) {
     }
     for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
-        sweep(c, L, kBuffer, L.r, cur, none, dst);
+        hipEvent_t* ev = (c.timing && l == 0 && k == o.post_sweeps) ? next_timing_pair(c) : nullptr;
+        if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
+        if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst);
+        else sweep(c, L, kBuffer, L.r, cur, none, dst);
+        hip_check(hipGetLastError(), "mg_sweep_kernel");
+        if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
         cur = dst;
     }
 }
@@ -563,8 +769,16 @@ bool multilevel_options_valid(const Context& c, const SmokeSimulationMultilevelO
     return true;
 }
 
+void multilevel_drain_timing(Context& c) {
+    for (int k = 0; k < Multilevel::kTimingRing; ++k) fold_timing_pair(c, k);
+}
+
 void multilevel_release(Context& c) {
     Multilevel& m = c.mg;
+    for (auto& pair : m.timing_events)
+        for (hipEvent_t& e : pair)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+    for (bool& p : m.timing_pending) p = false;
     for (int l = 0; l < kMaxLevels; ++l) {
         MgLevel& L = m.level[l];
         if (l > 0) {
@@ -608,7 +822,7 @@ void multilevel_build(Context& c) {
             if (l == levels - 1) {
                 alloc_level(L.av, L.cells);
                 alloc_level(L.rd, L.cells);
-                alloc_level(L.defl, 2);
+                alloc_level(L.defl, 2 + kDeflBlocks);   // v.Av, c, partial sums
             }
             for (int k = 0; k < 3; ++k) n[k] = (n[k] + 1) / 2;
         }
@@ -641,6 +855,69 @@ void multilevel_apply(Context& c, const float* r, float* z) {
     vcycle(c, 0);
     L0.r = nullptr;
     L0.z = nullptr;
+}
+
+}  // namespace smoke
+
+namespace smoke {
+
+SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
+    multilevel_build(c);
+    Multilevel& m = c.mg;
+    if (m.built < 2) throw HipError("no hierarchy");
+    if (!c.pcg_z) throw HipError("multilevel timing needs the preconditioner buffers");
+    MgLevel& L = m.level[0];
+    const MgLevel& C = m.level[1];
+    const LevelArgs a = level_args(c, L);
+    const bool vec = L.cells >= (int64_t(1) << 20) && L.n[0] % 8 == 0;
+    const dim3 g = cell_grid(L.n);
+    const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
+    const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
+    const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
+    const Coarse none{nullptr, 0, 0};
+    const Coarse cc{C.z, C.n[0], C.n[0] * C.n[1]};
+    const float* r = c.pcg_r;
+    hipEvent_t t0, t1;
+    hip_check(hipEventCreate(&t0), "event");
+    hip_check(hipEventCreate(&t1), "event");
+    auto time = [&](auto&& launch) {
+        launch();   // warm-up
+        hip_check(hipEventRecord(t0, c.stream), "event");
+        for (int k = 0; k < reps; ++k) launch();
+        hip_check(hipEventRecord(t1, c.stream), "event");
+        hip_check(hipEventSynchronize(t1), "event sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+        hip_check(hipGetLastError(), "multilevel timing");
+        return 1e3f * ms / float(reps);
+    };
+    SmokeSimulationMultilevelTiming out{};
+    out.vector_rows = vec ? 1 : 0;
+    out.repetitions = reps;
+    out.scale_us = time([&] {
+        if (vec) mg_scale4_kernel<<<g4, 256, 0, c.stream>>>(a, r, L.ta);
+        else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, r, L.ta);
+    });
+    out.sweep_us = time([&] {
+        if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, r, L.ta, L.tb);
+        else mg_sweep_kernel<kBuffer><<<g, 256, 0, c.stream>>>(a, r, L.ta, none, L.tb);
+    });
+    out.restrict_us = time([&] {
+        if (vec) mg_restrict4_kernel<<<gc4, 256, 0, c.stream>>>(a, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
+        else mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
+    });
+    hip_check(hipMemsetAsync(C.z, 0, size_t(C.cells) * sizeof(float), c.stream), "zero correction");
+    out.prolong_us = time([&] {
+        if (vec) mg_prolong4_kernel<<<g4, 256, 0, c.stream>>>(a, cc, L.tb);
+        else mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, L.tb);
+    });
+    float* saved = m.capture;
+    m.capture = nullptr;   // timing must not add training samples
+    out.vcycle_us = time([&] { multilevel_apply(c, r, c.pcg_z); });
+    m.capture = saved;
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return out;
 }
 
 }  // namespace smoke

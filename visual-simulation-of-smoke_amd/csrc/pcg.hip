@@ -1031,13 +1031,28 @@ void run_tolerance_pcg(Context& c) {
     // PCG, convergence checked after every iteration (an application of the
     // preconditioner costs more than the host round trip).  Returns false when the residual fell
     // by less than 10 % over 5 iterations or turned non-finite.
+    // Two iterations stay in flight: iteration k's scalars are copied to a
+    // pinned slot and signalled by an event, and the host judges iteration k
+    // while iteration k + 1 runs.  An iteration enqueued after the solve
+    // finished is harmless: its update kernels skip on the done flag and its
+    // preconditioner application only rewrites z.
     auto learned_inner = [&]() -> bool {
         learned_apply(c);
         learned_rz(c, true);
         std::vector<double> rr{host.rr};
-        for (;;) {
+        auto enqueue = [&](int s) {
             learned_iteration(c);
-            read_scalars();
+            hip_check(hipMemcpyAsync(&c.host_scalars[s], c.scalars, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream), "scalars copy");
+            hip_check(hipEventRecord(c.status_event[s], c.stream), "status event");
+            learned_apply(c);
+            learned_rz(c, false);
+        };
+        enqueue(0);
+        enqueue(1);
+        int slot = 0;
+        for (;;) {
+            hip_check(hipEventSynchronize(c.status_event[slot]), "status wait");
+            host = c.host_scalars[slot];
             ++learned_iterations;
             if (!std::isfinite(host.rr) || !std::isfinite(host.rz)) return false;
             rr.push_back(host.rr);
@@ -1056,8 +1071,8 @@ void run_tolerance_pcg(Context& c) {
                 const double before = *std::min_element(rr.begin(), rr.end() - 10);
                 if (recent > 0.81 * before) return false;
             }
-            learned_apply(c);
-            learned_rz(c, false);
+            enqueue(slot);
+            slot ^= 1;
         }
     };
     while (!host.done) {

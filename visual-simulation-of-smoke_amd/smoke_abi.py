@@ -109,7 +109,7 @@ class KernelTiming(C.Structure):
     _fields_ = [("pcg_iterations", C.c_uint64), ("pcg_timed_iterations", C.c_uint64),
                 ("pcg_direction_ms", C.c_double), ("pcg_curvature_ms", C.c_double), ("pcg_update_ms", C.c_double),
                 ("pcg_reduction_ms", C.c_double),
-                ("step_ms", C.c_double), ("steps", C.c_uint64)]
+                ("step_ms", C.c_double), ("steps", C.c_uint64), ("mg_sweep_ms", C.c_double), ("mg_sweeps", C.c_uint64)]
 
 
 class LaunchShape(C.Structure):
@@ -136,6 +136,11 @@ class MultilevelOptions(C.Structure):
 
 class MultilevelInfo(C.Structure):
     _fields_ = [("levels", C.c_int32), ("dims", (C.c_int32 * 3) * 8), ("captured", C.c_uint32)]
+
+
+class MultilevelTiming(C.Structure):
+    _fields_ = [("sweep_us", C.c_float), ("restrict_us", C.c_float), ("scale_us", C.c_float), ("prolong_us", C.c_float),
+                ("vcycle_us", C.c_float), ("vector_rows", C.c_int32), ("repetitions", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PlumeSource(C.Structure):
@@ -297,6 +302,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_apply_multilevel": [C.c_void_p, C.c_void_p, C.c_void_p],
         "smoke_simulation_ext_export_level_flags": [C.c_void_p, C.c_int32, C.c_void_p],
         "smoke_simulation_ext_set_multilevel_capture": [C.c_void_p, C.c_void_p, C.c_uint32],
+        "smoke_simulation_ext_time_multilevel": [C.c_void_p, C.c_int32, C.POINTER(MultilevelTiming)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -514,6 +520,11 @@ class SmokeContext:
             return self.hip.download(d, (nz, ny, nx), np.uint8)
         finally:
             self.hip.free(d)
+
+    def time_multilevel(self, repetitions: int = 20) -> MultilevelTiming:
+        t = MultilevelTiming()
+        check(self.lib.smoke_simulation_ext_time_multilevel(self.handle, repetitions, C.byref(t)), "time_multilevel")
+        return t
 
     def set_multilevel_capture(self, d_buffer: int | None, capacity: int):
         check(self.lib.smoke_simulation_ext_set_multilevel_capture(self.handle, d_buffer, capacity), "set_multilevel_capture")
