@@ -10,13 +10,15 @@ step).  Every pressure solve reaches a true relative residual
 ||b - Ax|| / ||b|| <= 1e-6 (FP64 reductions and FP64 iterative refinement).
 
 Solvers, each timed on its own context over the same steps:
-  learned     flexible PCG preconditioned by a 4-level V-cycle whose coarsest
-              (32^3) level is solved by the trained network
-              (weights/preconditioner_coarse32.bin) inside an exact deflation
+  learned     flexible PCG preconditioned by a 5-level V-cycle whose coarsest
+              (16^3) level is solved by the trained network
+              (weights/preconditioner_coarse16.bin) inside an exact deflation
               -- the headline `value`;
   jacobi      plain Jacobi PCG (the hand-written baseline solver);
   multilevel  the same flexible PCG with a 7-level V-cycle and Jacobi sweeps
-              on its 4^3 coarsest level (no network), reported beside them.
+              on its 4^3 coarsest level (no network), reported beside them;
+  smoothed    (on request) the learned solver's 5-level cycle with the
+              network replaced by 16 Jacobi sweeps: what the network gains.
 
 One "step" is one full smoke step (forces, advection, pressure solve to
 tolerance, projection, scalar advection) with all inputs resident in HBM.
@@ -42,7 +44,7 @@ import smoke_abi as abi  # noqa: E402
 
 METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
-WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_coarse32.bin"
+WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_coarse16.bin"
 # algorithmic HBM bytes per cell
 SWEEP_BYTES_PER_CELL = 13       # damped-Jacobi sweep: reads r, x (4+4), flags (1); writes x' (4)
 RESTRICT_BYTES_PER_CELL = 9     # reads r, x, flags (+ 1/8 of a cell written)
@@ -90,9 +92,14 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
     if solver == "jacobi":
         solver_options(ctx, check_interval, warm_start)
     elif solver == "learned":
-        ctx.set_multilevel_options(levels=4, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
+        ctx.set_multilevel_options(levels=5, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
                                    coarse_sweeps=0)
         ctx.set_learned_preconditioner(net)
+        solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
+    elif solver == "smoothed":
+        # the learned solver's cycle with its network replaced by Jacobi sweeps (the ablation)
+        ctx.set_multilevel_options(levels=5, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+                                   coarse_sweeps=16)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "multilevel":
         ctx.set_multilevel_options(levels=7, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
@@ -197,7 +204,7 @@ def main():
     ap.add_argument("--scene", choices=("plume", "spheres"), default="plume",
                     help="plume: the closed box; spheres: the same with 12 seeded solid spheres and confinement 2.5")
     ap.add_argument("--solvers", default="learned,jacobi,multilevel",
-                    help="comma-separated, from learned, jacobi, multilevel; the first one is the reported value")
+                    help="comma-separated, from learned, jacobi, multilevel, smoothed; the first one is the reported value")
     ap.add_argument("--weights", default=str(WEIGHTS))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -276,8 +283,9 @@ def main():
                                 f"adaptive dt; value = {head}"),
                    "grid": [n, n, n], "scene": args.scene, "relative_tolerance": 1e-6,
                    "warm_start": ["zero", "previous", "extrapolated"][args.warm_start],
-                   "solvers": {"learned": "FPCG + 4-level V-cycle, network coarse solver at 32^3 (" + Path(args.weights).name + ")",
-                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V-cycle, Jacobi coarse solver (no network)"}},
+                   "solvers": {"learned": "FPCG + 5-level V-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + ")",
+                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V-cycle, Jacobi coarse solver (no network)",
+                               "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,
         "roofline": roofline, "cpu_baseline": None,
     }

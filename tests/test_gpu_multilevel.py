@@ -23,7 +23,7 @@ import smoke_abi as abi
 pytestmark = pytest.mark.gpu
 
 WEIGHTS_DIR = Path(__file__).resolve().parents[1] / "visual-simulation-of-smoke_amd" / "weights"
-COARSE_WEIGHTS = WEIGHTS_DIR / "preconditioner_coarse32.bin"
+COARSE_WEIGHTS = WEIGHTS_DIR / "preconditioner_coarse16.bin"
 W = (abi.NO_SLIP_WALL, 0.0, 0.0, 0.0, 0.0)
 P = (abi.PERIODIC, 0.0, 0.0, 0.0, 0.0)
 
@@ -136,7 +136,7 @@ def test_multilevel_solve_reaches_tolerance_in_few_iterations():
 def test_coarse_network_gains_over_the_cycle_without_it():
     if not COARSE_WEIGHTS.exists():
         pytest.skip("coarse-grid weights not trained yet")
-    base = dict(levels=2, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0)
+    base = dict(levels=3, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0)   # 16^3 coarsest level, as trained
     with abi.LearnedPreconditioner(seed=0, path=COARSE_WEIGHTS) as net:
         net_its, net_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, net=net, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=0, **base)
     smooth_its, smooth_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=4, **base)
@@ -156,3 +156,27 @@ def test_network_coarse_solver_requires_a_network():
         ctx.plume_sources(src, 1)
         with pytest.raises(abi.SmokeError):
             ctx.step()
+
+
+def test_multilevel_solve_on_vector_row_grid():
+    """A pressure grid over 2^20 cells with nx % 8 == 0 runs the float4-row
+    kernels, and its last sweep also produces the r.z partial sums of flexible
+    PCG: every solve must still reach the true FP64 residual 1e-6 in few
+    iterations without falling back."""
+    cfg = abi.make_config(128, 96, 96, cell_size=1.0 / 128, dt=0.01, pressure_iterations=64, ambient=0.0, buoyancy_density=0.05,
+                          buoyancy_temperature=0.25, vorticity=2.0, cubic=True)
+    src = abi.PlumeSource(64.0, 12.0, 48.0, 10.0, 1.0, 1.0, 0.05, 2)
+    occ = _spheres((96, 96, 128), 4, 11)
+    occ[:20] = 0
+    print("vector-row multilevel solve: 128 x 96 x 96", flush=True)
+    with abi.SmokeContext(cfg) as ctx:
+        ctx.set_occupancy(occ)
+        ctx.set_multilevel_options(levels=5, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=8)
+        _tolerance(ctx, abi.PRECONDITIONER_MULTILEVEL)
+        for k in range(5):
+            ctx.plume_sources(src, k)
+            ctx.step()
+            st = ctx.stats()
+            assert st.learned_fallback == 0, k
+            assert st.relative_residual <= 1e-6, (k, st.relative_residual)
+            assert st.iterations < 40, (k, st.iterations)

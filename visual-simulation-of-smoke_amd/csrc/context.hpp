@@ -75,6 +75,9 @@ struct Multilevel {
     int allocated = 0;           // levels whose buffers exist
     int built = 0;               // levels whose occupancy / flags match the pressure grid's
     MgLevel level[kMaxLevels];
+    double* dot_partials = nullptr;   // current application: per-block r.z of the last sweep (or nullptr)
+    int64_t dot_capacity = 0;
+    int dot_blocks = 0;
     float* capture = nullptr;    // coarsest-level right-hand sides (training data), not owned
     uint32_t capacity = 0;
     uint32_t captured = 0;
@@ -139,6 +142,7 @@ struct Context {
     double* partials = nullptr;
     size_t partial_capacity = 0;
     int pcg_stencil_blocks = 0;          // blocks of the last stencil-pass launch (partials to sum)
+    int rz_fused_blocks = 0;             // flexible PCG: r.z partials written by the preconditioner (0: none)
     PcgStatus* host_status = nullptr;   // pinned, two slots
     PcgScalars* host_scalars = nullptr; // pinned, two slots (flexible PCG: whole scalar block per iteration)
     hipEvent_t status_event[2] = {};
@@ -203,7 +207,10 @@ void table_advect_blocks(Context& c);
 SmokeSimulationMultilevelOptions default_multilevel_options();
 bool multilevel_options_valid(const Context& c, const SmokeSimulationMultilevelOptions& o);
 void multilevel_build(Context& c);                          // hierarchy of the current occupancy (if stale)
-void multilevel_apply(Context& c, const float* r, float* z);   // z = M(r), one V-cycle
+// z = M(r), one V-cycle; with dot_partials, the pressure grid's last sweep
+// also writes per-block partial sums of r.z there when it can (returns their
+// count, 0 when not fused)
+int multilevel_apply(Context& c, const float* r, float* z, double* dot_partials = nullptr, size_t dot_capacity = 0);
 void multilevel_release(Context& c);
 SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps);
 void multilevel_drain_timing(Context& c);                  // fold pending sweep events into timing_totals

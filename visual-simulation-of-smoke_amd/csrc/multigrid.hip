@@ -245,19 +245,36 @@ __device__ __forceinline__ void row4_residual(const LevelArgs& a, const float* _
     }
 }
 
+// DOT: also the block's partial sum of r . out (FP64), written to
+// partials[block] -- the r.z of flexible PCG when `out` is its z
+template <bool DOT>
 __global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb,
-                                                        float* __restrict__ out) {
+                                                        float* __restrict__ out, double* __restrict__ partials) {
     const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
-    if (x0 >= a.n[0] || y >= a.n[1]) return;
-    const Row4 q = row4_at(a, x0, y, z);
-    float res[4], xi[4], inv[4];
-    row4_residual(a, r, xb, q, res, xi, inv);
-    float o[4];
+    const bool inside = x0 < a.n[0] && y < a.n[1];
+    double dot = 0.0;
+    if (inside) {
+        const Row4 q = row4_at(a, x0, y, z);
+        float res[4], xi[4], inv[4];
+        row4_residual(a, r, xb, q, res, xi, inv);
+        float o[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = (q.f[k] & kIdentity) ? 0.0f : xi[k] + a.omega * res[k] * inv[k];
-    *reinterpret_cast<float4*>(out + q.i0) = make_float4(o[0], o[1], o[2], o[3]);
+        for (int k = 0; k < 4; ++k) o[k] = (q.f[k] & kIdentity) ? 0.0f : xi[k] + a.omega * res[k] * inv[k];
+        *reinterpret_cast<float4*>(out + q.i0) = make_float4(o[0], o[1], o[2], o[3]);
+        if constexpr (DOT) {
+            const float4 rv = ld4(r, q.i0);
+            dot = (double(rv.x) * double(o[0]) + double(rv.y) * double(o[1])) + (double(rv.z) * double(o[2]) + double(rv.w) * double(o[3]));
+        }
+    }
+    if constexpr (DOT) {
+        __shared__ double red[4];
+        for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+        __syncthreads();
+        if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
 }
 
 __global__ __launch_bounds__(256) void mg_scale4_kernel(LevelArgs a, const float* __restrict__ r, float* __restrict__ out) {
@@ -688,7 +705,7 @@ void vcycle(Context& c, int l) {
         xb = L.ta;
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = xb == L.ta ? L.tb : L.ta;
-            if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, xb, dst);
+            if (vec) mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, L.r, xb, dst, nullptr);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
             xb = dst;
         }
@@ -726,10 +743,18 @@ void vcycle(Context& c, int l) {
     }
     for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
-        hipEvent_t* ev = (c.timing && l == 0 && k == o.post_sweeps) ? next_timing_pair(c) : nullptr;
+        const bool last0 = l == 0 && k == o.post_sweeps;
+        hipEvent_t* ev = (c.timing && last0) ? next_timing_pair(c) : nullptr;
         if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
-        if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst);
-        else sweep(c, L, kBuffer, L.r, cur, none, dst);
+        const int64_t blocks4 = int64_t(g4.x) * g4.y * g4.z;
+        if (vec && last0 && m.dot_partials && blocks4 <= m.dot_capacity) {
+            mg_sweep4_kernel<true><<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst, m.dot_partials);
+            m.dot_blocks = int(blocks4);
+        } else if (vec) {
+            mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst, nullptr);
+        } else {
+            sweep(c, L, kBuffer, L.r, cur, none, dst);
+        }
         hip_check(hipGetLastError(), "mg_sweep_kernel");
         if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
         cur = dst;
@@ -847,14 +872,20 @@ void multilevel_build(Context& c) {
     m.built = levels;
 }
 
-void multilevel_apply(Context& c, const float* r, float* z) {
+int multilevel_apply(Context& c, const float* r, float* z, double* dot_partials, size_t dot_capacity) {
     multilevel_build(c);
-    MgLevel& L0 = c.mg.level[0];
+    Multilevel& m = c.mg;
+    MgLevel& L0 = m.level[0];
     L0.r = const_cast<float*>(r);
     L0.z = z;
+    m.dot_partials = dot_partials;
+    m.dot_capacity = int64_t(dot_capacity);
+    m.dot_blocks = 0;
     vcycle(c, 0);
     L0.r = nullptr;
     L0.z = nullptr;
+    m.dot_partials = nullptr;
+    return m.dot_blocks;
 }
 
 }  // namespace smoke
@@ -899,7 +930,7 @@ SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
         else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, r, L.ta);
     });
     out.sweep_us = time([&] {
-        if (vec) mg_sweep4_kernel<<<g4, 256, 0, c.stream>>>(a, r, L.ta, L.tb);
+        if (vec) mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, r, L.ta, L.tb, nullptr);
         else mg_sweep_kernel<kBuffer><<<g, 256, 0, c.stream>>>(a, r, L.ta, none, L.tb);
     });
     out.restrict_us = time([&] {

@@ -828,8 +828,9 @@ void pass_finalize(Context& c, int mode) {
 // ---- flexible PCG with the learned preconditioner
 // z = M(r)
 void learned_apply(Context& c) {
+    c.rz_fused_blocks = 0;
     if (c.options.preconditioner == SMOKE_SIMULATION_PRECONDITIONER_MULTILEVEL) {
-        multilevel_apply(c, c.pcg_r, c.pcg_z);
+        c.rz_fused_blocks = multilevel_apply(c, c.pcg_r, c.pcg_z, c.partials, c.partial_capacity);
         return;
     }
     const int dims[3] = {c.grid.n[0], c.grid.n[1], c.grid.n[2]};
@@ -839,6 +840,11 @@ void learned_apply(Context& c) {
 // r.z, then beta (or a fresh Krylov start)
 void learned_rz(Context& c, bool start) {
     const PcgArgs a = make_args(c);
+    if (c.rz_fused_blocks > 0) {   // the preconditioner's last pass wrote the partials
+        fpcg_beta_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, c.rz_fused_blocks, start ? 1 : 0);
+        hip_check(hipGetLastError(), "flexible pcg beta");
+        return;
+    }
     const unsigned blocks = unsigned(std::min<size_t>(1024, c.partial_capacity));
     fpcg_dot_kernel<<<blocks, 256, 0, c.stream>>>(a, c.pcg_r, c.pcg_z);
     fpcg_beta_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, int(blocks), start ? 1 : 0);

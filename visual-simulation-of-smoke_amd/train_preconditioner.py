@@ -21,7 +21,7 @@ network learns z = A_c^-1 r_c on those whole coarse volumes with the energy
 loss (z^T A z / 2 - z^T r) / ||r||^2.
 
 Usage: python train_preconditioner.py --out weights.bin [--grid 256 --steps 40 ...]
-       python train_preconditioner.py --coarse --out coarse.bin [--grid 256 --levels 4 --steps 30 --spheres]
+       python train_preconditioner.py --coarse --out coarse.bin [--grid 256 --levels 5 --steps 30 --spheres]
 """
 from __future__ import annotations
 
@@ -179,7 +179,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--reuse-data", action="store_true")
     ap.add_argument("--coarse", action="store_true", help="train the multilevel preconditioner's coarse-grid solver")
-    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--capacity", type=int, default=1500, help="coarse volumes captured per scene")
     ap.add_argument("--spheres", action="store_true", help="also capture the obstacle scene's solves")
     args = ap.parse_args()
