@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(ConvArgs a, const uint16_t* 
         const int oz = int((r / a.out[1]) % a.out[2]);
         const int b = int(r / (int64_t(a.out[1]) * a.out[2]));
         int iy, iz;
-        const bool row_ok = tap_source(a.mode, oy, ty, a.in[1], iy) & tap_source(a.mode, oz, tz, a.in[2], iz);
+        const bool row_ok = bool(int(tap_source(a.mode, oy, ty, a.in[1], iy)) & int(tap_source(a.mode, oz, tz, a.in[2], iz)));   // both taps resolved
         const int64_t in_row = (int64_t(b) * a.in[2] + iz) * a.in[1] + iy;
         const int64_t out_row = r;   // (b * D + z) * H + y
         for (int x0 = 0; x0 < a.out[0]; x0 += kStep) {

@@ -70,7 +70,6 @@ constexpr LayerSpec kLayers[] = {
     {SMOKE_NN_CONV_SAME, 16, 16, 0, 0, 1, 16, 16},  // c0u
     {SMOKE_NN_CONV_SAME, 16, 16, 0, 0, 0, 16, 1},   // out
 };
-constexpr int kLayerCount = int(sizeof(kLayers) / sizeof(kLayers[0]));
 constexpr uint8_t kIdentity = 0x40;
 
 struct Dims {
