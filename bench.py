@@ -10,12 +10,12 @@ step).  Every pressure solve reaches a true relative residual
 ||b - Ax|| / ||b|| <= 1e-6 (FP64 reductions and FP64 iterative refinement).
 
 Solvers, each timed on its own context over the same steps:
-  learned     flexible PCG preconditioned by a 5-level V-cycle whose coarsest
+  learned     flexible PCG preconditioned by a 5-level V(4,4)-cycle whose coarsest
               (16^3) level is solved by the trained network
               (weights/preconditioner_coarse16.bin) inside an exact deflation
               -- the headline `value`;
   jacobi      plain Jacobi PCG (the hand-written baseline solver);
-  multilevel  the same flexible PCG with a 7-level V-cycle and Jacobi sweeps
+  multilevel  the same flexible PCG with a 7-level V(4,4)-cycle and Jacobi sweeps
               on its 4^3 coarsest level (no network), reported beside them;
   smoothed    (on request) the learned solver's 5-level cycle with the
               network replaced by 16 Jacobi sweeps: what the network gains.
@@ -92,17 +92,17 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
     if solver == "jacobi":
         solver_options(ctx, check_interval, warm_start)
     elif solver == "learned":
-        ctx.set_multilevel_options(levels=5, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
+        ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
                                    coarse_sweeps=0)
         ctx.set_learned_preconditioner(net)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "smoothed":
         # the learned solver's cycle with its network replaced by Jacobi sweeps (the ablation)
-        ctx.set_multilevel_options(levels=5, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+        ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
                                    coarse_sweeps=16)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "multilevel":
-        ctx.set_multilevel_options(levels=7, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+        ctx.set_multilevel_options(levels=7, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
                                    coarse_sweeps=8)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     else:
@@ -283,8 +283,8 @@ def main():
                                 f"adaptive dt; value = {head}"),
                    "grid": [n, n, n], "scene": args.scene, "relative_tolerance": 1e-6,
                    "warm_start": ["zero", "previous", "extrapolated"][args.warm_start],
-                   "solvers": {"learned": "FPCG + 5-level V-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + ")",
-                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V-cycle, Jacobi coarse solver (no network)",
+                   "solvers": {"learned": "FPCG + 5-level V(4,4)-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + ")",
+                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(4,4)-cycle, Jacobi coarse solver (no network)",
                                "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,
         "roofline": roofline, "cpu_baseline": None,

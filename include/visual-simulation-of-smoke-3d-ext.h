@@ -285,8 +285,8 @@ typedef struct SmokeSimulationMultilevelOptions {
     int32_t reserved[2];
 } SmokeSimulationMultilevelOptions;
 
-/* Defaults: 4 levels, 2 pre- and 2 post-sweeps, omega 2/3, network coarse
-   solver with no extra sweeps. */
+/* Defaults: 5 levels (a 16^3 coarsest level for a 256^3 grid), 4 pre- and 4
+   post-sweeps, omega 2/3, network coarse solver with no extra sweeps. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_default_multilevel_options(SmokeSimulationMultilevelOptions* out_options);
 /* Validates and stores the options; the hierarchy is (re)built on the next
    multilevel solve.  Fails when a periodic axis does not halve evenly down to

@@ -765,9 +765,9 @@ void vcycle(Context& c, int l) {
 
 SmokeSimulationMultilevelOptions default_multilevel_options() {
     SmokeSimulationMultilevelOptions o{};
-    o.levels = 4;
-    o.pre_sweeps = 2;
-    o.post_sweeps = 2;
+    o.levels = 5;
+    o.pre_sweeps = 4;
+    o.post_sweeps = 4;
     o.omega = 2.0f / 3.0f;
     o.coarse_solver = SMOKE_SIMULATION_COARSE_NETWORK;
     o.coarse_sweeps = 0;

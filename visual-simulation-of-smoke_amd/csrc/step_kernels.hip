@@ -281,9 +281,10 @@ __global__ void advect_scalars_kernel(Grid g, const float* __restrict__ u, const
     }
     const float h = g.h;
     const float3 p = trace_rk2(g, u, v, w, occ, *dt_ptr, (float(x) + 0.5f) * h, (float(y) + 0.5f) * h, (float(z) + 0.5f) * h);
-    const bool cubic = g.cubic != 0;
-    T[i] = sample_scalar(g, g.temperature, Ttmp, p.x, p.y, p.z, cubic);
-    rho[i] = sample_scalar(g, g.density, rtmp, p.x, p.y, p.z, cubic);
+    float tv, rv;
+    sample_scalar2(g, g.temperature, Ttmp, g.density, rtmp, p.x, p.y, p.z, g.cubic != 0, tv, rv);
+    T[i] = tv;
+    rho[i] = rv;
 }
 
 // B:1243-1290 as a gather through the nearest-fluid map built at occupancy update.

@@ -86,7 +86,7 @@ def spheres_occupancy(n: int, count: int = 12, seed: int = 5) -> np.ndarray:
     return occ
 
 
-def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plume",), coarse_sweeps: int = 16, log=print):
+def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plume",), coarse_sweeps: int = 16, sweeps: int = 2, log=print):
     """Coarsest-level right-hand sides of the V-cycles of `steps` multilevel
     solves per scene; returns {"r": (count, D, H, W) float32, "flags": uint8,
     "crop": (W, H, D)}."""
@@ -99,7 +99,7 @@ def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plu
         with abi.SmokeContext(cfg, hip=hip) as ctx:
             if scene == "spheres":
                 ctx.set_occupancy(spheres_occupancy(n))
-            ctx.set_multilevel_options(levels=levels, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+            ctx.set_multilevel_options(levels=levels, pre_sweeps=sweeps, post_sweeps=sweeps, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
                                        coarse_sweeps=coarse_sweeps)
             ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_MULTILEVEL,
                                    relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1,
@@ -182,10 +182,11 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--capacity", type=int, default=1500, help="coarse volumes captured per scene")
     ap.add_argument("--spheres", action="store_true", help="also capture the obstacle scene's solves")
+    ap.add_argument("--sweeps", type=int, default=3, help="pre- and post-sweeps of the V-cycle that produces the coarse data")
     args = ap.parse_args()
     if args.coarse:
         scenes = ("plume", "spheres") if args.spheres else ("plume",)
-        data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes)
+        data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes, sweeps=args.sweeps)
         print(f"data: {data['r'].shape[0]} coarse volumes of {data['crop']}", flush=True)
         with abi.LearnedPreconditioner(seed=args.seed) as net:
             net.set_loss(abi.NN_LOSS_ENERGY)
