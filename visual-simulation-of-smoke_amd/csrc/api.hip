@@ -93,6 +93,8 @@ void destroy_resources(Context& c) {
     release(c.stencil_flags);
     release(c.anchor);
     release(c.solid_pairs);
+    release(c.advect_fallback);
+    release(c.advect_fallback_count);
     release(c.solid_count);
     release(c.x64[0]);
     release(c.x64[1]);
@@ -358,6 +360,8 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         smoke::alloc(c.stencil_flags, cells);
         smoke::alloc(c.anchor, 1);
         smoke::alloc(c.solid_pairs, 2 * cells);
+        smoke::alloc(c.advect_fallback, cells);
+        smoke::alloc(c.advect_fallback_count, 1);
         smoke::alloc(c.solid_count, 1);
         smoke::alloc(c.x64[0], cells);
         smoke::alloc(c.x64[1], cells);

@@ -163,6 +163,10 @@ struct Context {
     // u, v, w and of the scalars); set from the fixed table at creation
     int pcg_shape[2] = {0, 0};
     int advect_block[4] = {0, 0, 0, 0};   // set from the table at creation
+    // LDS-staged scalar advection (same results as the gather form): the
+    // cells it leaves to the global path and their count
+    int* advect_fallback = nullptr;
+    unsigned* advect_fallback_count = nullptr;
 
     Multilevel mg;
 

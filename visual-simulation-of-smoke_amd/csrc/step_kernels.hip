@@ -476,9 +476,278 @@ void launch_advect(Context& c, int block_index) {
                                                         c.dt_device);
 }
 
+// ---- LDS-staged scalar advection (the same results as advect_scalars_kernel).
+// A block owns a 32 x 8 x 2 tile of cells.  It stages, with the ghost
+// rules applied once per staged index, the temperature and density sources
+// over the tile plus a halo of 2 (every cubic tap of a backtrace shorter than
+// one cell), the three face components over the boxes the two linear samples
+// of such a trace touch, and the occupancy of the tile plus a halo of 1.  A
+// cell whose every sample stencil and hit-test cell lies in its staged box is
+// advected from LDS -- the same values in the same arithmetic order as the
+// global samplers, which return f at in-domain indices and the ghost value
+// elsewhere.  Any other cell (a trace longer than a cell, a periodic wrap, a
+// hit cell outside the box) is appended to a list that a second kernel
+// advects with the global path, so the common path carries none of the
+// ghost-rule code and keeps its register budget.
+constexpr int kSaX = 32, kSaY = 8, kSaZ = 2;
+
+struct LdsBox {
+    int o[3];   // origin (global index of the first staged entry)
+    int d[3];   // extent
+};
+
+__device__ __forceinline__ int box_size(const LdsBox& b) { return b.d[0] * b.d[1] * b.d[2]; }
+
+template <int LO, int HI>
+__device__ __forceinline__ bool in_box(const LdsBox& bx, const Stencil& s) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ok = ok && s.b[a] + LO >= bx.o[a] && s.b[a] + HI < bx.o[a] + bx.d[a];
+    return ok;
+}
+
+__device__ __forceinline__ int box_base(const LdsBox& bx, int x, int y, int z) {
+    return ((z - bx.o[2]) * bx.d[1] + (y - bx.o[1])) * bx.d[0] + (x - bx.o[0]);
+}
+
+// sample_linear on staged values; the stencil must lie in the box
+__device__ __forceinline__ float staged_linear(const float* lds, const LdsBox& bx, const Stencil& s) {
+    const int base = box_base(bx, s.b[0], s.b[1], s.b[2]);
+    const int sy = bx.d[0], sz = bx.d[0] * bx.d[1];
+    float c[8];
+    c[0] = lds[base];
+    c[1] = lds[base + 1];
+    c[2] = lds[base + sy];
+    c[3] = lds[base + sy + 1];
+    c[4] = lds[base + sz];
+    c[5] = lds[base + sz + 1];
+    c[6] = lds[base + sz + sy];
+    c[7] = lds[base + sz + sy + 1];
+    const float c00 = c[0] + (c[1] - c[0]) * s.t[0];
+    const float c10 = c[2] + (c[3] - c[2]) * s.t[0];
+    const float c01 = c[4] + (c[5] - c[4]) * s.t[0];
+    const float c11 = c[6] + (c[7] - c[6]) * s.t[0];
+    const float c0 = c00 + (c10 - c00) * s.t[1];
+    const float c1 = c01 + (c11 - c01) * s.t[1];
+    return c0 + (c1 - c0) * s.t[2];
+}
+
+// sample_cubic of two staged fields at one stencil (which must lie in the box)
+__device__ __forceinline__ void staged_cubic2(const float* la, const float* lb, const LdsBox& bx, const Stencil& s, float& va, float& vb) {
+    const int sy = bx.d[0], sz = bx.d[0] * bx.d[1];
+    const int base = box_base(bx, s.b[0] - 1, s.b[1] - 1, s.b[2] - 1);
+    float za[4], zb[4];
+#pragma unroll
+    for (int dz = 0; dz < 4; ++dz) {
+        float ya[4], yb[4];
+#pragma unroll
+        for (int dy = 0; dy < 4; ++dy) {
+            const int o = base + dz * sz + dy * sy;
+            ya[dy] = monotonic_cubic(la[o], la[o + 1], la[o + 2], la[o + 3], s.t[0]);
+            yb[dy] = monotonic_cubic(lb[o], lb[o + 1], lb[o + 2], lb[o + 3], s.t[0]);
+        }
+        za[dz] = monotonic_cubic(ya[0], ya[1], ya[2], ya[3], s.t[1]);
+        zb[dz] = monotonic_cubic(yb[0], yb[1], yb[2], yb[3], s.t[1]);
+    }
+    va = monotonic_cubic(za[0], za[1], za[2], za[3], s.t[2]);
+    vb = monotonic_cubic(zb[0], zb[1], zb[2], zb[3], s.t[2]);
+}
+
+// hits_solid (B:785-794) against the staged occupancy: 1 hit, 0 free,
+// -1 when the hit cell lies outside the staged box (the caller falls back)
+__device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, const LdsBox& bx, float qx, float qy, float qz) {
+    float q[3] = {qx, qy, qz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]) * g.h);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (q[a] < 0.0f || q[a] > float(g.n[a]) * g.h) return 1;
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = int(floorf(q[a] / g.h));
+        if (c[a] == g.n[a]) c[a] = g.n[a] - 1;
+    }
+    if (c[0] < 0 || c[0] >= g.n[0] || c[1] < 0 || c[1] >= g.n[1] || c[2] < 0 || c[2] >= g.n[2]) return 1;
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) inside = inside && c[a] >= bx.o[a] && c[a] < bx.o[a] + bx.d[a];
+    if (!inside) return -1;
+    return locc[box_base(bx, c[0], c[1], c[2])] != 0 ? 1 : 0;
+}
+
+// the global path for one cell (the body of advect_scalars_kernel)
+__device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* __restrict__ u, const float* __restrict__ v,
+                                                   const float* __restrict__ w, const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp,
+                                                   const float* __restrict__ rtmp, float* __restrict__ T, float* __restrict__ rho,
+                                                   const float* __restrict__ solid_T, float dt, int x, int y, int z) {
+    const int64_t i = cidx(g, x, y, z);
+    if (occ[i] != 0) {
+        T[i] = solid_T[i];
+        rho[i] = 0.0f;
+        return;
+    }
+    const float h = g.h;
+    const float3 p = trace_rk2(g, u, v, w, occ, dt, (float(x) + 0.5f) * h, (float(y) + 0.5f) * h, (float(z) + 0.5f) * h);
+    float tv, rv;
+    sample_scalar2(g, g.temperature, Ttmp, g.density, rtmp, p.x, p.y, p.z, g.cubic != 0, tv, rv);
+    T[i] = tv;
+    rho[i] = rv;
+}
+
+__global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                           const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                                           const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                                                           float* __restrict__ T, float* __restrict__ rho,
+                                                                           const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
+                                                                           int* __restrict__ fallback, unsigned* __restrict__ fallback_count) {
+    constexpr int kS = (kSaX + 4) * (kSaY + 4) * (kSaZ + 4);
+    constexpr int kU = (kSaX + 3) * (kSaY + 2) * (kSaZ + 2);
+    constexpr int kV = (kSaX + 2) * (kSaY + 3) * (kSaZ + 2);
+    constexpr int kW = (kSaX + 2) * (kSaY + 2) * (kSaZ + 3);
+    constexpr int kO = (kSaX + 2) * (kSaY + 2) * (kSaZ + 2);
+    __shared__ float sT[kS], sR[kS], su[kU], sv[kV], sw[kW];
+    __shared__ uint8_t so[kO];
+    const int tid = threadIdx.x;
+    const int X0 = blockIdx.x * kSaX, Y0 = blockIdx.y * kSaY, Z0 = blockIdx.z * kSaZ;
+    const LdsBox bs{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
+    const LdsBox bu{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 2, kSaZ + 2}};
+    const LdsBox bv{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 3, kSaZ + 2}};
+    const LdsBox bw{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 3}};
+    const LdsBox bo{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 2}};
+    // staging: every staged entry is what the global samplers' taps return
+    for (int q = tid; q < kS; q += kSaX * kSaY) {
+        const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
+        sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
+        sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
+    }
+    auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
+        const int n = box_size(bx);
+        for (int q = tid; q < n; q += kSaX * kSaY) {
+            const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
+            lds[q] = load_face(g, f, comp, i, j, k);
+        }
+    };
+    stage_face(0, u, su, bu);
+    stage_face(1, v, sv, bv);
+    stage_face(2, w, sw, bw);
+    for (int q = tid; q < kO; q += kSaX * kSaY) {
+        const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+        // entries outside the domain are never read (the hit test returns first)
+        so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+    }
+    __syncthreads();
+    const int x = X0 + tid % kSaX, y = Y0 + tid / kSaX;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    const float h = g.h;
+    const float dt = *dt_ptr;
+    const bool cubic = g.cubic != 0;
+    const bool same_periodic = g.temperature.periodic[0] == g.density.periodic[0] && g.temperature.periodic[1] == g.density.periodic[1] &&
+                               g.temperature.periodic[2] == g.density.periodic[2];
+    for (int tz = 0; tz < kSaZ; ++tz) {
+        const int z = Z0 + tz;
+        if (z >= g.n[2]) break;
+        const int64_t i = cidx(g, x, y, z);
+        if (so[box_base(bo, x, y, z)] != 0) {
+            T[i] = solid_T[i];
+            rho[i] = 0.0f;
+            continue;
+        }
+        bool ok = same_periodic;
+        // trace_rk2 (B:756-838) with staged linear velocity samples
+        const float sx = (float(x) + 0.5f) * h, sy = (float(y) + 0.5f) * h, sz = (float(z) + 0.5f) * h;
+        const Stencil a0 = locate(g, g.flow_periodic, 0, sx, sy, sz);
+        const Stencil a1 = locate(g, g.flow_periodic, 1, sx, sy, sz);
+        const Stencil a2 = locate(g, g.flow_periodic, 2, sx, sy, sz);
+        ok = ok && in_box<0, 1>(bu, a0) && in_box<0, 1>(bv, a1) && in_box<0, 1>(bw, a2);
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        if (ok) {
+            const float v0x = staged_linear(su, bu, a0), v0y = staged_linear(sv, bv, a1), v0z = staged_linear(sw, bw, a2);
+            const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
+            const Stencil m0 = locate(g, g.flow_periodic, 0, mx, my, mz);
+            const Stencil m1 = locate(g, g.flow_periodic, 1, mx, my, mz);
+            const Stencil m2 = locate(g, g.flow_periodic, 2, mx, my, mz);
+            ok = in_box<0, 1>(bu, m0) && in_box<0, 1>(bv, m1) && in_box<0, 1>(bw, m2);
+            if (ok) {
+                const float v1x = staged_linear(su, bu, m0), v1y = staged_linear(sv, bv, m1), v1z = staged_linear(sw, bw, m2);
+                const float tx = sx - dt * v1x, ty = sy - dt * v1y, tzp = sz - dt * v1z;
+                px = tx;
+                py = ty;
+                pz = tzp;
+                const int hit = hit_staged(g, so, bo, tx, ty, tzp);
+                if (hit < 0) {
+                    ok = false;
+                } else if (hit > 0) {
+                    float lo = 0.0f, hi = 1.0f;
+                    for (int it = 0; it < 10 && ok; ++it) {
+                        const float mt = 0.5f * (lo + hi);
+                        const int hm = hit_staged(g, so, bo, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tzp - sz) * mt);
+                        if (hm < 0) ok = false;
+                        else if (hm > 0) hi = mt;
+                        else lo = mt;
+                    }
+                    px = sx + (tx - sx) * lo;
+                    py = sy + (ty - sy) * lo;
+                    pz = sz + (tzp - sz) * lo;
+                }
+            }
+        }
+        Stencil st{};
+        if (ok) {
+            st = locate(g, g.temperature.periodic, -1, px, py, pz);
+            ok = cubic ? in_box<-1, 2>(bs, st) : in_box<0, 1>(bs, st);
+        }
+        if (!ok) {
+            fallback[atomicAdd(fallback_count, 1u)] = int(i);
+            continue;
+        }
+        float tv, rv;
+        if (cubic) {
+            staged_cubic2(sT, sR, bs, st, tv, rv);
+        } else {
+            tv = staged_linear(sT, bs, st);
+            rv = staged_linear(sR, bs, st);
+        }
+        T[i] = tv;
+        rho[i] = rv;
+    }
+}
+
+// the cells the staged kernel could not advect from LDS, by the global path
+__global__ __launch_bounds__(256) void advect_scalars_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                      const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                                      const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                                                      float* __restrict__ T, float* __restrict__ rho,
+                                                                      const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
+                                                                      const int* __restrict__ fallback, const unsigned* __restrict__ fallback_count) {
+    const unsigned n = *fallback_count;
+    const float dt = *dt_ptr;
+    for (unsigned q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+        const int i = fallback[q];
+        const int x = i % g.n[0], y = (i / g.n[0]) % g.n[1], z = i / (g.n[0] * g.n[1]);
+        advect_scalar_cell(g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt, x, y, z);
+    }
+}
+
+__global__ void zero_count_kernel(unsigned* __restrict__ count) { *count = 0u; }
+
 // T and rho advection from (Ttmp, rtmp) into (T, rho)
 void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
     const Grid& g = c.grid;
+    if (c.advect_fallback) {
+        const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
+        zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
+        advect_scalars_staged_kernel<<<tiles, kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
+                                                                         c.density_tmp, T, rho, c.solid_temperature, c.dt_device,
+                                                                         c.advect_fallback, c.advect_fallback_count);
+        advect_scalars_fallback_kernel<<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
+                                                                 c.density_tmp, T, rho, c.solid_temperature, c.dt_device, c.advect_fallback,
+                                                                 c.advect_fallback_count);
+        hip_check(hipGetLastError(), "staged scalar advection");
+        return;
+    }
     const dim3 block = kAdvectBlocks[block_index];
     advect_scalars_kernel<<<grid3(g.n[0], g.n[1], g.n[2], block), block, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
                                                                                       c.temperature_tmp, c.density_tmp, T, rho,
