@@ -181,20 +181,33 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_measure_copy_ban
    fixed table chosen from measurements on MI355X, which keeps results
    bitwise reproducible across contexts.  autotune_launches times candidate
    shapes on this context's own grid and state -- the tile of each of the two
-   PCG stencil passes and the block of each advection kernel -- applies the
-   fastest and caches the choice for the grid class (dimensions and boundary
-   kinds) for the rest of the process; later calls for the same class reuse
-   it without timing.  It is synchronous, leaves the simulation state
-   unchanged (it writes only the intermediate advected velocity and solver
-   scratch, which the next step recomputes) and rebuilds the step graph.  A
-   PCG tile other than the table's changes the grouping of the FP64 partial
-   sums, so results may differ from the table's in the last bits.
-   reset_launches returns to the table. */
+   PCG stencil passes and the form and block of each advection kernel --
+   applies the fastest and caches the choice for the grid class (dimensions
+   and boundary kinds) for the rest of the process; later calls for the same
+   class reuse it without timing.  It is synchronous, leaves the simulation
+   state unchanged (it writes only the intermediate advected velocity and
+   solver scratch, which the next step recomputes) and rebuilds the step
+   graph.  A PCG tile other than the table's changes the grouping of the FP64
+   partial sums, so results may differ from the table's in the last bits.
+   reset_launches returns to the table; set_launches applies the shapes of a
+   report (from get_launches or autotune_launches, possibly of another
+   process) and fails on a shape that is not a candidate for this grid.
+
+   Advection has two forms with bitwise-equal results.  The gather form runs
+   one thread per output and reads every tap from global memory (cells_per_thread
+   1, any candidate block; the table's choice for u).  The LDS-staged form
+   (32 x 8 x 1 threads, cells_per_thread 2: tiles of 32 x 8 x 2 outputs, the
+   table's choice for v, w and the scalars) stages
+   each tile's sources, velocities and occupancy with their ghost values once,
+   advects every output whose backtrace (shorter than a cell, no periodic
+   wrap) stays inside the staged boxes from LDS, and advects the rest in a
+   second pass by the gather form's code. */
 typedef struct SmokeSimulationLaunchShape {
-    int32_t threads_x;  /* threads along x (each owns `cells_x` consecutive cells) */
+    int32_t threads_x;        /* threads along x */
     int32_t threads_y;
-    int32_t threads_z;  /* advection: threads along z; PCG: z planes per tile */
-    int32_t cells_x;    /* PCG: 4 (float4 rows) or 1; advection: 1 */
+    int32_t threads_z;        /* advection: threads along z; PCG: z planes per tile */
+    int32_t cells_per_thread; /* PCG: consecutive x cells per thread (4: float4 rows, or 1);
+                                 advection: outputs per thread along z (1: gather form, 2: staged form) */
 } SmokeSimulationLaunchShape;
 
 typedef struct SmokeSimulationLaunchReport {
@@ -211,6 +224,7 @@ typedef struct SmokeSimulationLaunchReport {
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_autotune_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_reset_launches(SmokeSimulationContext context);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_launches(SmokeSimulationContext context, const SmokeSimulationLaunchReport* shapes);
 
 /* Training pairs of the pressure solve, for a learned preconditioner: crops
    of the last solve's right-hand side r = b, its solution z = A^-1 b (the

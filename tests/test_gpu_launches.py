@@ -2,12 +2,14 @@
 occupancy-derived block choice, backend-cuda.cu:1881-1911).
 
 The tuner times candidate tiles of the two PCG stencil passes and candidate
-blocks of the four advection kernels on the context's own grid.  Checked here:
+launches (gather-form blocks and the LDS-staged form) of the four advection
+kernels on the context's own grid.  Checked here:
 it leaves the simulation state untouched, the chosen shapes come from the
 grid class's candidates, steps with tuned shapes agree with steps on the table
 shapes (to the solve tolerance: a different PCG tile regroups the FP64
 partial sums), a second context of the same class reuses the cached choice,
-and reset returns to the table.
+reset returns to the table, and set_launches applies a report's shapes and
+rejects shapes that are not candidates.
 """
 import numpy as np
 import pytest
@@ -35,9 +37,10 @@ CLASSES = {
     "float4_narrow": ((36, 28, 20), {(16, 16, 2, 4), (16, 8, 2, 4)}, ((16, 16, 2, 4), (16, 16, 2, 4))),
     "scalar_rows": ((45, 30, 22), {(64, 4, 2, 1), (64, 8, 2, 1), (32, 8, 2, 1)}, ((64, 4, 2, 1), (64, 4, 2, 1))),
 }
+# gather-form blocks (one output per thread), then the LDS-staged form
 ADVECT_BLOCKS = {(64, 4, 1, 1), (32, 8, 1, 1), (16, 16, 1, 1), (32, 4, 2, 1), (64, 2, 2, 1), (128, 2, 1, 1), (8, 8, 4, 1), (8, 8, 8, 1),
-                 (16, 4, 4, 1), (8, 16, 2, 1)}
-ADVECT_TABLE = [(16, 16, 1, 1), (128, 2, 1, 1), (128, 2, 1, 1), (128, 2, 1, 1)]
+                 (16, 4, 4, 1), (8, 16, 2, 1), (32, 8, 1, 2)}
+ADVECT_TABLE = [(16, 16, 1, 1)] + [(32, 8, 1, 2)] * 3
 
 
 def _context(cfg, st, tolerance=True):
@@ -101,3 +104,35 @@ def test_autotune_cache_reset_and_fixed_mode_graph():
             assert (r.pcg_curvature.tuple(), r.pcg_update.tuple()) == ((32, 8, 2, 4), (32, 8, 2, 4))
             assert [r.advect[k].tuple() for k in range(4)] == ADVECT_TABLE
             b.step()
+
+
+def test_set_launches_applies_and_rejects_shapes():
+    dims = (128, 40, 24)
+    cfg = abi.make_config(*dims, cell_size=1.0 / 40, dt=0.01, vorticity=2.0, cubic=True)
+    st = make_state(cfg, seed=9, solids=2)
+    with _context(cfg, st, tolerance=False) as a, _context(cfg, st, tolerance=False) as plain:
+        r = a.launches()
+        chosen = [(64, 4, 1, 1), (16, 16, 1, 1), (8, 8, 4, 1), (32, 8, 1, 2)]
+        for k, shape in enumerate(chosen):
+            r.advect[k].threads_x, r.advect[k].threads_y, r.advect[k].threads_z, r.advect[k].cells_per_thread = shape
+        r.pcg_curvature.threads_x, r.pcg_curvature.threads_y, r.pcg_curvature.threads_z, r.pcg_curvature.cells_per_thread = (64, 4, 4, 4)
+        a.set_launches(r)
+        got = a.launches()
+        assert [got.advect[k].tuple() for k in range(4)] == chosen
+        assert got.pcg_curvature.tuple() == (64, 4, 4, 4) and got.pcg_update.tuple() == (32, 8, 2, 4)
+        # the fixed-iteration graph is rebuilt with the new shapes; every
+        # advection form gives the same bits
+        a.step(2)
+        plain.step(2)
+        x, y = a.export_state(), plain.export_state()
+        for f in FIELDS:
+            assert rel_l2(x[f], y[f]) < 1e-5, f
+        bad = a.launches()
+        bad.advect[2].threads_x = 48
+        with pytest.raises(abi.SmokeError):
+            a.set_launches(bad)
+        bad = a.launches()
+        bad.pcg_update.threads_x, bad.pcg_update.threads_y, bad.pcg_update.threads_z, bad.pcg_update.cells_per_thread = (64, 4, 2, 1)
+        with pytest.raises(abi.SmokeError):     # a scalar-row tile is not a candidate of a float4 grid
+            a.set_launches(bad)
+        assert [a.launches().advect[k].tuple() for k in range(4)] == chosen    # a rejected report changes nothing

@@ -785,6 +785,19 @@ SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext c
     });
 }
 
+SmokeSimulationResult smoke_simulation_ext_set_launches(SmokeSimulationContext context, const SmokeSimulationLaunchReport* shapes) {
+    if (!shapes) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) {
+        // every shape is resolved before any is applied
+        const int pcg[2] = {smoke::find_pcg_shape(c, shapes->pcg_curvature), smoke::find_pcg_shape(c, shapes->pcg_update)};
+        int advect[4];
+        for (int k = 0; k < 4; ++k) advect[k] = smoke::find_advect_block(shapes->advect[k]);
+        std::copy(std::begin(pcg), std::end(pcg), c.pcg_shape);
+        std::copy(std::begin(advect), std::end(advect), c.advect_block);
+        smoke::rebuild_graph(c);
+    });
+}
+
 SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops) {
     if (!crops) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) { smoke::write_training_crops(c, *crops); });

@@ -163,8 +163,8 @@ struct Context {
     // u, v, w and of the scalars); set from the fixed table at creation
     int pcg_shape[2] = {0, 0};
     int advect_block[4] = {0, 0, 0, 0};   // set from the table at creation
-    // LDS-staged scalar advection (same results as the gather form): the
-    // cells it leaves to the global path and their count
+    // LDS-staged advection: the outputs (at most one per cell: its tiles
+    // cover indices below n) that it leaves to its second pass, and their count
     int* advect_fallback = nullptr;
     unsigned* advect_fallback_count = nullptr;
 
@@ -204,6 +204,7 @@ double measure_copy(uint64_t bytes, int reps, double* ms_per_copy);
 // per launch of the table shape and of the chosen one.
 void table_pcg_shapes(Context& c);
 void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out);
+int find_pcg_shape(const Context& c, const SmokeSimulationLaunchShape& shape);   // a candidate of this grid; throws when none matches
 int tune_pcg_shapes(Context& c, SmokeSimulationLaunchReport& report);   // returns candidates timed
 void table_advect_blocks(Context& c);
 
@@ -222,6 +223,7 @@ void multilevel_drain_timing(Context& c);                  // fold pending sweep
 // Training-pair crops of the last pressure solve (training_data.hip).
 void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& crops);
 void describe_advect_block(int index, SmokeSimulationLaunchShape& out);
+int find_advect_block(const SmokeSimulationLaunchShape& shape);   // candidate index; throws when none matches
 int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report);
 
 }  // namespace smoke

@@ -900,6 +900,16 @@ void describe_pcg_shape(int index, SmokeSimulationLaunchShape& out) {
     out = {sh.txv, sh.ty, sh.zr, sh.vec};
 }
 
+int find_pcg_shape(const Context& c, const SmokeSimulationLaunchShape& shape) {
+    int first, last;
+    stencil_candidates(c, first, last);
+    for (int i = first; i < last; ++i) {
+        const StencilShape& sh = kStencilShapes[i];
+        if (sh.txv == shape.threads_x && sh.ty == shape.threads_y && sh.zr == shape.threads_z && sh.vec == shape.cells_per_thread) return i;
+    }
+    throw HipError("PCG launch shape is not a candidate for this grid");
+}
+
 // Times the candidate tiles of passes 2 and 3 on this grid and keeps the
 // fastest.  The scalars are set so that every launch does its full work and
 // leaves the vectors unchanged: not done, no stop test, alpha = 0 (e and r

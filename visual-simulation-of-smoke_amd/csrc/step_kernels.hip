@@ -17,16 +17,19 @@ inline dim3 grid3(int nx, int ny, int nz, dim3 b) { return dim3((nx + b.x - 1) /
 inline unsigned grid1(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
 const dim3 kCellBlock(64, 4, 1);
 
-// Candidate blocks of the advection kernels (threads along x, y, z).  Every
-// thread computes one output independently, so the block shape changes only
-// the speed.  The table default, measured on MI355X at 256^3 (plume state,
-// cubic advection): 16 x 16 x 1 for u (0.60 ms against 0.72 ms with 64 x 4 x 1),
-// 128 x 2 x 1 for v, w and the scalars (1.08, 1.13 and 2.34 ms against 1.52,
-// 1.56 and 3.11 ms).
+// Candidate launches of the advection kernels: the blocks of the gather form
+// (threads along x, y, z; every thread computes one output independently, so
+// the block shape changes only the speed), then the LDS-staged form
+// (kAdvectStaged).  Measured on MI355X at 256^3 (plume state, cubic
+// advection): the best gather blocks take 0.61 ms for u (16 x 16 x 1), 1.13,
+// 1.14 and 2.24 ms for v, w and the scalars (128 x 2 x 1); the staged form
+// 0.86, 0.85, 0.84 and 1.26 ms.  The table takes the faster of the two.
 const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
                               dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
-constexpr int kAdvectTable[4] = {2, 5, 5, 5};
+constexpr int kAdvectStaged = kAdvectBlockCount;
+constexpr int kAdvectCandidates = kAdvectBlockCount + 1;
+constexpr int kAdvectTable[4] = {2, kAdvectStaged, kAdvectStaged, kAdvectStaged};
 
 __global__ void fill_kernel(float* __restrict__ p, float value, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -199,6 +202,19 @@ __global__ void boundary_faces_kernel(Grid g, float* __restrict__ f) {
 // ------------------------------------------------------------ velocity advection (stage 1)
 // B:1191-1219 + the enforce that follows (B:2025-2030): every interior face is
 // traced and sampled unless a solid override replaces it.
+// face (i,j,k) of component C traced back and sampled by the global path
+template <int C>
+__device__ __forceinline__ float advect_face_value(const Grid& g, const float* __restrict__ u, const float* __restrict__ v,
+                                                   const float* __restrict__ w, const uint8_t* __restrict__ occ, float dt, int i, int j, int k) {
+    const float h = g.h;
+    const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
+    const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
+    const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
+    const float3 p = trace_rk2(g, u, v, w, occ, dt, sx, sy, sz);
+    const float* src = C == 0 ? u : (C == 1 ? v : w);
+    return sample_face<C>(g, src, p.x, p.y, p.z, g.cubic != 0);
+}
+
 template <int C>
 __global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                    float* __restrict__ dst, const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
@@ -215,13 +231,7 @@ __global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const fl
         dst[fi] = value;
         return;
     }
-    const float h = g.h;
-    const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
-    const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
-    const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
-    const float3 p = trace_rk2(g, u, v, w, occ, *dt_ptr, sx, sy, sz);
-    const float* src = C == 0 ? u : (C == 1 ? v : w);
-    dst[fi] = sample_face<C>(g, src, p.x, p.y, p.z, g.cubic != 0);
+    dst[fi] = advect_face_value<C>(g, u, v, w, occ, *dt_ptr, i, j, k);
 }
 
 // ------------------------------------------------------------ projection (stage 3)
@@ -467,14 +477,6 @@ void launch_boundaries(Context& c, float* const f[3]) {
     launch_boundary<2>(c, f[2]);
 }
 
-template <int C>
-void launch_advect(Context& c, int block_index) {
-    const Grid& g = c.grid;
-    const dim3 block = kAdvectBlocks[block_index];
-    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), block);
-    advect_face_kernel<C><<<grid, block, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
-                                                        c.dt_device);
-}
 
 // ---- LDS-staged scalar advection (the same results as advect_scalars_kernel).
 // A block owns a 32 x 8 x 2 tile of cells.  It stages, with the ghost
@@ -578,6 +580,45 @@ __device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, co
     return locc[box_base(bx, c[0], c[1], c[2])] != 0 ? 1 : 0;
 }
 
+// trace_rk2 (B:756-838) from (sx, sy, sz) with linear velocity samples of
+// the staged components and the staged hit test: false (nothing written) when
+// a sample stencil or a hit-test cell lies outside its staged box
+__device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, const LdsBox& bu, const float* lv, const LdsBox& bv, const float* lw,
+                                             const LdsBox& bw, const uint8_t* locc, const LdsBox& bo, float dt, float sx, float sy, float sz,
+                                             float& px, float& py, float& pz) {
+    const Stencil a0 = locate(g, g.flow_periodic, 0, sx, sy, sz);
+    const Stencil a1 = locate(g, g.flow_periodic, 1, sx, sy, sz);
+    const Stencil a2 = locate(g, g.flow_periodic, 2, sx, sy, sz);
+    if (!(in_box<0, 1>(bu, a0) && in_box<0, 1>(bv, a1) && in_box<0, 1>(bw, a2))) return false;
+    const float v0x = staged_linear(lu, bu, a0), v0y = staged_linear(lv, bv, a1), v0z = staged_linear(lw, bw, a2);
+    const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
+    const Stencil m0 = locate(g, g.flow_periodic, 0, mx, my, mz);
+    const Stencil m1 = locate(g, g.flow_periodic, 1, mx, my, mz);
+    const Stencil m2 = locate(g, g.flow_periodic, 2, mx, my, mz);
+    if (!(in_box<0, 1>(bu, m0) && in_box<0, 1>(bv, m1) && in_box<0, 1>(bw, m2))) return false;
+    const float v1x = staged_linear(lu, bu, m0), v1y = staged_linear(lv, bv, m1), v1z = staged_linear(lw, bw, m2);
+    const float tx = sx - dt * v1x, ty = sy - dt * v1y, tz = sz - dt * v1z;
+    px = tx;
+    py = ty;
+    pz = tz;
+    const int hit = hit_staged(g, locc, bo, tx, ty, tz);
+    if (hit < 0) return false;
+    if (hit > 0) {
+        float lo = 0.0f, hi = 1.0f;
+        for (int it = 0; it < 10; ++it) {
+            const float mt = 0.5f * (lo + hi);
+            const int hm = hit_staged(g, locc, bo, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tz - sz) * mt);
+            if (hm < 0) return false;
+            if (hm > 0) hi = mt;
+            else lo = mt;
+        }
+        px = sx + (tx - sx) * lo;
+        py = sy + (ty - sy) * lo;
+        pz = sz + (tz - sz) * lo;
+    }
+    return true;
+}
+
 // the global path for one cell (the body of advect_scalars_kernel)
 __device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* __restrict__ u, const float* __restrict__ v,
                                                    const float* __restrict__ w, const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp,
@@ -655,45 +696,9 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
             rho[i] = 0.0f;
             continue;
         }
-        bool ok = same_periodic;
-        // trace_rk2 (B:756-838) with staged linear velocity samples
         const float sx = (float(x) + 0.5f) * h, sy = (float(y) + 0.5f) * h, sz = (float(z) + 0.5f) * h;
-        const Stencil a0 = locate(g, g.flow_periodic, 0, sx, sy, sz);
-        const Stencil a1 = locate(g, g.flow_periodic, 1, sx, sy, sz);
-        const Stencil a2 = locate(g, g.flow_periodic, 2, sx, sy, sz);
-        ok = ok && in_box<0, 1>(bu, a0) && in_box<0, 1>(bv, a1) && in_box<0, 1>(bw, a2);
-        float px = 0.0f, py = 0.0f, pz = 0.0f;
-        if (ok) {
-            const float v0x = staged_linear(su, bu, a0), v0y = staged_linear(sv, bv, a1), v0z = staged_linear(sw, bw, a2);
-            const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
-            const Stencil m0 = locate(g, g.flow_periodic, 0, mx, my, mz);
-            const Stencil m1 = locate(g, g.flow_periodic, 1, mx, my, mz);
-            const Stencil m2 = locate(g, g.flow_periodic, 2, mx, my, mz);
-            ok = in_box<0, 1>(bu, m0) && in_box<0, 1>(bv, m1) && in_box<0, 1>(bw, m2);
-            if (ok) {
-                const float v1x = staged_linear(su, bu, m0), v1y = staged_linear(sv, bv, m1), v1z = staged_linear(sw, bw, m2);
-                const float tx = sx - dt * v1x, ty = sy - dt * v1y, tzp = sz - dt * v1z;
-                px = tx;
-                py = ty;
-                pz = tzp;
-                const int hit = hit_staged(g, so, bo, tx, ty, tzp);
-                if (hit < 0) {
-                    ok = false;
-                } else if (hit > 0) {
-                    float lo = 0.0f, hi = 1.0f;
-                    for (int it = 0; it < 10 && ok; ++it) {
-                        const float mt = 0.5f * (lo + hi);
-                        const int hm = hit_staged(g, so, bo, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tzp - sz) * mt);
-                        if (hm < 0) ok = false;
-                        else if (hm > 0) hi = mt;
-                        else lo = mt;
-                    }
-                    px = sx + (tx - sx) * lo;
-                    py = sy + (ty - sy) * lo;
-                    pz = sz + (tzp - sz) * lo;
-                }
-            }
-        }
+        float px, py, pz;
+        bool ok = same_periodic && staged_trace(g, su, bu, sv, bv, sw, bw, so, bo, dt, sx, sy, sz, px, py, pz);
         Stencil st{};
         if (ok) {
             st = locate(g, g.temperature.periodic, -1, px, py, pz);
@@ -733,10 +738,162 @@ __global__ __launch_bounds__(256) void advect_scalars_fallback_kernel(Grid g, co
 
 __global__ void zero_count_kernel(unsigned* __restrict__ count) { *count = 0u; }
 
+// sample_cubic of one staged field (the stencil must lie in the box)
+__device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx, const Stencil& s) {
+    const int sy = bx.d[0], sz = bx.d[0] * bx.d[1];
+    const int base = box_base(bx, s.b[0] - 1, s.b[1] - 1, s.b[2] - 1);
+    float zs[4];
+#pragma unroll
+    for (int dz = 0; dz < 4; ++dz) {
+        float ys[4];
+#pragma unroll
+        for (int dy = 0; dy < 4; ++dy) {
+            const int o = base + dz * sz + dy * sy;
+            ys[dy] = monotonic_cubic(lds[o], lds[o + 1], lds[o + 2], lds[o + 3], s.t[0]);
+        }
+        zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+    }
+    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], s.t[2]);
+}
+
+// ---- LDS-staged velocity advection (the same results as advect_face_kernel).
+// A block owns a 32 x 8 x 2 tile of the faces of component C (face indices
+// below n on every axis: face n is always a boundary face).  For a backtrace
+// shorter than one cell it stages component C over the tile plus a halo of 2
+// (the cubic taps of the foot, the linear taps of C's own velocity samples),
+// the other two components over the tile plus a halo of 1 (their linear taps
+// at the start and the midpoint) and the occupancy plus a halo of 1 (the hit
+// test and the solid-face test).  Faces whose trace or foot stencil leaves the
+// staged boxes go to the fallback list.
+template <int C>
+__global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                        const float* __restrict__ w, float* __restrict__ dst,
+                                                                        const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
+                                                                        const float* __restrict__ dt_ptr, int* __restrict__ fallback,
+                                                                        unsigned* __restrict__ fallback_count) {
+    constexpr int kF = (kSaX + 4) * (kSaY + 4) * (kSaZ + 4);
+    constexpr int kG = (kSaX + 3) * (kSaY + 3) * (kSaZ + 3);
+    constexpr int kO = (kSaX + 2) * (kSaY + 2) * (kSaZ + 2);
+    // the other two components, in increasing order
+    constexpr int D0 = C == 0 ? 1 : 0, D1 = C == 2 ? 1 : 2;
+    __shared__ float sf[kF], sg0[kG], sg1[kG];
+    __shared__ uint8_t so[kO];
+    const int tid = threadIdx.x;
+    const int X0 = blockIdx.x * kSaX, Y0 = blockIdx.y * kSaY, Z0 = blockIdx.z * kSaZ;
+    const LdsBox bf{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
+    const LdsBox bg{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 3, kSaZ + 3}};
+    const LdsBox bo{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 2}};
+    const float* fc = C == 0 ? u : (C == 1 ? v : w);
+    const float* f0 = D0 == 0 ? u : v;
+    const float* f1 = D1 == 1 ? v : w;
+    auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
+        const int n = box_size(bx);
+        for (int q = tid; q < n; q += kSaX * kSaY) {
+            const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
+            lds[q] = load_face(g, f, comp, i, j, k);
+        }
+    };
+    stage_face(C, fc, sf, bf);
+    stage_face(D0, f0, sg0, bg);
+    stage_face(D1, f1, sg1, bg);
+    for (int q = tid; q < kO; q += kSaX * kSaY) {
+        const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+        // entries outside the domain are never read (both tests check the domain first)
+        so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+    }
+    __syncthreads();
+    const int i = X0 + tid % kSaX, j = Y0 + tid / kSaX;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    // staged copies of u, v, w in component order
+    const float* lu = C == 0 ? sf : sg0;
+    const LdsBox& bu = C == 0 ? bf : bg;
+    const float* lv = C == 1 ? sf : (C == 0 ? sg0 : sg1);
+    const LdsBox& bv = C == 1 ? bf : bg;
+    const float* lw = C == 2 ? sf : sg1;
+    const LdsBox& bw = C == 2 ? bf : bg;
+    const float h = g.h;
+    const float dt = *dt_ptr;
+    const bool cubic = g.cubic != 0;
+    for (int tz = 0; tz < kSaZ; ++tz) {
+        const int k = Z0 + tz;
+        if (k >= g.n[2]) break;
+        const int c3[3] = {i, j, k};
+        if (boundary_face(g, C, c3[C])) continue;
+        const int64_t fi = fidx(g, C, i, j, k);
+        // solid_face_override (B:1007-1012) returns false when both cells
+        // beside the face lie in the domain and are fluid
+        bool free_face = false;
+        if (c3[C] >= 1 && c3[C] < g.n[C]) {
+            int l[3] = {i, j, k};
+            l[C] -= 1;
+            free_face = so[box_base(bo, l[0], l[1], l[2])] == 0 && so[box_base(bo, i, j, k)] == 0;
+        }
+        if (!free_face) {
+            float value;
+            if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
+                dst[fi] = value;
+                continue;
+            }
+        }
+        const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
+        const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
+        const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
+        float px, py, pz;
+        bool ok = staged_trace(g, lu, bu, lv, bv, lw, bw, so, bo, dt, sx, sy, sz, px, py, pz);
+        Stencil st{};
+        if (ok) {
+            st = locate(g, g.flow_periodic, C, px, py, pz);
+            ok = cubic ? in_box<-1, 2>(bf, st) : in_box<0, 1>(bf, st);
+        }
+        if (!ok) {
+            fallback[atomicAdd(fallback_count, 1u)] = int(fi);
+            continue;
+        }
+        dst[fi] = cubic ? staged_cubic(sf, bf, st) : staged_linear(sf, bf, st);
+    }
+}
+
+// the faces the staged kernel could not advect from LDS, by the global path
+template <int C>
+__global__ __launch_bounds__(256) void advect_face_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                   const float* __restrict__ w, float* __restrict__ dst, const uint8_t* __restrict__ occ,
+                                                                   const float* __restrict__ dt_ptr, const int* __restrict__ fallback,
+                                                                   const unsigned* __restrict__ fallback_count) {
+    const unsigned n = *fallback_count;
+    const float dt = *dt_ptr;
+    const int d0 = g.n[0] + (C == 0), d1 = g.n[1] + (C == 1);
+    for (unsigned q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+        const int fi = fallback[q];
+        const int i = fi % d0, j = (fi / d0) % d1, k = fi / (d0 * d1);
+        dst[fi] = advect_face_value<C>(g, u, v, w, occ, dt, i, j, k);
+    }
+}
+
+template <int C>
+void launch_advect(Context& c, int block_index) {
+    const Grid& g = c.grid;
+    if (block_index == kAdvectStaged) {
+        // tiles over face indices below n (at most `cells` faces reach the list)
+        const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
+        zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
+        advect_face_staged_kernel<C><<<tiles, kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy,
+                                                                         c.solid_velocity[C], c.dt_device, c.advect_fallback,
+                                                                         c.advect_fallback_count);
+        advect_face_fallback_kernel<C><<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.dt_device,
+                                                                   c.advect_fallback, c.advect_fallback_count);
+        hip_check(hipGetLastError(), "staged velocity advection");
+        return;
+    }
+    const dim3 block = kAdvectBlocks[block_index];
+    const dim3 grid = grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), block);
+    advect_face_kernel<C><<<grid, block, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
+                                                        c.dt_device);
+}
+
 // T and rho advection from (Ttmp, rtmp) into (T, rho)
 void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
     const Grid& g = c.grid;
-    if (c.advect_fallback) {
+    if (block_index == kAdvectStaged) {
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
         advect_scalars_staged_kernel<<<tiles, kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
@@ -863,9 +1020,24 @@ void table_advect_blocks(Context& c) {
 }
 
 void describe_advect_block(int index, SmokeSimulationLaunchShape& out) {
-    if (index < 0 || index >= kAdvectBlockCount) throw HipError("invalid advection block");
+    if (index < 0 || index >= kAdvectCandidates) throw HipError("invalid advection launch");
+    if (index == kAdvectStaged) {
+        out = {kSaX, kSaY, 1, kSaZ};
+        return;
+    }
     const dim3 b = kAdvectBlocks[index];
     out = {int32_t(b.x), int32_t(b.y), int32_t(b.z), 1};
+}
+
+int find_advect_block(const SmokeSimulationLaunchShape& shape) {
+    for (int b = 0; b < kAdvectCandidates; ++b) {
+        SmokeSimulationLaunchShape d;
+        describe_advect_block(b, d);
+        if (d.threads_x == shape.threads_x && d.threads_y == shape.threads_y && d.threads_z == shape.threads_z &&
+            d.cells_per_thread == shape.cells_per_thread)
+            return b;
+    }
+    throw HipError("advection launch shape is not a candidate");
 }
 
 // Times every candidate block of the four advection kernels on the current
@@ -881,7 +1053,7 @@ int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report) {
     for (int k = 0; k < 4; ++k) {
         float best_us = 0.0f;
         int best = c.advect_block[k];
-        for (int b = 0; b < kAdvectBlockCount; ++b) {
+        for (int b = 0; b < kAdvectCandidates; ++b) {
             auto launch = [&] {
                 if (k == 0) launch_advect<0>(c, b);
                 else if (k == 1) launch_advect<1>(c, b);

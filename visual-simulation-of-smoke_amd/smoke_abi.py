@@ -113,10 +113,10 @@ class KernelTiming(C.Structure):
 
 
 class LaunchShape(C.Structure):
-    _fields_ = [("threads_x", C.c_int32), ("threads_y", C.c_int32), ("threads_z", C.c_int32), ("cells_x", C.c_int32)]
+    _fields_ = [("threads_x", C.c_int32), ("threads_y", C.c_int32), ("threads_z", C.c_int32), ("cells_per_thread", C.c_int32)]
 
     def tuple(self):
-        return (self.threads_x, self.threads_y, self.threads_z, self.cells_x)
+        return (self.threads_x, self.threads_y, self.threads_z, self.cells_per_thread)
 
 
 class LaunchReport(C.Structure):
@@ -293,6 +293,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_autotune_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_reset_launches": [C.c_void_p],
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
+        "smoke_simulation_ext_set_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
         "smoke_simulation_ext_set_learned_preconditioner": [C.c_void_p, C.c_void_p],
         "smoke_simulation_ext_default_multilevel_options": [C.POINTER(MultilevelOptions)],
@@ -486,6 +487,10 @@ class SmokeContext:
         r = LaunchReport()
         check(self.lib.smoke_simulation_ext_get_launches(self.handle, C.byref(r)), "get_launches")
         return r
+
+    def set_launches(self, report: LaunchReport):
+        """Applies the launch shapes of `report` (see launches())."""
+        check(self.lib.smoke_simulation_ext_set_launches(self.handle, C.byref(report)), "set_launches")
 
     def set_multilevel_options(self, **kw):
         o = MultilevelOptions()
