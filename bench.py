@@ -180,6 +180,46 @@ def cpu_baseline(n: int, iterations_per_step: float, threads: int):
     }
 
 
+def batch_levels(cfg: abi.Config) -> int:
+    """V-cycle depth whose coarsest level keeps at least 4 cells per axis."""
+    m, levels = min(cfg.nx, cfg.ny, cfg.nz), 1
+    while m >= 8 and levels < 7:
+        m, levels = (m + 1) // 2, levels + 1
+    return levels
+
+
+def run_batch(count: int, steps: int, warmup: int):
+    """The batched-plume workload (batch_plumes.plume_set(count, seed=3)):
+    steps/s of the whole batch on 4 lanes and on 1, in the reference's
+    fixed-iteration mode (64 plain CG iterations, one captured graph per
+    plume) and to tolerance 1e-6 (FPCG + V(4,4)-cycle)."""
+    import batch_plumes as bp
+    plumes = bp.plume_set(count, seed=3)
+    cells = bp.total_cells(plumes)
+    out = {"plumes": count, "cells": cells, "grids": sorted({f"{c.nx}x{c.ny}x{c.nz}" for c, _ in plumes}), "steps": steps}
+    for mode in ("fixed", "tolerance"):
+        for lanes in (4, 1):
+            with abi.SmokeBatch([c for c, _ in plumes], lanes=lanes) as batch:
+                for (cfg, _), p in zip(plumes, batch.plumes):
+                    if mode == "tolerance":
+                        p.set_multilevel_options(levels=batch_levels(cfg), pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0,
+                                                 coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=8)
+                        solver_options(p, 32, preconditioner=abi.PRECONDITIONER_MULTILEVEL)
+                srcs = [s for _, s in plumes]
+                for k in range(warmup):
+                    batch.step(srcs, k)
+                batch.synchronize()
+                t0 = time.perf_counter()
+                for k in range(warmup, warmup + steps):
+                    batch.step(srcs, k)
+                batch.synchronize()
+                dt = time.perf_counter() - t0
+            out[f"{mode}_lanes{lanes}"] = {"batch_steps_per_s": round(steps / dt, 3), "mcells_per_s": round(cells * steps / dt / 1e6, 1)}
+        out[f"{mode}_lane_speedup"] = round(out[f"{mode}_lanes4"]["batch_steps_per_s"] / out[f"{mode}_lanes1"]["batch_steps_per_s"], 3)
+        print(f"batch {mode}: {out[f'{mode}_lanes4']} on 4 lanes, {out[f'{mode}_lanes1']} on 1", file=sys.stderr, flush=True)
+    return out
+
+
 def load_traffic(name: str):
     f = REPO / "profiles" / name
     if f.exists():
@@ -206,6 +246,9 @@ def main():
     ap.add_argument("--solvers", default="learned,jacobi,multilevel",
                     help="comma-separated, from learned, jacobi, multilevel, smoothed; the first one is the reported value")
     ap.add_argument("--weights", default=str(WEIGHTS))
+    ap.add_argument("--batch-plumes", type=int, default=0,
+                    help="also time the batched-plume workload with this many plumes (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -289,6 +332,8 @@ def main():
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,
         "roofline": roofline, "cpu_baseline": None,
     }
+    if args.batch_plumes > 0:
+        out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2)
     if not args.no_cpu_baseline and "jacobi" in results:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, results["jacobi"]["pcg_iterations_per_step"], threads)

@@ -14,7 +14,8 @@
  *     backend-cuda.cu:2243-2248),
  *   - running a sub-range of the step's stages (stage-level parity checks),
  *   - on-disk checkpoint / restore of the state,
- *   - per-kernel device timing and a stream-copy bandwidth probe.
+ *   - per-kernel device timing and a stream-copy bandwidth probe,
+ *   - batches of independent plumes of uneven sizes advanced together.
  * Host-pointer entry points here are synchronous; the rest are stream-ordered.
  */
 #ifndef VISUAL_SIMULATION_OF_SMOKE_3D_EXT_H
@@ -352,6 +353,48 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_time_multilevel(
    until it is full.  NULL stops the capture; setting a buffer restarts the
    count. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_multilevel_capture(SmokeSimulationContext context, float* buffer, uint32_t capacity);
+
+/* Batched plumes: many independent simulations of uneven grid sizes advanced
+   together (the reference drives one context per scene, scene_plume.cpp:246-392).
+   A batch owns `lanes` HIP streams (1..8, 0 for 4: the device's default
+   hardware queues per process) and one context per plume, created from
+   `plumes[i]` with its stream set to the plume's lane (the desc's own stream
+   field is ignored).  Plumes are assigned to lanes longest first: in
+   decreasing cell count (ties: lower index first), each to the lane with the
+   fewest cells so far (ties: lower lane); within a lane they run in index
+   order.  The per-plume contexts take every entry point of the context ABI
+   (updates, views, options, learned preconditioner); destroy them only
+   through destroy_batch.  A plume's results are bitwise those of a context
+   created alone from the same desc and driven the same way. */
+typedef struct SmokeSimulationBatch_t* SmokeSimulationBatch;
+
+typedef struct SmokeSimulationBatchCreateDesc {
+    const SmokeSimulationContextCreateDesc* plumes;
+    uint32_t count;
+    int32_t lanes;
+} SmokeSimulationBatchCreateDesc;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_create_batch(const SmokeSimulationBatchCreateDesc* desc, SmokeSimulationBatch* out_batch);
+/* Synchronizes every lane, then destroys the plumes' contexts and the lanes.
+   NULL is a no-op. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_destroy_batch(SmokeSimulationBatch batch);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_batch_context(SmokeSimulationBatch batch, uint32_t index, SmokeSimulationContext* out_context,
+                                                                            int32_t* out_lane);
+/* One step of every plume; with `sources` (one per plume, or NULL) each
+   plume's plume_sources(sources[i], step) is enqueued just before its step.
+   Plumes in FIXED_ITERATIONS mode enqueue one captured graph each, from the
+   calling thread, interleaved over the lanes, and the call returns without
+   waiting; when any plume solves to a tolerance (its step waits on its
+   solve), one host thread per lane drives that lane's plumes and the call
+   returns when every lane has enqueued its last step.  A learned network
+   attached to plumes on two lanes would run on two streams at once: such a
+   batch fails here (attach one network per lane). */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_step_batch(SmokeSimulationBatch batch, const SmokeSimulationPlumeSource* sources,
+                                                                         uint32_t step);
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_synchronize_batch(SmokeSimulationBatch batch);
+/* The lane assignment rule above, on the host (no device needed):
+   out_lane[i] for plume i of `count` plumes with cells[i] cells. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_batch_lanes(const uint64_t* cells, uint32_t count, int32_t lanes, int32_t* out_lane);
 
 #ifdef __cplusplus
 }

@@ -97,8 +97,8 @@ def load_oracle(s: orc.OracleState, st):
         s["pressure"] = st["pressure"]
 
 
-def tolerance_options(ctx, tol=1e-6, adaptive=False, cfl=0.5, dt_max=0.01, warm=1):
-    ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI, relative_tolerance=tol,
+def tolerance_options(ctx, tol=1e-6, adaptive=False, cfl=0.5, dt_max=0.01, warm=1, preconditioner=abi.PRECONDITIONER_JACOBI):
+    ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=preconditioner, relative_tolerance=tol,
                            max_iterations=20000, max_refinements=6, warm_start=warm, adaptive_dt=1 if adaptive else 0, cfl=cfl,
                            dt_max=dt_max, check_interval=16)
 

@@ -21,7 +21,6 @@
 #include "context.hpp"
 #include "visual-simulation-of-smoke-3d-ext.h"
 
-struct SmokeSimulationContext_t : smoke::Context {};
 
 namespace smoke {
 namespace {

@@ -227,3 +227,6 @@ int find_advect_block(const SmokeSimulationLaunchShape& shape);   // candidate i
 int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report);
 
 }  // namespace smoke
+
+// The ABI's opaque context handle.
+struct SmokeSimulationContext_t : smoke::Context {};

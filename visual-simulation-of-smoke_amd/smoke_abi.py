@@ -149,6 +149,10 @@ class PlumeSource(C.Structure):
                 ("temperature_noise", C.c_float), ("seed", C.c_uint32)]
 
 
+class BatchCreateDesc(C.Structure):
+    _fields_ = [("plumes", C.c_void_p), ("count", C.c_uint32), ("lanes", C.c_int32)]
+
+
 # ---------------------------------------------------------------- config helpers
 def make_config(nx, ny, nz, *, cell_size=None, dt=0.01, pressure_iterations=64, ambient=0.0,
                 buoyancy_density=0.05, buoyancy_temperature=0.25, vorticity=2.0, cubic=True,
@@ -304,6 +308,12 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_export_level_flags": [C.c_void_p, C.c_int32, C.c_void_p],
         "smoke_simulation_ext_set_multilevel_capture": [C.c_void_p, C.c_void_p, C.c_uint32],
         "smoke_simulation_ext_time_multilevel": [C.c_void_p, C.c_int32, C.POINTER(MultilevelTiming)],
+        "smoke_simulation_ext_create_batch": [C.POINTER(BatchCreateDesc), C.POINTER(C.c_void_p)],
+        "smoke_simulation_ext_destroy_batch": [C.c_void_p],
+        "smoke_simulation_ext_batch_context": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)],
+        "smoke_simulation_ext_step_batch": [C.c_void_p, C.c_void_p, C.c_uint32],
+        "smoke_simulation_ext_synchronize_batch": [C.c_void_p],
+        "smoke_simulation_ext_batch_lanes": [C.POINTER(C.c_uint64), C.c_uint32, C.c_int32, C.POINTER(C.c_int32)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -604,3 +614,80 @@ def measure_copy_bandwidth(nbytes: int, reps: int):
     gbps, ms = C.c_double(), C.c_double()
     check(lib.smoke_simulation_ext_measure_copy_bandwidth(nbytes, reps, C.byref(gbps), C.byref(ms)), "measure_copy_bandwidth")
     return gbps.value, ms.value
+
+
+# ---------------------------------------------------------------- batched plumes
+def batch_lanes(cells, lanes: int) -> list[int]:
+    """Lane of each plume under the batch's assignment rule (host only)."""
+    lib = load_library()
+    n = len(cells)
+    arr = (C.c_uint64 * max(n, 1))(*[int(c) for c in cells])
+    out = (C.c_int32 * max(n, 1))()
+    check(lib.smoke_simulation_ext_batch_lanes(arr, n, lanes, out), "batch_lanes")
+    return list(out[:n])
+
+
+class _BatchPlume(SmokeContext):
+    """A plume context owned by a SmokeBatch: every context method applies;
+    closing it frees only its uploads (the batch destroys the context)."""
+
+    def __init__(self, batch, handle, cfg):
+        self.lib = batch.lib
+        self.hip = batch.hip
+        self.cfg = cfg
+        self.handle = handle
+        self._owned = []
+
+    def close(self):
+        for p in self._owned:
+            self.hip.free(p)
+        self._owned = []
+
+
+class SmokeBatch:
+    """Independent plumes of uneven sizes stepped together on `lanes` streams."""
+
+    def __init__(self, cfgs, lanes: int = 0, initial_density=0.0, hip: Hip | None = None):
+        self.lib = load_library()
+        self.hip = hip or Hip()
+        n = len(cfgs)
+        descs = (CreateDesc * n)()
+        for i, cfg in enumerate(cfgs):
+            descs[i].config = cfg
+            descs[i].stream = None
+            descs[i].initial_density = initial_density
+            descs[i].initial_temperature = cfg.ambient_temperature
+        d = BatchCreateDesc(C.cast(descs, C.c_void_p), n, lanes)
+        h = C.c_void_p()
+        check(self.lib.smoke_simulation_ext_create_batch(C.byref(d), C.byref(h)), "create_batch")
+        self.handle = h
+        self.plumes, self.lanes = [], []
+        for i, cfg in enumerate(cfgs):
+            ctx, lane = C.c_void_p(), C.c_int32()
+            check(self.lib.smoke_simulation_ext_batch_context(h, i, C.byref(ctx), C.byref(lane)), "batch_context")
+            self.plumes.append(_BatchPlume(self, ctx, cfg))
+            self.lanes.append(lane.value)
+
+    def step(self, sources=None, step: int = 0):
+        buf = None
+        if sources is not None:
+            assert len(sources) == len(self.plumes)
+            buf = (PlumeSource * len(sources))(*sources)
+        check(self.lib.smoke_simulation_ext_step_batch(self.handle, C.cast(buf, C.c_void_p) if buf is not None else None, step), "step_batch")
+
+    def synchronize(self):
+        check(self.lib.smoke_simulation_ext_synchronize_batch(self.handle), "synchronize_batch")
+
+    def close(self):
+        if self.handle:
+            self.synchronize()
+            for p in self.plumes:
+                p.close()
+            check(self.lib.smoke_simulation_ext_destroy_batch(self.handle), "destroy_batch")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
