@@ -128,8 +128,9 @@ typedef struct SmokeSimulationKernelTiming {
     double pcg_reduction_ms;      /* summed device time of the two one-block finalize kernels (alpha; beta, stop test) */
     double step_ms;               /* summed device time of whole steps */
     uint64_t steps;
-    double mg_sweep_ms;           /* multilevel preconditioner: summed device time of the last damped-Jacobi
-                                     sweep on the pressure grid of every V-cycle (bracketed by events) */
+    double mg_sweep_ms;           /* multilevel preconditioner: summed device time of the damped-Jacobi
+                                     sweeps on the pressure grid (all but the first pre-sweep, which is
+                                     x = omega D^-1 r; each bracketed by events) */
     uint64_t mg_sweeps;           /* sweeps so timed */
 } SmokeSimulationKernelTiming;
 
@@ -356,8 +357,7 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_multilevel_c
 
 /* Batched plumes: many independent simulations of uneven grid sizes advanced
    together (the reference drives one context per scene, scene_plume.cpp:246-392).
-   A batch owns `lanes` HIP streams (1..8, 0 for 4: the device's default
-   hardware queues per process) and one context per plume, created from
+   A batch owns `lanes` HIP streams (1..16, 0 for 4) and one context per plume, created from
    `plumes[i]` with its stream set to the plume's lane (the desc's own stream
    field is ignored).  Plumes are assigned to lanes longest first: in
    decreasing cell count (ties: lower index first), each to the lane with the

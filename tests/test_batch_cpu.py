@@ -35,7 +35,7 @@ def test_lane_rule_matches_restatement(seed):
     rng = np.random.default_rng(seed)
     n = int(rng.integers(1, 70))
     cells = [int(c) for c in rng.choice([32 ** 3, 48 ** 3, 64 ** 3, 96 * 64 * 64, 128 * 64 * 64, 96 ** 3, 128 ** 3], size=n)]
-    for lanes in (1, 2, 3, 4, 8):
+    for lanes in (1, 2, 3, 4, 8, 16):
         assert abi.batch_lanes(cells, lanes) == _lpt(cells, lanes)
 
 
@@ -53,7 +53,7 @@ def test_lane_rule_ties_and_arguments():
     assert abi.batch_lanes([5, 5, 5, 5], 2) == [0, 1, 0, 1]
     assert abi.batch_lanes([1, 9], 4) == [1, 0]
     assert abi.batch_lanes([], 4) == []
-    for bad in (0, 9, -1):
+    for bad in (0, 17, -1):
         with pytest.raises(abi.SmokeError):
             abi.batch_lanes([1, 2], bad)
 

@@ -28,7 +28,7 @@ struct SmokeSimulationBatch_t {
 namespace {
 
 constexpr int kDefaultLanes = 4;
-constexpr int kMaxLanes = 8;
+constexpr int kMaxLanes = 16;
 
 // Longest processing time first: plumes by decreasing cells (stable), each to
 // the least loaded lane (lowest index on ties).
@@ -174,8 +174,8 @@ SmokeSimulationResult smoke_simulation_ext_step_batch(SmokeSimulationBatch batch
         return SMOKE_SIMULATION_RESULT_OK;
     }
     std::vector<SmokeSimulationResult> results(lanes, SMOKE_SIMULATION_RESULT_OK);
+    std::vector<std::thread> workers;
     try {
-        std::vector<std::thread> workers;
         workers.reserve(lanes);
         for (size_t l = 0; l < lanes; ++l) {
             workers.emplace_back([&b, &results, l, sources, step] {
@@ -190,9 +190,10 @@ SmokeSimulationResult smoke_simulation_ext_step_batch(SmokeSimulationBatch batch
             });
         }
         for (std::thread& t : workers) t.join();
-    } catch (const std::bad_alloc&) {
-        return SMOKE_SIMULATION_RESULT_OUT_OF_MEMORY;
     } catch (...) {
+        // a worker that could not start: wait for the ones that did
+        for (std::thread& t : workers)
+            if (t.joinable()) t.join();
         return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     }
     for (SmokeSimulationResult r : results)

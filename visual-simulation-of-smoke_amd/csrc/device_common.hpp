@@ -12,6 +12,30 @@
 
 namespace smoke {
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
+// with its own L2.  A kernel over a grid of tiles (numbered x fastest, then y,
+// then z, as a 3-D launch numbers its blocks) launched as xcd_blocks(tiles)
+// one-dimensional blocks takes its tile from xcd_tile: XCD k walks the
+// contiguous tiles [k ceil(T/8), (k+1) ceil(T/8)), so the halo rows a tile
+// reads from its neighbours were mostly brought into the same L2 by them.
+// Blocks past the last tile return at once (the whole block: the test
+// depends on blockIdx only).
+constexpr unsigned kXcds = 8;
+
+inline unsigned xcd_blocks(const dim3& tiles) { return (tiles.x * tiles.y * tiles.z + kXcds - 1) / kXcds * kXcds; }
+
+__device__ __forceinline__ bool xcd_tile(const uint3& tiles, uint3& tile, unsigned& id) {
+    const unsigned total = tiles.x * tiles.y * tiles.z;
+    const unsigned per = (total + kXcds - 1) / kXcds;
+    id = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (id >= total) return false;
+    tile.x = id % tiles.x;
+    const unsigned rest = id / tiles.x;
+    tile.y = rest % tiles.y;
+    tile.z = rest / tiles.y;
+    return true;
+}
+
 // Boundary kinds (values of the ABI enums).
 enum : uint32_t { kNoSlip = 0, kFreeSlip = 1, kOutflow = 2, kPeriodic = 3 };
 enum : uint32_t { kFixedValue = 0, kZeroFlux = 1, kScalarPeriodic = 2 };

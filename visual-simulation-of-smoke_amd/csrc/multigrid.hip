@@ -248,11 +248,14 @@ __device__ __forceinline__ void row4_residual(const LevelArgs& a, const float* _
 // DOT: also the block's partial sum of r . out (FP64), written to
 // partials[block] -- the r.z of flexible PCG when `out` is its z
 template <bool DOT>
-__global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb,
+__global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, uint3 tiles, const float* __restrict__ r, const float* __restrict__ xb,
                                                         float* __restrict__ out, double* __restrict__ partials) {
-    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
+    const int x0 = (int(tile.x) * 64 + (threadIdx.x & 63)) * 4;
+    const int y = int(tile.y) * 4 + (threadIdx.x >> 6);
+    const int z = int(tile.z);
     const bool inside = x0 < a.n[0] && y < a.n[1];
     double dot = 0.0;
     if (inside) {
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, const float
         for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
         __syncthreads();
-        if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (threadIdx.x == 0) partials[id] = (red[0] + red[1]) + (red[2] + red[3]);
     }
 }
 
@@ -311,13 +314,16 @@ __global__ __launch_bounds__(256) void mg_prolong4_kernel(LevelArgs a, Coarse cc
 // restriction with float4 rows: a 64 x 4 block covers 256 x cells, 4 rows and
 // two z planes; each thread sums its 2 x 4 cells into two coarse x cells, the
 // y pair is summed through LDS
-__global__ __launch_bounds__(256) void mg_restrict4_kernel(LevelArgs a, const float* __restrict__ r, const float* __restrict__ xb, int nc0, int nc1,
-                                                           const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
+__global__ __launch_bounds__(256) void mg_restrict4_kernel(LevelArgs a, uint3 tiles, const float* __restrict__ r, const float* __restrict__ xb, int nc0,
+                                                           int nc1, const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
     __shared__ float2 rows[4][64];
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
     const int lane = threadIdx.x & 63, rowi = threadIdx.x >> 6;
-    const int x0 = (blockIdx.x * 64 + lane) * 4;
-    const int y = blockIdx.y * 4 + rowi;
-    const int Z = blockIdx.z;
+    const int x0 = (int(tile.x) * 64 + lane) * 4;
+    const int y = int(tile.y) * 4 + rowi;
+    const int Z = int(tile.z);
     float s0 = 0.0f, s1 = 0.0f;
     if (x0 < a.n[0] && y < a.n[1]) {
 #pragma unroll
@@ -697,6 +703,7 @@ void vcycle(Context& c, int l) {
     // the restriction must be a float2-aligned pair: nc0 even)
     const bool vec = split && L.n[0] % 8 == 0;
     const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
+    const uint3 t4 = make_uint3(g4.x, g4.y, g4.z);
     float* xb = nullptr;   // x in a buffer; nullptr: x = omega D^-1 r on the fly
     if (split) {
         if (vec) mg_scale4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, L.ta);
@@ -705,22 +712,29 @@ void vcycle(Context& c, int l) {
         xb = L.ta;
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = xb == L.ta ? L.tb : L.ta;
-            if (vec) mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, L.r, xb, dst, nullptr);
+            hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
+            if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
+            if (vec) mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, xb, dst, nullptr);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
+            if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
             xb = dst;
         }
     } else {
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = bufs[k & 1];
+            hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
+            if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
             if (k == 2) sweep(c, L, kFromR, L.r, nullptr, none, dst);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
+            if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
             xb = dst;
         }
     }
     // fine-cell threads, two z planes per block
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
     const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    if (vec) mg_restrict4_kernel<<<gc4, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
+    const uint3 tc4 = make_uint3(gc4.x, gc4.y, gc4.z);
+    if (vec) mg_restrict4_kernel<<<xcd_blocks(gc4), 256, 0, c.stream>>>(a, tc4, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
     else if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
     else mg_restrict_kernel<kFromR><<<gc, 256, 0, c.stream>>>(a, L.r, nullptr, C.n[0], C.n[1], C.flags, C.r);
     hip_check(hipGetLastError(), "mg_restrict_kernel");
@@ -744,14 +758,14 @@ void vcycle(Context& c, int l) {
     for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
         const bool last0 = l == 0 && k == o.post_sweeps;
-        hipEvent_t* ev = (c.timing && last0) ? next_timing_pair(c) : nullptr;
+        hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
         if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
         const int64_t blocks4 = int64_t(g4.x) * g4.y * g4.z;
         if (vec && last0 && m.dot_partials && blocks4 <= m.dot_capacity) {
-            mg_sweep4_kernel<true><<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst, m.dot_partials);
+            mg_sweep4_kernel<true><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, cur, dst, m.dot_partials);
             m.dot_blocks = int(blocks4);
         } else if (vec) {
-            mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, L.r, cur, dst, nullptr);
+            mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, cur, dst, nullptr);
         } else {
             sweep(c, L, kBuffer, L.r, cur, none, dst);
         }
@@ -905,6 +919,7 @@ SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
     const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
     const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
+    const uint3 t4 = make_uint3(g4.x, g4.y, g4.z), tc4 = make_uint3(gc4.x, gc4.y, gc4.z);
     const Coarse none{nullptr, 0, 0};
     const Coarse cc{C.z, C.n[0], C.n[0] * C.n[1]};
     const float* r = c.pcg_r;
@@ -930,11 +945,11 @@ SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
         else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, r, L.ta);
     });
     out.sweep_us = time([&] {
-        if (vec) mg_sweep4_kernel<false><<<g4, 256, 0, c.stream>>>(a, r, L.ta, L.tb, nullptr);
+        if (vec) mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, r, L.ta, L.tb, nullptr);
         else mg_sweep_kernel<kBuffer><<<g, 256, 0, c.stream>>>(a, r, L.ta, none, L.tb);
     });
     out.restrict_us = time([&] {
-        if (vec) mg_restrict4_kernel<<<gc4, 256, 0, c.stream>>>(a, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
+        if (vec) mg_restrict4_kernel<<<xcd_blocks(gc4), 256, 0, c.stream>>>(a, tc4, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
         else mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
     });
     hip_check(hipMemsetAsync(C.z, 0, size_t(C.cells) * sizeof(float), c.stream), "zero correction");

@@ -643,7 +643,8 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
                                                                            const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                                                            float* __restrict__ T, float* __restrict__ rho,
                                                                            const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
-                                                                           int* __restrict__ fallback, unsigned* __restrict__ fallback_count) {
+                                                                           int* __restrict__ fallback, unsigned* __restrict__ fallback_count,
+                                                                           uint3 tiles) {
     constexpr int kS = (kSaX + 4) * (kSaY + 4) * (kSaZ + 4);
     constexpr int kU = (kSaX + 3) * (kSaY + 2) * (kSaZ + 2);
     constexpr int kV = (kSaX + 2) * (kSaY + 3) * (kSaZ + 2);
@@ -651,8 +652,11 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
     constexpr int kO = (kSaX + 2) * (kSaY + 2) * (kSaZ + 2);
     __shared__ float sT[kS], sR[kS], su[kU], sv[kV], sw[kW];
     __shared__ uint8_t so[kO];
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
     const int tid = threadIdx.x;
-    const int X0 = blockIdx.x * kSaX, Y0 = blockIdx.y * kSaY, Z0 = blockIdx.z * kSaZ;
+    const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
     const LdsBox bs{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
     const LdsBox bu{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 2, kSaZ + 2}};
     const LdsBox bv{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 3, kSaZ + 2}};
@@ -770,7 +774,7 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, 
                                                                         const float* __restrict__ w, float* __restrict__ dst,
                                                                         const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
                                                                         const float* __restrict__ dt_ptr, int* __restrict__ fallback,
-                                                                        unsigned* __restrict__ fallback_count) {
+                                                                        unsigned* __restrict__ fallback_count, uint3 tiles) {
     constexpr int kF = (kSaX + 4) * (kSaY + 4) * (kSaZ + 4);
     constexpr int kG = (kSaX + 3) * (kSaY + 3) * (kSaZ + 3);
     constexpr int kO = (kSaX + 2) * (kSaY + 2) * (kSaZ + 2);
@@ -778,8 +782,11 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, 
     constexpr int D0 = C == 0 ? 1 : 0, D1 = C == 2 ? 1 : 2;
     __shared__ float sf[kF], sg0[kG], sg1[kG];
     __shared__ uint8_t so[kO];
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
     const int tid = threadIdx.x;
-    const int X0 = blockIdx.x * kSaX, Y0 = blockIdx.y * kSaY, Z0 = blockIdx.z * kSaZ;
+    const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
     const LdsBox bf{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
     const LdsBox bg{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 3, kSaZ + 3}};
     const LdsBox bo{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 2}};
@@ -876,9 +883,10 @@ void launch_advect(Context& c, int block_index) {
         // tiles over face indices below n (at most `cells` faces reach the list)
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_face_staged_kernel<C><<<tiles, kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy,
-                                                                         c.solid_velocity[C], c.dt_device, c.advect_fallback,
-                                                                         c.advect_fallback_count);
+        advect_face_staged_kernel<C><<<xcd_blocks(tiles), kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C],
+                                                                                      c.occupancy, c.solid_velocity[C], c.dt_device,
+                                                                                      c.advect_fallback, c.advect_fallback_count,
+                                                                                      make_uint3(tiles.x, tiles.y, tiles.z));
         advect_face_fallback_kernel<C><<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.dt_device,
                                                                    c.advect_fallback, c.advect_fallback_count);
         hip_check(hipGetLastError(), "staged velocity advection");
@@ -896,9 +904,10 @@ void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
     if (block_index == kAdvectStaged) {
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_scalars_staged_kernel<<<tiles, kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
-                                                                         c.density_tmp, T, rho, c.solid_temperature, c.dt_device,
-                                                                         c.advect_fallback, c.advect_fallback_count);
+        advect_scalars_staged_kernel<<<xcd_blocks(tiles), kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
+                                                                                      c.temperature_tmp, c.density_tmp, T, rho, c.solid_temperature,
+                                                                                      c.dt_device, c.advect_fallback, c.advect_fallback_count,
+                                                                                      make_uint3(tiles.x, tiles.y, tiles.z));
         advect_scalars_fallback_kernel<<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
                                                                  c.density_tmp, T, rho, c.solid_temperature, c.dt_device, c.advect_fallback,
                                                                  c.advect_fallback_count);
