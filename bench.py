@@ -302,8 +302,8 @@ def main():
             "traffic": load_traffic("mg_sweep_256.traffic.json"), "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
             "avg_launch_ms": round(sweep_us * 1e-3, 5), "launches_timed": int(lk.mg_sweeps),
             "timed": ("every sweep of the pressure grid in the V-cycles of every 8th timed step (7 per cycle: the 3 "
-                      "pre-sweeps after x = omega D^-1 r and the 4 post-sweeps, the last of which also sums r.z), each "
-                      "bracketed by HIP events on the context stream"),
+                      "pre-sweeps after x = omega D^-1 r and the 4 post-sweeps, the last of which also sums r.z); HIP "
+                      "events on the context stream around each run of back-to-back sweeps, divided by its sweeps"),
             "isolated_launch_ms": round(lt.sweep_us * 1e-3, 5),
             "vcycle_kernels": {
                 "restrict": {"avg_launch_ms": round(lt.restrict_us * 1e-3, 5), "achieved_gbps": round(gb(RESTRICT_BYTES_PER_CELL, lt.restrict_us), 1)},

@@ -130,7 +130,8 @@ typedef struct SmokeSimulationKernelTiming {
     uint64_t steps;
     double mg_sweep_ms;           /* multilevel preconditioner: summed device time of the damped-Jacobi
                                      sweeps on the pressure grid (all but the first pre-sweep, which is
-                                     x = omega D^-1 r; each bracketed by events) */
+                                     x = omega D^-1 r), one event pair around each run of back-to-back
+                                     sweeps (the pre-sweeps, the post-sweeps) */
     uint64_t mg_sweeps;           /* sweeps so timed */
 } SmokeSimulationKernelTiming;
 

@@ -81,12 +81,14 @@ struct Multilevel {
     float* capture = nullptr;    // coarsest-level right-hand sides (training data), not owned
     uint32_t capacity = 0;
     uint32_t captured = 0;
-    // kernel timing: a ring of event pairs around the pressure grid's last
-    // sweep of each V-cycle, read back once the ring comes round (by then the
-    // pair has long completed) or when the totals are read
+    // kernel timing: a ring of event pairs, each around one run of
+    // back-to-back sweeps of the pressure grid (timing_count of them), read
+    // back once the ring comes round (by then the pair has long completed) or
+    // when the totals are read
     static constexpr int kTimingRing = 64;
     hipEvent_t timing_events[kTimingRing][2] = {};
     bool timing_pending[kTimingRing] = {};
+    int timing_count[kTimingRing] = {};
     int timing_next = 0;
 };
 

@@ -631,12 +631,13 @@ void fold_timing_pair(Context& c, int k) {
     float ms = 0.0f;
     hip_check(hipEventElapsedTime(&ms, m.timing_events[k][0], m.timing_events[k][1]), "elapsed");
     c.timing_totals.mg_sweep_ms += ms;
-    c.timing_totals.mg_sweeps += 1;
+    c.timing_totals.mg_sweeps += uint64_t(m.timing_count[k]);
     m.timing_pending[k] = false;
 }
 
-// the event pair for the next timed launch (created on first use)
-hipEvent_t* next_timing_pair(Context& c) {
+// the event pair around the next `count` back-to-back timed launches (created
+// on first use)
+hipEvent_t* next_timing_pair(Context& c, int count) {
     Multilevel& m = c.mg;
     const int k = m.timing_next;
     m.timing_next = (k + 1) % Multilevel::kTimingRing;
@@ -646,6 +647,7 @@ hipEvent_t* next_timing_pair(Context& c) {
         hip_check(hipEventCreate(&m.timing_events[k][1]), "event");
     }
     m.timing_pending[k] = true;
+    m.timing_count[k] = count;
     return m.timing_events[k];
 }
 
@@ -710,25 +712,26 @@ void vcycle(Context& c, int l) {
         else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, L.r, L.ta);
         hip_check(hipGetLastError(), "mg_scale_kernel");
         xb = L.ta;
+        // the pre-sweeps after the first run back to back: one event pair
+        hipEvent_t* ev = (c.timing && l == 0 && o.pre_sweeps >= 2) ? next_timing_pair(c, o.pre_sweeps - 1) : nullptr;
+        if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = xb == L.ta ? L.tb : L.ta;
-            hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
-            if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
             if (vec) mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, xb, dst, nullptr);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
-            if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
             xb = dst;
         }
+        if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
     } else {
+        hipEvent_t* ev = (c.timing && l == 0 && o.pre_sweeps >= 2) ? next_timing_pair(c, o.pre_sweeps - 1) : nullptr;
+        if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = bufs[k & 1];
-            hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
-            if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
             if (k == 2) sweep(c, L, kFromR, L.r, nullptr, none, dst);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
-            if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
             xb = dst;
         }
+        if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
     }
     // fine-cell threads, two z planes per block
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
@@ -755,11 +758,13 @@ void vcycle(Context& c, int l) {
         cur = first_out;
         done_sweeps = 1;
     }
+    // the remaining post-sweeps run back to back: one event pair
+    const int post_timed = o.post_sweeps - done_sweeps;
+    hipEvent_t* ev = (c.timing && l == 0 && post_timed > 0) ? next_timing_pair(c, post_timed) : nullptr;
+    if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
     for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
         const bool last0 = l == 0 && k == o.post_sweeps;
-        hipEvent_t* ev = (c.timing && l == 0) ? next_timing_pair(c) : nullptr;
-        if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
         const int64_t blocks4 = int64_t(g4.x) * g4.y * g4.z;
         if (vec && last0 && m.dot_partials && blocks4 <= m.dot_capacity) {
             mg_sweep4_kernel<true><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, cur, dst, m.dot_partials);
@@ -770,9 +775,9 @@ void vcycle(Context& c, int l) {
             sweep(c, L, kBuffer, L.r, cur, none, dst);
         }
         hip_check(hipGetLastError(), "mg_sweep_kernel");
-        if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
         cur = dst;
     }
+    if (ev) hip_check(hipEventRecord(ev[1], c.stream), "event");
 }
 
 }  // namespace
