@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--batch-plumes", type=int, default=0,
                     help="also time the batched-plume workload with this many plumes (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--batch-lanes", type=int, default=4)
+    ap.add_argument("--batch-lanes", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)

@@ -27,7 +27,7 @@ struct SmokeSimulationBatch_t {
 
 namespace {
 
-constexpr int kDefaultLanes = 4;
+constexpr int kDefaultLanes = 16;
 constexpr int kMaxLanes = 16;
 
 // Longest processing time first: plumes by decreasing cells (stable), each to
