@@ -180,6 +180,11 @@ __global__ __launch_bounds__(256) void mg_sweep_kernel(LevelArgs a, const float*
     out[i] = xi + a.omega * res * inv;
 }
 
+__global__ void mg_copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // ---- float4 row kernels for large levels with nx % 4 == 0: a thread owns
 // four consecutive x cells of one row; the rows at y +- 1 and z +- 1 are
 // float4 loads, the cells at x - 1 and x + 4 scalar loads.  A neighbour
@@ -681,7 +686,10 @@ void vcycle(Context& c, int l) {
                 sweep(c, L, kBuffer, L.rd, cur, none, dst);
                 cur = dst;
             }
-            if (cur != L.z) hip_check(hipMemcpyAsync(L.z, cur, size_t(L.cells) * sizeof(float), hipMemcpyDeviceToDevice, c.stream), "coarse copy");
+            if (cur != L.z) {
+                mg_copy_kernel<<<eb, 256, 0, c.stream>>>(cur, L.z, cells);
+                hip_check(hipGetLastError(), "mg_copy_kernel");
+            }
         } else {
             sweeps_from_zero(c, L, std::max(1, o.coarse_sweeps), L.rd, L.z);
         }

@@ -149,6 +149,10 @@ __device__ __forceinline__ uint16_t to_bf16(float v) {
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
+__global__ void zero_sums_kernel(double* __restrict__ p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
 // per volume (blockIdx.y): sum of r^2
 __global__ __launch_bounds__(256) void sum_squares_kernel(const float* __restrict__ r, int64_t cells, double* __restrict__ sums) {
     const int b = blockIdx.y;
@@ -337,7 +341,7 @@ void backward(Net& net, int batch, const Dims& d, hipStream_t s) {
 }
 
 void prepare_input(Net& net, const float* r, const uint8_t* flags, int batch, int64_t cells, hipStream_t s) {
-    hip_check(hipMemsetAsync(net.sums, 0, size_t(3 * batch) * sizeof(double), s), "zero sums");
+    zero_sums_kernel<<<1, 64, 0, s>>>(net.sums, 3 * batch);
     const dim3 grid(grid_for(cells), batch);
     sum_squares_kernel<<<grid, 256, 0, s>>>(r, cells, net.sums);
     input_kernel<<<grid, 256, 0, s>>>(r, flags, cells, net.sums, net.T[kIn], net.rn);
