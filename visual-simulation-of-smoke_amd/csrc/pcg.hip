@@ -592,14 +592,18 @@ __global__ __launch_bounds__(kFinalizeThreads) void fpcg_beta_kernel(PcgArgs a, 
 
 // r = b - A x' with x' = x_old + e_add (either may be absent) evaluated in
 // FP64; optionally stores x' (FP64) and starts a fresh inner solve.
-__global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* __restrict__ b, const double* __restrict__ x_old,
+__global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
                                                        const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
                                                        const PcgScalars* __restrict__ S) {
-    const int nblocks = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    // 256 threads = 64 (x) x 4 (y) columns, kResidualPlanes z planes per block
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    // tiles of 64 (x) x 4 (y) columns by kResidualPlanes z planes, in the
+    // XCD-aware order (neighbouring tiles' rows are read through one L2)
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
+    const int nblocks = int(tiles.x * tiles.y * tiles.z);
+    const int bid = int(id);
+    const int x = int(tile.x) * 64 + (threadIdx.x & 63);
+    const int y = int(tile.y) * 4 + (threadIdx.x >> 6);
     double rr = 0.0, rz = 0.0, bb = 0.0;
     auto xv = [&](int cx, int cy, int cz) -> double {
         if (!resolve(a, cx, cy, cz)) return 0.0;
@@ -608,7 +612,7 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, const float* _
         if (e_add) s += double(e_add[i]);
         return s;
     };
-    const int zb = blockIdx.z * kResidualPlanes;
+    const int zb = int(tile.z) * kResidualPlanes;
     const int ze = min(zb + kResidualPlanes, a.n[2]);
     for (int z = zb; z < ze && x < a.n[0] && y < a.n[1]; ++z) {
         const int64_t i = lin(a, x, y, z);
@@ -866,7 +870,8 @@ void learned_iteration(Context& c) {
 void launch_residual(Context& c, const double* x_old, const float* e_add, double* x_new, bool bb, bool reset) {
     const PcgArgs a = make_args(c);
     const dim3 grid((c.grid.n[0] + 63) / 64, (c.grid.n[1] + 3) / 4, (c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
-    residual_kernel<<<grid, 256, 0, c.stream>>>(a, c.pressure_rhs, x_old, e_add, x_new, c.pcg_r, c.scalars);
+    residual_kernel<<<xcd_blocks(grid), 256, 0, c.stream>>>(a, make_uint3(grid.x, grid.y, grid.z), c.pressure_rhs, x_old, e_add, x_new,
+                                                             c.pcg_r, c.scalars);
     hip_check(hipGetLastError(), "residual_kernel");
     residual_finalize_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, int(grid.x * grid.y * grid.z), bb ? 1 : 0, reset ? 1 : 0);
     hip_check(hipGetLastError(), "residual_finalize_kernel");

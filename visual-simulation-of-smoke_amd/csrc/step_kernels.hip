@@ -42,14 +42,26 @@ __global__ void fill_u8_kernel(uint8_t* __restrict__ p, uint8_t value, int64_t n
 }
 
 // ------------------------------------------------------------ time step
+// max |f| over one array: float4 loads (the arrays are separate, 256-byte
+// aligned allocations), then the tail; the maximum does not depend on the order
+__device__ __forceinline__ float abs_max(const float* __restrict__ f, int64_t n, int64_t first, int64_t stride, float m) {
+    const int64_t n4 = n / 4;
+    const float4* __restrict__ f4 = reinterpret_cast<const float4*>(f);
+    for (int64_t i = first; i < n4; i += stride) {
+        const float4 q = f4[i];
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fmaxf(fabsf(q.z), fabsf(q.w))));
+    }
+    for (int64_t i = 4 * n4 + first; i < n; i += stride) m = fmaxf(m, fabsf(f[i]));
+    return m;
+}
+
 __global__ void max_speed_kernel(const float* __restrict__ u, int64_t nu, const float* __restrict__ v, int64_t nv, const float* __restrict__ w,
                                  int64_t nw, unsigned* __restrict__ max_bits) {
-    float m = 0.0f;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nu + nv + nw; i += stride) {
-        const float s = i < nu ? u[i] : (i < nu + nv ? v[i - nu] : w[i - nu - nv]);
-        m = fmaxf(m, fabsf(s));
-    }
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float m = abs_max(u, nu, first, stride, 0.0f);
+    m = abs_max(v, nv, first, stride, m);
+    m = abs_max(w, nw, first, stride, m);
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off));
     __shared__ float red[kBlock / 64];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
