@@ -1,9 +1,10 @@
 """Batched plumes (csrc/batch.hip) through the C ABI: every plume of a batch
 of uneven grids, stepped on several lanes, equals bitwise a context created
 alone from the same description and driven the same way -- in the reference's
-fixed-iteration mode (captured graphs enqueued from one thread), in the
-tolerance mode (one host thread per lane) with Jacobi PCG and with the
-multilevel preconditioner -- and a network attached on two lanes is refused."""
+fixed-iteration mode (the CG iterations of all plumes batched into one launch
+per pass and group of plumes), in the tolerance mode (one host thread per
+lane) with Jacobi PCG and with the multilevel preconditioner -- and a network
+attached on two lanes is refused."""
 import numpy as np
 import pytest
 
@@ -24,11 +25,15 @@ def _require_gpu():
 
 
 def _scenes():
-    """Five plumes of uneven sizes (at most 48^3 cells) from the plume-set rule."""
-    shapes = [(24, 24, 24), (40, 32, 24), (16, 16, 16), (48, 32, 32), (32, 24, 40)]
+    """Five plumes of uneven sizes (at most 48^3 cells).  In fixed-iteration
+    mode they form three groups of the batched solve: float4-row grids with
+    24 iterations, a scalar-row grid (x extent not a multiple of 4) and a
+    plume with 20 iterations."""
+    shapes = [(24, 24, 24), (40, 32, 24), (18, 16, 16), (48, 32, 32), (32, 24, 40)]
+    iterations = [24, 24, 24, 20, 24]
     out = []
     for i, (nx, ny, nz) in enumerate(shapes):
-        cfg = abi.make_config(nx, ny, nz, cell_size=1.0 / 32, dt=0.01, pressure_iterations=24, ambient=0.0, buoyancy_density=0.05,
+        cfg = abi.make_config(nx, ny, nz, cell_size=1.0 / 32, dt=0.01, pressure_iterations=iterations[i], ambient=0.0, buoyancy_density=0.05,
                               buoyancy_temperature=0.25, vorticity=1.0 + 0.5 * i, cubic=i % 2 == 0)
         src = abi.PlumeSource(nx / 2 + 0.3 * i, 0.08 * ny, nz / 2 - 0.2 * i, min(nx, nz) / 6, 1.0, 0.8 + 0.2 * i, 0.05, 30 + i)
         out.append((cfg, src))

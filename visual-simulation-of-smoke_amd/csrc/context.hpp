@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "device_common.hpp"
 
@@ -198,6 +199,17 @@ void enqueue_pressure_from_x64(Context& c);
 void enqueue_fixed_pcg(Context& c);                        // reference: K plain CG iterations from 0
 void run_tolerance_pcg(Context& c);                        // host-polled tolerance solve
 void enqueue_true_residual(Context& c);                    // rr_true, bb for the FP32 pressure
+
+// Batched fixed-iteration solves (pcg.hip, api.hip): contexts in
+// FIXED_ITERATIONS mode with the same stencil launch shapes and iteration
+// count run their K CG iterations as one launch per pass.  The table image
+// (fill_batched_table) goes to device memory of batched_table_bytes(n); the
+// step around the iterations is enqueue_fixed_step_before/after_iterations.
+size_t batched_table_bytes(size_t count);
+void fill_batched_table(const std::vector<Context*>& plumes, std::vector<unsigned char>& image);
+void enqueue_batched_iterations(const std::vector<Context*>& plumes, const void* table, hipStream_t s);
+void enqueue_fixed_step_before_iterations(Context& c);
+void enqueue_fixed_step_after_iterations(Context& c);
 void configure_pcg(Context& c);
 double measure_copy(uint64_t bytes, int reps, double* ms_per_copy);
 

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -75,6 +76,18 @@ struct PcgArgs {
     double* partials;
 };
 
+// One context of a batched fixed-iteration solve: its arguments with the
+// tiles of the pass-2 and pass-3 launch shapes, its vectors, and where its
+// blocks start in each batched launch.
+struct BatchedPlume {
+    PcgArgs a[2];
+    float* p;
+    float* e;
+    float* r;
+    int update_first, update_blocks;
+    int stencil_first[2], stencil_blocks[2];
+};
+
 // OUTFLOW faces of cell (x, y, z) (each adds 1 to the diagonal of a fluid row).
 __device__ __forceinline__ int outflow_faces(const PcgArgs& a, int x, int y, int z) {
     if (!a.any_outflow) return 0;
@@ -112,11 +125,10 @@ __device__ __forceinline__ bool resolve(const PcgArgs& a, int& x, int& y, int& z
 
 __device__ __forceinline__ int64_t lin(const PcgArgs& a, int x, int y, int z) { return (int64_t(z) * a.n[1] + y) * a.n[0] + x; }
 
-// Tile index of this block; consecutive tiles are kept on one XCD (blocks are
-// dealt round-robin over the 8 XCDs) so that neighbouring tiles share halo
-// rows through the same L2.
-__device__ __forceinline__ int tile_id(int nblocks) {
-    const int b = blockIdx.x;
+// Tile index of block b of a launch of nblocks; consecutive tiles are kept on
+// one XCD (blocks are dealt round-robin over the 8 XCDs) so that neighbouring
+// tiles share halo rows through the same L2.
+__device__ __forceinline__ int tile_id(int b, int nblocks) {
     if ((nblocks & 7) != 0) return b;
     return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
@@ -214,9 +226,9 @@ __device__ __forceinline__ float if_bit(unsigned w, int pos, float v) {
 // kept on one XCD (blocks are dealt round-robin over the 8 XCDs), so that
 // the neighbours a tile reads through the cache were just loaded by the
 // tiles beside it in the same L2.
-__device__ __forceinline__ void stencil_tile(const PcgArgs& a, int& bx, int& by, int& bz) {
+__device__ __forceinline__ void stencil_tile(const PcgArgs& a, int block, int& bx, int& by, int& bz) {
     const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
-    const unsigned b = unsigned(tile_id(nb));
+    const unsigned b = unsigned(tile_id(block, nb));
     const unsigned t = divide(b, a.tile_div[0]);
     const unsigned u = divide(t, a.tile_div[1]);
     bz = int(b - t * unsigned(a.tiles[2]));
@@ -227,14 +239,16 @@ __device__ __forceinline__ void stencil_tile(const PcgArgs& a, int& bx, int& by,
 // Pass 1 of an iteration: p <- z + beta p in place, z = r / diag (Jacobi)
 // or r; a fresh Krylov space (first) takes p = z.  Elementwise, 4 cells per
 // thread: reads r, p, flags, writes p (13 B/cell).
+// (the bodies of the pass kernels take their block index: the batched
+// launches over many contexts run them with each context's own numbering)
 template <bool JACOBI>
-__global__ __launch_bounds__(256) void pcg_update_p_kernel(PcgArgs a, const float* __restrict__ r, float* __restrict__ p) {
+__device__ __forceinline__ void update_p_body(const PcgArgs& a, const float* __restrict__ r, float* __restrict__ p, int block) {
     const PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
     const bool first = S->first != 0;
     const float beta = first ? 0.0f : float(S->beta);
     const int64_t cells = int64_t(a.n[0]) * a.n[1] * a.n[2];
-    const int64_t base = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    const int64_t base = (int64_t(block) * 256 + threadIdx.x) * 4;
     if (base >= cells) return;
     auto zval = [&](float rv, uint8_t f, int64_t i) {
         if constexpr (!JACOBI) return rv;
@@ -259,6 +273,11 @@ __global__ __launch_bounds__(256) void pcg_update_p_kernel(PcgArgs a, const floa
     } else {
         for (int64_t i = base; i < cells; ++i) p[i] = zval(r[i], a.flags[i], i) + beta * (first ? 0.0f : p[i]);
     }
+}
+
+template <bool JACOBI>
+__global__ __launch_bounds__(256) void pcg_update_p_kernel(PcgArgs a, const float* __restrict__ r, float* __restrict__ p) {
+    update_p_body<JACOBI>(a, r, p, blockIdx.x);
 }
 
 // Stencil passes on the stored direction p, VEC cells per thread along x
@@ -311,14 +330,14 @@ __device__ __forceinline__ void store_row(float* __restrict__ f, unsigned i, con
 }
 
 template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
-__global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r,
-                                                              const float* __restrict__ zold) {
+__device__ __forceinline__ void stencil_body(const PcgArgs& a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r,
+                                             const float* __restrict__ zold, int block) {
     PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
     constexpr int NWAVES = TXV * TY / 64;
     const int nblocks = a.tiles[0] * a.tiles[1] * a.tiles[2];
     int bx, by, bz;
-    stencil_tile(a, bx, by, bz);
+    stencil_tile(a, block, bx, by, bz);
     const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
     const int xv = (bx * TXV + int(threadIdx.x) % TXV) * VEC;
     const int y = by * TY + int(threadIdx.x) / TXV;
@@ -500,15 +519,21 @@ __global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const f
 
     if constexpr (MODE == 0) {
         double v[1] = {acc0};
-        store_block_partials<1, NWAVES>(v, a.partials, nblocks, blockIdx.x);
+        store_block_partials<1, NWAVES>(v, a.partials, nblocks, block);
     } else {
         double v[2] = {acc0, acc1};
-        store_block_partials<2, NWAVES>(v, a.partials, nblocks, blockIdx.x);
+        store_block_partials<2, NWAVES>(v, a.partials, nblocks, block);
     }
 }
 
+template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
+__global__ __launch_bounds__(TXV* TY) void pcg_stencil_kernel(PcgArgs a, const float* __restrict__ p, float* __restrict__ e, float* __restrict__ r,
+                                                              const float* __restrict__ zold) {
+    stencil_body<TXV, TY, ZR, VEC, MODE, JACOBI>(a, p, e, r, zold, blockIdx.x);
+}
+
 // alpha = r.z / p.Ap from the partials of pass 2.
-__global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_kernel(PcgArgs a, int nblocks) {
+__device__ __forceinline__ void alpha_body(const PcgArgs& a, int nblocks) {
     PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
     double tot[1];
@@ -518,7 +543,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_kernel(PcgArgs a, 
 
 // beta = r.z_new / r.z_old, iteration count and stop test from the partials
 // of pass 3.
-__global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, int nblocks) {
+__device__ __forceinline__ void beta_body(const PcgArgs& a, int nblocks) {
     PcgScalars* __restrict__ S = a.s;
     if (S->done) return;
     double tot[2];
@@ -533,6 +558,48 @@ __global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, i
         S->iteration = it;
         if (!S->fixed_mode && (tot[1] <= S->stop2 || it >= S->max_iterations)) S->done = 1;
     }
+}
+
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_kernel(PcgArgs a, int nblocks) { alpha_body(a, nblocks); }
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, int nblocks) { beta_body(a, nblocks); }
+
+// ---- batched fixed-iteration solves (batched plumes)
+// One launch per pass for many contexts that share the stencil launch shapes
+// and the iteration count.  A block finds its context in the table by the
+// context's first block in the launch and runs the pass body with the
+// context's own block numbering, so every context computes exactly what its
+// own launches would.
+template <int W>   // 0: update pass; 1, 2: stencil passes 2 and 3
+__device__ __forceinline__ int plume_of(const BatchedPlume* __restrict__ t, int count, int b) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int first = W == 0 ? t[mid].update_first : t[mid].stencil_first[W - 1];
+        if (first <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void pcg_update_p_batched_kernel(const BatchedPlume* __restrict__ t, int count) {
+    const BatchedPlume& q = t[plume_of<0>(t, count, int(blockIdx.x))];
+    update_p_body<false>(q.a[0], q.r, q.p, int(blockIdx.x) - q.update_first);
+}
+
+template <int TXV, int TY, int ZR, int VEC, int MODE>
+__global__ __launch_bounds__(TXV* TY) void pcg_stencil_batched_kernel(const BatchedPlume* __restrict__ t, int count) {
+    const BatchedPlume& q = t[plume_of<MODE + 1>(t, count, int(blockIdx.x))];
+    stencil_body<TXV, TY, ZR, VEC, MODE, false>(q.a[MODE], q.p, q.e, q.r, nullptr, int(blockIdx.x) - q.stencil_first[MODE]);
+}
+
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_batched_kernel(const BatchedPlume* __restrict__ t) {
+    const BatchedPlume& q = t[blockIdx.x];
+    alpha_body(q.a[0], q.stencil_blocks[0]);
+}
+
+__global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_batched_kernel(const BatchedPlume* __restrict__ t) {
+    const BatchedPlume& q = t[blockIdx.x];
+    beta_body(q.a[1], q.stencil_blocks[1]);
 }
 
 // ---- flexible PCG (learned preconditioner): z is a stored vector
@@ -1207,6 +1274,85 @@ double measure_copy(uint64_t bytes, int reps, double* ms_per_copy) {
     const double per = double(ms) / std::max(1, reps);
     if (ms_per_copy) *ms_per_copy = per;
     return per > 0 ? 2.0 * double(n4) * 16.0 / (per * 1e-3) / 1e9 : 0.0;
+}
+
+size_t batched_table_bytes(size_t count) { return count * sizeof(BatchedPlume); }
+
+void fill_batched_table(const std::vector<Context*>& plumes, std::vector<unsigned char>& image) {
+    image.assign(plumes.size() * sizeof(BatchedPlume), 0);
+    int update = 0, stencil[2] = {0, 0};
+    for (size_t i = 0; i < plumes.size(); ++i) {
+        Context& c = *plumes[i];
+        BatchedPlume q{};
+        for (int m = 0; m < 2; ++m) {
+            const StencilShape& sh = kStencilShapes[c.pcg_shape[m]];
+            PcgArgs a = make_args(c);
+            a.tiles[0] = (c.grid.n[0] + sh.txv * sh.vec - 1) / (sh.txv * sh.vec);
+            a.tiles[1] = (c.grid.n[1] + sh.ty - 1) / sh.ty;
+            a.tiles[2] = (c.grid.n[2] + sh.zr - 1) / sh.zr;
+            a.tile_div[0] = make_divisor(unsigned(a.tiles[2]));
+            a.tile_div[1] = make_divisor(unsigned(a.tiles[1]));
+            q.a[m] = a;
+            q.stencil_blocks[m] = a.tiles[0] * a.tiles[1] * a.tiles[2];
+            q.stencil_first[m] = stencil[m];
+            stencil[m] += q.stencil_blocks[m];
+        }
+        q.p = c.pcg_p;
+        q.e = c.pressure;   // fixed-iteration mode accumulates the solution in the pressure
+        q.r = c.pcg_r;
+        q.update_blocks = int((c.grid.cells + 1023) / 1024);
+        q.update_first = update;
+        update += q.update_blocks;
+        std::memcpy(image.data() + i * sizeof(BatchedPlume), &q, sizeof(q));
+    }
+}
+
+template <int TXV, int TY, int ZR, int VEC, int MODE>
+void stencil_batched(unsigned blocks, const BatchedPlume* t, int count, hipStream_t s) {
+    pcg_stencil_batched_kernel<TXV, TY, ZR, VEC, MODE><<<blocks, TXV * TY, 0, s>>>(t, count);
+}
+
+template <int MODE>
+void launch_stencils_batched(int shape, unsigned blocks, const BatchedPlume* t, int count, hipStream_t s) {
+    switch (shape) {
+        case 0: stencil_batched<64, 4, 2, 4, MODE>(blocks, t, count, s); break;
+        case 1: stencil_batched<64, 8, 2, 4, MODE>(blocks, t, count, s); break;
+        case 2: stencil_batched<32, 8, 2, 4, MODE>(blocks, t, count, s); break;
+        case 3: stencil_batched<64, 4, 4, 4, MODE>(blocks, t, count, s); break;
+        case 4: stencil_batched<16, 16, 2, 4, MODE>(blocks, t, count, s); break;
+        case 5: stencil_batched<16, 8, 2, 4, MODE>(blocks, t, count, s); break;
+        case 6: stencil_batched<64, 4, 2, 1, MODE>(blocks, t, count, s); break;
+        case 7: stencil_batched<64, 8, 2, 1, MODE>(blocks, t, count, s); break;
+        case 8: stencil_batched<32, 8, 2, 1, MODE>(blocks, t, count, s); break;
+        default: throw HipError("invalid stencil shape");
+    }
+}
+
+void enqueue_batched_iterations(const std::vector<Context*>& plumes, const void* table, hipStream_t s) {
+    if (plumes.empty()) return;
+    const Context& c0 = *plumes.front();
+    const int count = int(plumes.size());
+    int64_t update = 0, stencil[2] = {0, 0};
+    for (const Context* c : plumes) {
+        if (c->options.pressure_mode != SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS || c->pcg_shape[0] != c0.pcg_shape[0] ||
+            c->pcg_shape[1] != c0.pcg_shape[1] || c->config.pressure_iterations != c0.config.pressure_iterations)
+            throw HipError("batched solve: contexts differ in mode, launch shapes or iterations");
+        update += (c->grid.cells + 1023) / 1024;
+        for (int m = 0; m < 2; ++m) {
+            const StencilShape& sh = kStencilShapes[c->pcg_shape[m]];
+            stencil[m] += int64_t((c->grid.n[0] + sh.txv * sh.vec - 1) / (sh.txv * sh.vec)) * ((c->grid.n[1] + sh.ty - 1) / sh.ty) *
+                          ((c->grid.n[2] + sh.zr - 1) / sh.zr);
+        }
+    }
+    const BatchedPlume* t = static_cast<const BatchedPlume*>(table);
+    for (int k = 0; k < c0.config.pressure_iterations; ++k) {
+        pcg_update_p_batched_kernel<<<unsigned(update), 256, 0, s>>>(t, count);
+        launch_stencils_batched<0>(c0.pcg_shape[0], unsigned(stencil[0]), t, count, s);
+        pcg_alpha_batched_kernel<<<unsigned(count), kFinalizeThreads, 0, s>>>(t);
+        launch_stencils_batched<1>(c0.pcg_shape[1], unsigned(stencil[1]), t, count, s);
+        pcg_beta_batched_kernel<<<unsigned(count), kFinalizeThreads, 0, s>>>(t);
+    }
+    hip_check(hipGetLastError(), "batched pcg iterations");
 }
 
 }  // namespace smoke
