@@ -384,11 +384,14 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_batch_context(Sm
                                                                             int32_t* out_lane);
 /* One step of every plume; with `sources` (one per plume, or NULL) each
    plume's plume_sources(sources[i], step) is enqueued just before its step.
-   Plumes in FIXED_ITERATIONS mode enqueue one captured graph each, from the
-   calling thread, interleaved over the lanes, and the call returns without
-   waiting; when any plume solves to a tolerance (its step waits on its
-   solve), one host thread per lane drives that lane's plumes and the call
-   returns when every lane has enqueued its last step.  A learned network
+   When every plume is in FIXED_ITERATIONS mode (kernel timing off), the
+   lanes run each plume's step up to its CG iterations, a solve stream of the
+   batch runs the iterations of all plumes as one launch per pass for each
+   group of plumes with the same launch shapes and iteration count, and the
+   lanes finish the steps; the call returns without waiting.  When any plume
+   solves to a tolerance (its step waits on its solve), one host thread per
+   lane drives that lane's plumes and the call returns when every lane has
+   enqueued its last step.  A learned network
    attached to plumes on two lanes would run on two streams at once: such a
    batch fails here (attach one network per lane). */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_step_batch(SmokeSimulationBatch batch, const SmokeSimulationPlumeSource* sources,
