@@ -206,7 +206,7 @@ def batch_levels(cfg: abi.Config) -> int:
     return levels
 
 
-def run_batch(count: int, steps: int, warmup: int, lanes_max: int = 4):
+def run_batch(count: int, steps: int, warmup: int, lanes_max: int = 4, modes=("fixed", "tolerance")):
     """The batched-plume workload (batch_plumes.plume_set(count, seed=3)):
     steps/s of the whole batch on `lanes_max` lanes and on 1, in the reference's
     fixed-iteration mode (64 plain CG iterations, one captured graph per
@@ -215,7 +215,7 @@ def run_batch(count: int, steps: int, warmup: int, lanes_max: int = 4):
     plumes = bp.plume_set(count, seed=3)
     cells = bp.total_cells(plumes)
     out = {"plumes": count, "cells": cells, "grids": sorted({f"{c.nx}x{c.ny}x{c.nz}" for c, _ in plumes}), "steps": steps}
-    for mode in ("fixed", "tolerance"):
+    for mode in modes:
         for lanes in (lanes_max, 1):
             with abi.SmokeBatch([c for c, _ in plumes], lanes=lanes) as batch:
                 for (cfg, _), p in zip(plumes, batch.plumes):
@@ -269,6 +269,7 @@ def main():
                     help="also time the batched-plume workload with this many plumes (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-lanes", type=int, default=16)
+    ap.add_argument("--batch-modes", default="fixed,tolerance", help="comma-separated, from fixed, tolerance")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -355,7 +356,8 @@ def main():
         "roofline": roofline, "cpu_baseline": None,
     }
     if args.batch_plumes > 0:
-        out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2, args.batch_lanes)
+        modes = tuple(m.strip() for m in args.batch_modes.split(",") if m.strip())
+        out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2, args.batch_lanes, modes)
     if not args.no_cpu_baseline and "jacobi" in results:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, results["jacobi"]["pcg_iterations_per_step"], threads)
