@@ -164,8 +164,12 @@ def test_energy_loss_training_step_matches_float64_network(lib):
                 if norm == 0:
                     assert np.all(g == 0)
                 else:
-                    # 5 %: bfloat16 tie flips between the two summation orders
-                    assert np.linalg.norm(g - rg) <= 5e-2 * norm, (li, size, np.linalg.norm(g - rg) / norm)
+                    # 8 %: bfloat16 tie flips between the two summation
+                    # orders.  The worst layer is the 4^3 one (64 voxels per
+                    # volume): 3.1 % with the K steps of each output tile
+                    # split over 4 waves, 5.0 % with them split over 16
+                    # (another FP32 order of the same products)
+                    assert np.linalg.norm(g - rg) <= 8e-2 * norm, (li, size, np.linalg.norm(g - rg) / norm)
         first = loss.value
         for _ in range(30):
             assert L.smoke_nn_preconditioner_train_step(net, d_r, d_f, batch, cd, 1e-3, C.byref(loss), None, None) == 0

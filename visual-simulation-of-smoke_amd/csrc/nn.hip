@@ -72,7 +72,7 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, sh
 // The next K step's loads are issued before the current step's matrix-core
 // work.  Epilogue: + bias, + add, relu, * [mask > 0].
 template <int NT, int SPLIT>
-__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+__global__ __launch_bounds__(SPLIT == 1 ? 256 : 64 * SPLIT) void conv_kernel(ConvArgs a, const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                    const float* __restrict__ bias, const uint16_t* __restrict__ add,
                                                    const uint16_t* __restrict__ mask, int relu, uint16_t* __restrict__ y,
                                                    float* __restrict__ yf, int f32_channels) {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, const uint16_t* _
     const int64_t wave = SPLIT == 1 ? int64_t(blockIdx.x) * 4 + wib : int64_t(blockIdx.x);
     const int xt = (a.out[0] + 15) / 16;
     const int64_t tiles = int64_t(a.batch) * a.out[2] * a.out[1] * xt;
-    if (wave >= tiles) return;   // uniform per block when SPLIT = 4
+    if (wave >= tiles) return;   // uniform per block when SPLIT > 1
     int64_t t = wave;
     const int xtile = int(t % xt);
     t /= xt;
@@ -135,16 +135,21 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, const uint16_t* _
         }
     }
     if constexpr (SPLIT > 1) {
-        // waves 1..3 hand their partial sums to wave 0 (fixed order: deterministic)
-        __shared__ float4v part[3][NT][64];
+        // waves 1..SPLIT-1 hand their partial sums to wave 0, which adds them
+        // in wave order (deterministic)
+        __shared__ float4v part[SPLIT - 1][NT][64];
         if (wib > 0) {
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) part[wib - 1][nt][lane] = acc[nt];
         }
         __syncthreads();
         if (wib > 0) return;
+        // wave order outermost, not unrolled: NT partials live at a time
+#pragma unroll 1
+        for (int q = 0; q < SPLIT - 1; ++q) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = ((acc[nt] + part[0][nt][lane]) + part[1][nt][lane]) + part[2][nt][lane];
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = acc[nt] + part[q][nt][lane];
+        }
     }
     // lane holds D[4 g + i][m]: voxel xtile * 16 + 4 g + i, channel nt * 16 + m
 #pragma unroll
@@ -292,18 +297,24 @@ ConvArgs dual(const ConvArgs& a) {
     return d;
 }
 
-// Volumes with fewer tiles than this split each tile's K steps over the four
-// waves of a block.
+// Volumes with fewer tiles than kSplitTiles deal each tile's K steps over
+// the waves of a block: 4 waves, 8 below kSplit8Tiles, 16 below
+// kSplit16Tiles.  The coarse levels of the network (4^3 to 16^3: 16 to 256
+// tiles) would otherwise leave most CUs idle while each wave works through
+// its chain of dependent gathers.
 constexpr int64_t kSplitTiles = 8192;
+constexpr int64_t kSplit8Tiles = 512;
+constexpr int64_t kSplit16Tiles = 128;
 
 template <int SPLIT>
 void launch_conv_split(const ConvArgs& a, unsigned blocks, const uint16_t* x, const uint16_t* w, const float* bias, const uint16_t* add,
                        const uint16_t* mask, int relu, uint16_t* y, float* yf, int f32_channels, hipStream_t s) {
+    constexpr unsigned threads = SPLIT == 1 ? 256 : 64 * SPLIT;
     switch (a.cout / 16) {
-        case 1: conv_kernel<1, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 2: conv_kernel<2, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 3: conv_kernel<3, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 4: conv_kernel<4, SPLIT><<<blocks, 256, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 1: conv_kernel<1, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 2: conv_kernel<2, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 3: conv_kernel<3, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
+        case 4: conv_kernel<4, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
         default: throw HipError("conv: cout must be 16..64");
     }
 }
@@ -311,7 +322,9 @@ void launch_conv_split(const ConvArgs& a, unsigned blocks, const uint16_t* x, co
 void launch_conv(const ConvArgs& a, const uint16_t* x, const uint16_t* w, const float* bias, const uint16_t* add, const uint16_t* mask, int relu,
                  uint16_t* y, float* yf, int f32_channels, hipStream_t s) {
     const int64_t tiles = int64_t(a.batch) * a.out[2] * a.out[1] * ((a.out[0] + 15) / 16);
-    if (tiles < kSplitTiles) launch_conv_split<4>(a, unsigned(tiles), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
+    if (tiles < kSplit16Tiles) launch_conv_split<16>(a, unsigned(tiles), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
+    else if (tiles < kSplit8Tiles) launch_conv_split<8>(a, unsigned(tiles), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
+    else if (tiles < kSplitTiles) launch_conv_split<4>(a, unsigned(tiles), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
     else launch_conv_split<1>(a, unsigned((tiles + 3) / 4), x, w, bias, add, mask, relu, y, yf, f32_channels, s);
     hip_check(hipGetLastError(), "conv_kernel");
 }
