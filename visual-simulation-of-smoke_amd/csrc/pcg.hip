@@ -714,6 +714,110 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, uint3 tiles, c
     store_block_partials<3, 4>(v, a.partials, nblocks, bid);
 }
 
+// The same pass with float4 rows (nx % 4 == 0): a thread owns 4 consecutive
+// x cells of a row and marches kResidualPlanes z planes, keeping the rows at
+// z - 1, z and z + 1 of x' in registers; the rows at y +- 1 are vector loads,
+// the cells at x - 1 and x + 4 scalar loads.  Every cell subtracts its
+// neighbours in the order of residual_kernel, so r and x' are the same; only
+// the grouping of the FP64 partial sums differs.
+__global__ __launch_bounds__(256) void residual4_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
+                                                        const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
+                                                        const PcgScalars* __restrict__ S) {
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;
+    const int nblocks = int(tiles.x * tiles.y * tiles.z);
+    const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+    const int x0 = (int(tile.x) * 64 + (threadIdx.x & 63)) * 4;
+    const int y = int(tile.y) * 4 + (threadIdx.x >> 6);
+    double rr = 0.0, rz = 0.0, bb = 0.0;
+    // x' = x_old + e_add of the 4 cells of row (yy, zz) (an in-domain row)
+    auto row = [&](int yy, int zz, double (&v)[4]) {
+        const int64_t i = lin(a, x0, yy, zz);
+        if (x_old) {
+            const double2 p = *reinterpret_cast<const double2*>(x_old + i);
+            const double2 q = *reinterpret_cast<const double2*>(x_old + i + 2);
+            v[0] = p.x, v[1] = p.y, v[2] = q.x, v[3] = q.y;
+        } else {
+            v[0] = v[1] = v[2] = v[3] = 0.0;
+        }
+        if (e_add) {
+            const float4 e = *reinterpret_cast<const float4*>(e_add + i);
+            v[0] += double(e.x), v[1] += double(e.y), v[2] += double(e.z), v[3] += double(e.w);
+        }
+    };
+    auto cell = [&](int xx, int yy, int zz) -> double {
+        const int64_t i = lin(a, xx, yy, zz);
+        double s = x_old ? x_old[i] : 0.0;
+        if (e_add) s += double(e_add[i]);
+        return s;
+    };
+    const int zb = int(tile.z) * kResidualPlanes;
+    const int ze = min(zb + kResidualPlanes, nz);
+    if (x0 < nx && y < ny && zb < ze) {
+        // rows of a neighbour outside a closed axis are some existing row:
+        // no flag bit selects them
+        const int ym = neighbour_row(y - 1, ny, a.periodic[1]), yp = neighbour_row(y + 1, ny, a.periodic[1]);
+        const int xm = neighbour_row(x0 - 1, nx, a.periodic[0]), xp = neighbour_row(x0 + 4, nx, a.periodic[0]);
+        double zm[4], c[4], zp[4];
+        row(y, neighbour_row(zb - 1, nz, a.periodic[2]), zm);
+        row(y, zb, c);
+        for (int z = zb; z < ze; ++z) {
+            row(y, neighbour_row(z + 1, nz, a.periodic[2]), zp);
+            double ym4[4], yp4[4];
+            row(ym, z, ym4);
+            row(yp, z, yp4);
+            const double left = cell(xm, y, z), right = cell(xp, y, z);
+            const int64_t i = lin(a, x0, y, z);
+            const uchar4 f4 = *reinterpret_cast<const uchar4*>(a.flags + i);
+            const float4 b4 = *reinterpret_cast<const float4*>(b + i);
+            const uint8_t fs[4] = {f4.x, f4.y, f4.z, f4.w};
+            const float bs[4] = {b4.x, b4.y, b4.z, b4.w};
+            float rs[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint8_t f = fs[k];
+                const double xc = c[k];
+                double ax;
+                float d = 1.0f;
+                if (f & kIdentity) {
+                    ax = xc;
+                } else {
+                    d = diag_of(a, f, x0 + k, y, z);
+                    const unsigned m = unsigned(f) & ~a.skip;
+                    ax = double(d) * xc;
+                    if (m & 1u) ax -= k == 0 ? left : c[k - 1];
+                    if (m & 2u) ax -= k == 3 ? right : c[k + 1];
+                    if (m & 4u) ax -= ym4[k];
+                    if (m & 8u) ax -= yp4[k];
+                    if (m & 16u) ax -= zm[k];
+                    if (m & 32u) ax -= zp[k];
+                }
+                const double bv = double(bs[k]);
+                const double r64 = bv - ax;
+                const float rf = float(r64);
+                rs[k] = rf;
+                const float zf = S->jacobi ? rf / d : rf;
+                rr += r64 * r64;
+                rz += double(rf) * double(zf);
+                bb += bv * bv;
+            }
+            *reinterpret_cast<float4*>(r + i) = make_float4(rs[0], rs[1], rs[2], rs[3]);
+            if (x_new) {
+                *reinterpret_cast<double2*>(x_new + i) = make_double2(c[0], c[1]);
+                *reinterpret_cast<double2*>(x_new + i + 2) = make_double2(c[2], c[3]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                zm[k] = c[k];
+                c[k] = zp[k];
+            }
+        }
+    }
+    double v[3] = {rr, rz, bb};
+    store_block_partials<3, 4>(v, a.partials, nblocks, int(id));
+}
+
 // Totals of the residual kernel: true ||r||^2, r.z, ||b||^2; starts a fresh
 // inner solve (p = z at the next iteration) with its stop threshold.
 __global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(PcgArgs a, int nblocks, int compute_bb, int reset_iterations) {
@@ -936,9 +1040,15 @@ void learned_iteration(Context& c) {
 
 void launch_residual(Context& c, const double* x_old, const float* e_add, double* x_new, bool bb, bool reset) {
     const PcgArgs a = make_args(c);
-    const dim3 grid((c.grid.n[0] + 63) / 64, (c.grid.n[1] + 3) / 4, (c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
-    residual_kernel<<<xcd_blocks(grid), 256, 0, c.stream>>>(a, make_uint3(grid.x, grid.y, grid.z), c.pressure_rhs, x_old, e_add, x_new,
-                                                             c.pcg_r, c.scalars);
+    const bool rows4 = c.grid.n[0] % 4 == 0;
+    const dim3 grid((c.grid.n[0] + (rows4 ? 255 : 63)) / (rows4 ? 256 : 64), (c.grid.n[1] + 3) / 4,
+                    (c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
+    if (rows4)
+        residual4_kernel<<<xcd_blocks(grid), 256, 0, c.stream>>>(a, make_uint3(grid.x, grid.y, grid.z), c.pressure_rhs, x_old, e_add, x_new,
+                                                                  c.pcg_r, c.scalars);
+    else
+        residual_kernel<<<xcd_blocks(grid), 256, 0, c.stream>>>(a, make_uint3(grid.x, grid.y, grid.z), c.pressure_rhs, x_old, e_add, x_new,
+                                                                 c.pcg_r, c.scalars);
     hip_check(hipGetLastError(), "residual_kernel");
     residual_finalize_kernel<<<1, kFinalizeThreads, 0, c.stream>>>(a, int(grid.x * grid.y * grid.z), bb ? 1 : 0, reset ? 1 : 0);
     hip_check(hipGetLastError(), "residual_finalize_kernel");
