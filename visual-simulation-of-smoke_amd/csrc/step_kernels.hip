@@ -524,6 +524,30 @@ __device__ __forceinline__ int box_base(const LdsBox& bx, int x, int y, int z) {
     return ((z - bx.o[2]) * bx.d[1] + (y - bx.o[1])) * bx.d[0] + (x - bx.o[0]);
 }
 
+// True when every index of the box lies in [0, top[a]] on each axis a (n - 1
+// for cells, n on a face component's own axis): its entries are then the
+// array's values themselves and the ghost rules never apply.
+__device__ __forceinline__ bool box_inside(const LdsBox& bx, int t0, int t1, int t2) {
+    return bx.o[0] >= 0 && bx.o[1] >= 0 && bx.o[2] >= 0 && bx.o[0] + bx.d[0] - 1 <= t0 && bx.o[1] + bx.d[1] - 1 <= t1 &&
+           bx.o[2] + bx.d[2] - 1 <= t2;
+}
+
+// Staging of a box that lies inside an array of row pitch d0 and plane pitch
+// d0 * d1 (32-bit indices: the caller checks that the array has fewer than
+// 2^31 entries).
+__device__ __forceinline__ void stage_inside(float* lds, const LdsBox& bx, const float* __restrict__ f, int d0, int d1, int tid, int nthreads) {
+    const int n = box_size(bx);
+    for (int q = tid; q < n; q += nthreads) {
+        const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
+        lds[q] = f[(k * d1 + j) * d0 + i];
+    }
+}
+
+// arrays of the grid (faces included) are indexable in 32 bits
+__device__ __forceinline__ bool index32(const Grid& g) {
+    return (int64_t(g.n[0]) + 1) * (int64_t(g.n[1]) + 1) * (int64_t(g.n[2]) + 1) < (int64_t(1) << 31);
+}
+
 // sample_linear on staged values; the stencil must lie in the box
 __device__ __forceinline__ float staged_linear(const float* lds, const LdsBox& bx, const Stencil& s) {
     const int base = box_base(bx, s.b[0], s.b[1], s.b[2]);
@@ -674,26 +698,47 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
     const LdsBox bv{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 3, kSaZ + 2}};
     const LdsBox bw{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 3}};
     const LdsBox bo{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 2}};
-    // staging: every staged entry is what the global samplers' taps return
-    for (int q = tid; q < kS; q += kSaX * kSaY) {
-        const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
-        sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
-        sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
-    }
-    auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
-        const int n = box_size(bx);
-        for (int q = tid; q < n; q += kSaX * kSaY) {
-            const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
-            lds[q] = load_face(g, f, comp, i, j, k);
+    // staging: every staged entry is what the global samplers' taps return;
+    // a tile whose boxes lie inside the domain (most of them) stages the
+    // arrays' values directly
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const bool inside = index32(g) && box_inside(bs, n0 - 1, n1 - 1, n2 - 1) && box_inside(bu, n0, n1 - 1, n2 - 1) &&
+                        box_inside(bv, n0 - 1, n1, n2 - 1) && box_inside(bw, n0 - 1, n1 - 1, n2);
+    if (inside) {
+        for (int q = tid; q < kS; q += kSaX * kSaY) {
+            const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
+            const int c = (k * n1 + j) * n0 + i;
+            sT[q] = Ttmp[c];
+            sR[q] = rtmp[c];
         }
-    };
-    stage_face(0, u, su, bu);
-    stage_face(1, v, sv, bv);
-    stage_face(2, w, sw, bw);
-    for (int q = tid; q < kO; q += kSaX * kSaY) {
-        const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
-        // entries outside the domain are never read (the hit test returns first)
-        so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+        stage_inside(su, bu, u, n0 + 1, n1, tid, kSaX * kSaY);
+        stage_inside(sv, bv, v, n0, n1 + 1, tid, kSaX * kSaY);
+        stage_inside(sw, bw, w, n0, n1, tid, kSaX * kSaY);
+        for (int q = tid; q < kO; q += kSaX * kSaY) {   // bo lies inside bs
+            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+            so[q] = occ[(k * n1 + j) * n0 + i];
+        }
+    } else {
+        for (int q = tid; q < kS; q += kSaX * kSaY) {
+            const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
+            sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
+            sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
+        }
+        auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
+            const int n = box_size(bx);
+            for (int q = tid; q < n; q += kSaX * kSaY) {
+                const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
+                lds[q] = load_face(g, f, comp, i, j, k);
+            }
+        };
+        stage_face(0, u, su, bu);
+        stage_face(1, v, sv, bv);
+        stage_face(2, w, sw, bw);
+        for (int q = tid; q < kO; q += kSaX * kSaY) {
+            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+            // entries outside the domain are never read (the hit test returns first)
+            so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+        }
     }
     __syncthreads();
     const int x = X0 + tid % kSaX, y = Y0 + tid / kSaX;
@@ -805,20 +850,37 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, 
     const float* fc = C == 0 ? u : (C == 1 ? v : w);
     const float* f0 = D0 == 0 ? u : v;
     const float* f1 = D1 == 1 ? v : w;
-    auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
-        const int n = box_size(bx);
-        for (int q = tid; q < n; q += kSaX * kSaY) {
-            const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
-            lds[q] = load_face(g, f, comp, i, j, k);
-        }
+    // a tile whose boxes lie inside the arrays (most of them) stages the
+    // arrays' values directly; the others go through the ghost rules
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    auto inside_for = [&](int comp, const LdsBox& bx) {
+        return box_inside(bx, n0 - 1 + (comp == 0), n1 - 1 + (comp == 1), n2 - 1 + (comp == 2));
     };
-    stage_face(C, fc, sf, bf);
-    stage_face(D0, f0, sg0, bg);
-    stage_face(D1, f1, sg1, bg);
-    for (int q = tid; q < kO; q += kSaX * kSaY) {
-        const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
-        // entries outside the domain are never read (both tests check the domain first)
-        so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+    const bool inside = index32(g) && inside_for(C, bf) && inside_for(D0, bg) && inside_for(D1, bg) && box_inside(bo, n0 - 1, n1 - 1, n2 - 1);
+    if (inside) {
+        stage_inside(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), tid, kSaX * kSaY);
+        stage_inside(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), tid, kSaX * kSaY);
+        stage_inside(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), tid, kSaX * kSaY);
+        for (int q = tid; q < kO; q += kSaX * kSaY) {
+            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+            so[q] = occ[(k * n1 + j) * n0 + i];
+        }
+    } else {
+        auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
+            const int n = box_size(bx);
+            for (int q = tid; q < n; q += kSaX * kSaY) {
+                const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
+                lds[q] = load_face(g, f, comp, i, j, k);
+            }
+        };
+        stage_face(C, fc, sf, bf);
+        stage_face(D0, f0, sg0, bg);
+        stage_face(D1, f1, sg1, bg);
+        for (int q = tid; q < kO; q += kSaX * kSaY) {
+            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
+            // entries outside the domain are never read (both tests check the domain first)
+            so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
+        }
     }
     __syncthreads();
     const int i = X0 + tid % kSaX, j = Y0 + tid / kSaX;
