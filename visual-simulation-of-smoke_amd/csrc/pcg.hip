@@ -1229,26 +1229,36 @@ void run_tolerance_pcg(Context& c) {
     // PCG, convergence checked after every iteration (an application of the
     // preconditioner costs more than the host round trip).  Returns false when the residual fell
     // by less than 10 % over 5 iterations or turned non-finite.
-    // Two iterations stay in flight: iteration k's scalars are copied to a
-    // pinned slot and signalled by an event, and the host judges iteration k
-    // while iteration k + 1 runs.  An iteration enqueued after the solve
-    // finished is harmless: its update kernels skip on the done flag and its
-    // preconditioner application only rewrites z.
+    // Up to two iterations stay in flight: iteration k's scalars are copied
+    // to a pinned slot and signalled by an event, and the host judges
+    // iteration k while iteration k + 1 runs.  An iteration enqueued after
+    // the solve finished changes nothing (its update kernels skip on the done
+    // flag and its preconditioner application only rewrites z) but costs a
+    // whole iteration of device time, so when the last residual ratio
+    // predicts that the iteration in flight ends the solve, the next one is
+    // held back until that iteration has been judged.  A wrong guess costs
+    // one host round trip with the device idle.
     auto learned_inner = [&]() -> bool {
         learned_apply(c);
         learned_rz(c, true);
         std::vector<double> rr{host.rr};
-        auto enqueue = [&](int s) {
+        int pending[2] = {0, 0}, in_flight = 0, next_slot = 0;
+        auto enqueue = [&]() {
+            const int s = next_slot;
+            next_slot ^= 1;
             learned_iteration(c);
             hip_check(hipMemcpyAsync(&c.host_scalars[s], c.scalars, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream), "scalars copy");
             hip_check(hipEventRecord(c.status_event[s], c.stream), "status event");
             learned_apply(c);
             learned_rz(c, false);
+            pending[in_flight++] = s;
         };
-        enqueue(0);
-        enqueue(1);
-        int slot = 0;
+        enqueue();
+        enqueue();
         for (;;) {
+            const int slot = pending[0];
+            pending[0] = pending[1];
+            --in_flight;
             hip_check(hipEventSynchronize(c.status_event[slot]), "status wait");
             host = c.host_scalars[slot];
             ++learned_iterations;
@@ -1269,8 +1279,14 @@ void run_tolerance_pcg(Context& c) {
                 const double before = *std::min_element(rr.begin(), rr.end() - 10);
                 if (recent > 0.81 * before) return false;
             }
-            enqueue(slot);
-            slot ^= 1;
+            // iterations to have in flight after iteration k: just k + 1 when
+            // it is predicted to reach the stop test (r.r falling by the last
+            // ratio again; ratios vary, so within a factor 4 of the
+            // threshold) or the iteration cap
+            const double ratio = rr[k - 1] > 0.0 ? rr[k] / rr[k - 1] : 1.0;
+            const bool next_ends = (ratio < 1.0 && rr[k] * ratio <= 4.0 * host.stop2) || host.iteration + 1 >= host.max_iterations;
+            const int target = next_ends ? 1 : 2;
+            while (in_flight < target) enqueue();
         }
     };
     while (!host.done) {
