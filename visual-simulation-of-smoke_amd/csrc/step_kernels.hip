@@ -88,6 +88,17 @@ __global__ void time_step_kernel(unsigned* __restrict__ max_bits, float* __restr
     *dt = fminf(dt_max, cand);
 }
 
+// arrays of the grid (faces included) are indexable in 32 bits
+__device__ __forceinline__ bool index32(const Grid& g) {
+    return (int64_t(g.n[0]) + 1) * (int64_t(g.n[1]) + 1) * (int64_t(g.n[2]) + 1) < (int64_t(1) << 31);
+}
+
+// distance between neighbouring cells along axis C in the cell arrays
+template <int C>
+__device__ __forceinline__ int cell_stride(const Grid& g) {
+    return C == 0 ? 1 : (C == 1 ? g.n[0] : g.n[0] * g.n[1]);
+}
+
 // ------------------------------------------------------------ forces (stage 0)
 // B:2001-2003: solid temperature, then omega and |omega| from the
 // cell-centred velocity with the flow ghost rules (B:870-900).
@@ -115,7 +126,9 @@ __global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const floa
     ox[i] = wx;
     oy[i] = wy;
     oz[i] = wz;
-    om[i] = sqrtf(wx * wx + wy * wy + wz * wz);
+    // the sum of squares with its fused multiply-adds written out, so that
+    // the rounding does not depend on how the compiler contracts it
+    om[i] = sqrtf(fmaf(wz, wz, fmaf(wx, wx, wy * wy)));
 }
 
 // B:902-948: user force, buoyancy (+y) and vorticity confinement in fluid cells.
@@ -162,6 +175,19 @@ __global__ void face_force_kernel(Grid g, float* __restrict__ f, const float* __
     if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
     const int c[3] = {i, j, k};
     if (boundary_face(g, C, c[C])) return;
+    if (c[C] > 0 && c[C] < g.n[C] && index32(g)) {
+        // an inner face (0 < index < n): both neighbour cells are domain
+        // cells; when both are fluid the face gets the mean of their forces
+        const int ir = (k * g.n[1] + j) * g.n[0] + i, il = ir - cell_stride<C>(g);
+        if (occ[il] == 0 && occ[ir] == 0) {
+            const int fi32 = (k * (g.n[1] + (C == 1)) + j) * (g.n[0] + (C == 0)) + i;
+            float sum = 0.0f;
+            sum += force[il];
+            sum += force[ir];
+            f[fi32] += *dt_ptr * (sum / 2.0f);
+            return;
+        }
+    }
     const int64_t fi = fidx(g, C, i, j, k);
     float value;
     if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
@@ -258,6 +284,17 @@ __global__ void project_kernel(Grid g, const float* __restrict__ ustar, float* _
     if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
     const int c[3] = {i, j, k};
     if (boundary_face(g, C, c[C])) return;
+    if (c[C] > 0 && c[C] < g.n[C] && index32(g)) {
+        // an inner face between two fluid cells
+        const int ir = (k * g.n[1] + j) * g.n[0] + i, il = ir - cell_stride<C>(g);
+        if (occ[il] == 0 && occ[ir] == 0) {
+            const int fi32 = (k * (g.n[1] + (C == 1)) + j) * (g.n[0] + (C == 0)) + i;
+            float result = ustar[fi32];
+            result -= *dt_ptr * (p[ir] - p[il]) / g.h;
+            out[fi32] = result;
+            return;
+        }
+    }
     const int64_t fi = fidx(g, C, i, j, k);
     float value;
     if (solid_face_override(g, occ, solid_vel, C, i, j, k, value)) {
@@ -541,11 +578,6 @@ __device__ __forceinline__ void stage_inside(float* lds, const LdsBox& bx, const
         const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
         lds[q] = f[(k * d1 + j) * d0 + i];
     }
-}
-
-// arrays of the grid (faces included) are indexable in 32 bits
-__device__ __forceinline__ bool index32(const Grid& g) {
-    return (int64_t(g.n[0]) + 1) * (int64_t(g.n[1]) + 1) * (int64_t(g.n[2]) + 1) < (int64_t(1) << 31);
 }
 
 // sample_linear on staged values; the stencil must lie in the box
