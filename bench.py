@@ -110,16 +110,16 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
         raise ValueError(solver)
 
 
-def run_solver(solver: str, args, hip: abi.Hip, net, want_timing: bool):
+def run_solver(solver: str, args, hip: abi.Hip, net, want_timing: bool, scene: str):
     n = args.size
     cfg = workload_config(n)
-    if args.scene == "spheres":
+    if scene == "spheres":
         cfg.vorticity_confinement = 2.5
     src = plume(n)
     iters, resid, dts, fallbacks = [], [], [], 0
     out = {}
     with abi.SmokeContext(cfg, hip=hip) as ctx:
-        if args.scene == "spheres":
+        if scene == "spheres":
             ctx.set_occupancy(spheres_occupancy(n))
         configure(ctx, solver, net, args.warm_start, args.check_interval)
         step_index = 0
@@ -174,27 +174,24 @@ def add_timing(total, t):
     return total
 
 
-def cpu_baseline(n: int, iterations_per_step: float, threads: int):
-    """Bounded CPU sample of the same workload with the FP32 host port of the
-    step (the oracle source built with R = float, FP64 reductions, OpenMP):
-    one step's non-solve stages timed once plus 200 Jacobi-PCG iterations
-    timed (about 10 s of CPU work at 256^3), scaled to the GPU run's mean
-    plain-PCG iterations per step."""
+def cpu_baseline(n: int, scene: str):
+    """The CPU reference path on a bounded sample of the same workload: the
+    FP32 host port of the step (the oracle source built with R = float, FP64
+    reductions, OpenMP over the CPUs this process is given) runs step 0 of the
+    scene untimed (zero velocity, zero right-hand side) and then one whole
+    step, step 1 -- forces, advection, Jacobi-PCG to true rel-resid 1e-6 from
+    the previous pressure, projection, scalars -- timed with steady_clock."""
     sys.path.insert(0, str(REPO / "oracle"))
-    import oracle as orc
-    cfg = workload_config(n)
-    opts = orc.options(mode=1, preconditioner=1, tol=1e-6, max_refinements=4, warm_start=1, adaptive_dt=1, cfl=0.5,
-                       dt_max=0.01, inner_reduction=1e-4)
-    t0 = time.time()
-    sample_iterations = 200
-    other, per_iter = orc.f32_step_sample(cfg, opts, plume(n), sample_iterations, threads)
-    wall = time.time() - t0
-    seconds_per_step = other + iterations_per_step * per_iter
+    import cpu_baseline as cb
+    r = cb.run("c5" if scene == "spheres" else "c2", steps=1, warmup=1)
+    if n != 256:
+        raise ValueError("the CPU sample is defined on the 256^3 workloads")
     return {
-        "value": 1.0 / seconds_per_step, "unit": "steps/s", "cores": threads, "kind": "port",
-        "sample": (f"{n}^3 step from rest on {threads} host threads: non-solve stages {other:.2f} s once + {sample_iterations} Jacobi-PCG "
-                   f"iterations at {per_iter * 1e3:.1f} ms each, scaled to {iterations_per_step:.0f} iterations/step "
-                   f"(the GPU plain-PCG run's mean); {wall:.1f} s of CPU work"),
+        "value": r["steps_per_s"], "unit": "steps/s", "cores": r["threads"], "kind": "port",
+        "sample": (f"one whole step (step 1, after step 0 untimed) of the {n}^3 {'12-sphere' if scene == 'spheres' else 'plume'} "
+                   f"workload on {r['threads']} host threads: {r['step_seconds'][0]:.1f} s with {r['pressure_iterations'][0]} "
+                   f"Jacobi-PCG iterations to true rel-resid 1e-6 (no extrapolation; the GPU steps are later, warm-started ones)"),
+        "mlups": round(r["mlups"], 3), "host": r["host"],
     }
 
 
@@ -260,8 +257,11 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
                     help="initial guess of each solve: zero, previous pressure, extrapolation of the last two")
-    ap.add_argument("--scene", choices=("plume", "spheres"), default="plume",
-                    help="plume: the closed box; spheres: the same with 12 seeded solid spheres and confinement 2.5")
+    ap.add_argument("--scene", choices=("plume", "spheres"), default="spheres",
+                    help="spheres (default, the headline): the closed box with 12 seeded solid spheres and confinement 2.5; "
+                         "plume: the closed box alone")
+    ap.add_argument("--extra-scenes", default="plume",
+                    help="comma-separated scenes also timed (same solvers, no kernel events), reported under 'scenes'")
     ap.add_argument("--solvers", default="learned,jacobi,multilevel",
                     help="comma-separated, from learned, jacobi, multilevel, smoothed; the first one is the reported value")
     ap.add_argument("--weights", default=str(WEIGHTS))
@@ -280,10 +280,21 @@ def main():
     hip = abi.Hip()
     copy_gbps, _ = abi.measure_copy_bandwidth(1 << 30, 20)
     solvers = [s.strip() for s in args.solvers.split(",") if s.strip()]
+    extra_scenes = [s.strip() for s in args.extra_scenes.split(",") if s.strip() and s.strip() != args.scene]
     net = abi.LearnedPreconditioner(seed=0, path=args.weights) if "learned" in solvers else None
     results = {}
     for solver in solvers:
-        results[solver] = run_solver(solver, args, hip, net, not args.no_timing)
+        results[solver] = run_solver(solver, args, hip, net, not args.no_timing, args.scene)
+        print(f"bench {args.scene} {solver}: {results[solver]['steps_per_s']} steps/s", file=sys.stderr, flush=True)
+    scenes = {}
+    for scene in extra_scenes:
+        scenes[scene] = {}
+        for solver in solvers:
+            scenes[scene][solver] = run_solver(solver, args, hip, net, False, scene)
+            print(f"bench {scene} {solver}: {scenes[scene][solver]['steps_per_s']} steps/s", file=sys.stderr, flush=True)
+        if "learned" in scenes[scene] and "jacobi" in scenes[scene]:
+            scenes[scene]["learned_vs_plain_pcg_speedup"] = round(
+                scenes[scene]["learned"]["steps_per_s"] / scenes[scene]["jacobi"]["steps_per_s"], 3)
     if net is not None:
         net.close()
     head = solvers[0]
@@ -300,7 +311,11 @@ def main():
         roofline = {
             "bound": "hbm", "kernel": SWEEP_KERNEL + " (damped-Jacobi sweep of the V-cycle on the 256^3 pressure grid, float4 rows)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": load_traffic("mg_sweep_256.traffic.json"), "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
+            "traffic": load_traffic("mg_sweep_256.traffic.json"),
+            "traffic_source": ("profiles/mg_sweep_256.traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel at 256^3, "
+                               "scaled by the counters' reading for a copy kernel of known size; read from the file, not measured "
+                               "in this run"),
+            "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
             "avg_launch_ms": round(sweep_us * 1e-3, 5), "launches_timed": int(lk.mg_sweeps),
             "timed": ("every sweep of the pressure grid in the V-cycles of every 8th timed step (7 per cycle: the 3 "
                       "pre-sweeps after x = omega D^-1 r and the 4 post-sweeps, the last of which also sums r.z); HIP "
@@ -325,7 +340,9 @@ def main():
         pcg = {
             "kernel": UPDATE_KERNEL + " (plain PCG pass 3: e += alpha p, r -= alpha Ap, r.z, r.r)",
             "achieved": round(gbps["update"], 1), "frac": round(gbps["update"] / HBM_PEAK_GBPS, 4),
-            "traffic": load_traffic("pcg_update_256.traffic.json"), "algorithmic_bytes_per_launch": UPDATE_BYTES_PER_CELL * cells,
+            "traffic": load_traffic("pcg_update_256.traffic.json"),
+            "traffic_source": "profiles/pcg_update_256.traffic.json (rocprofv3 counters, read from the file)",
+            "algorithmic_bytes_per_launch": UPDATE_BYTES_PER_CELL * cells,
             "avg_launch_ms": round(ms["update"], 5), "launches_timed": k,
             "passes": {name: {"avg_launch_ms": round(ms[name], 5), "achieved_gbps": round(gbps[name], 1)} for name in ms},
             "finalize_kernels_ms": round(reduction_ms, 5),
@@ -344,7 +361,7 @@ def main():
         "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": results[head]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": (f"{n}^3 plume" + (" with 12 solid spheres" if args.scene == "spheres" else "") +
+        "config": {"workload": (f"{n}^3 plume" + (" with 12 seeded solid spheres (radius U(8,24)), confinement 2.5" if args.scene == "spheres" else "") +
                                 ", every pressure solve to true rel-resid 1e-6 (FP64 reductions + refinement), cubic advection, "
                                 f"adaptive dt; value = {head}"),
                    "grid": [n, n, n], "scene": args.scene, "relative_tolerance": 1e-6,
@@ -353,14 +370,15 @@ def main():
                                "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(4,4)-cycle, Jacobi coarse solver (no network)",
                                "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,
+        "scenes": scenes,
         "roofline": roofline, "cpu_baseline": None,
     }
     if args.batch_plumes > 0:
         modes = tuple(m.strip() for m in args.batch_modes.split(",") if m.strip())
         out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2, args.batch_lanes, modes)
-    if not args.no_cpu_baseline and "jacobi" in results:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(n, results["jacobi"]["pcg_iterations_per_step"], threads)
+    if not args.no_cpu_baseline and n == 256:
+        print("bench: CPU baseline (one whole 256^3 step on the host cores)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(n, args.scene)
     print(json.dumps(out), flush=True)
 
 

@@ -148,8 +148,32 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_export_pressure_
 /* Runs stages first..last (inclusive) of one step; the full step is 0..4. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_run_stages(SmokeSimulationContext context, int32_t first_stage, int32_t last_stage);
 
+/* Checkpoint file ("SMOKECK1", little-endian, no padding):
+     char[8] "SMOKECK1"; u32 version = 2; i32 nx, ny, nz; u32 field count = 18;
+     18 times: char[24] name (NUL-padded), u64 byte count, the field's bytes;
+     i32 warm-start history count (0..2); end of file.
+   Fields, in this order (x-fastest, the state layout): velocity_x/_y/_z
+   (MAC faces, FP32), density, temperature, density_source,
+   temperature_source, force_x/_y/_z, solid_velocity_x/_y/_z,
+   solid_temperature (FP32 cells), occupancy (u8 cells), pressure (FP32),
+   pressure_f64, pressure_f64_history (FP64 cells).  save writes the context's
+   state; load reads and checks the whole file before touching the context
+   (a truncated or mismatched file returns BACKEND_FAILURE and changes
+   nothing). */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContext context, const char* path);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContext context, const char* path);
+
+typedef struct SmokeSimulationCheckpointInfo {
+    uint32_t version;
+    int32_t dims[3];
+    uint32_t field_count;
+    int32_t solves_done;
+    char error[96];   /* why the file was rejected (empty when accepted) */
+} SmokeSimulationCheckpointInfo;
+
+/* Validates a checkpoint file on the host (no device call, no context):
+   layout, field names and sizes for its dimensions, end of file. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_inspect_checkpoint(const char* path, SmokeSimulationCheckpointInfo* out_info);
 
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_kernel_timing(SmokeSimulationContext context, int32_t enable);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_kernel_timing(SmokeSimulationContext context, SmokeSimulationKernelTiming* out_timing);

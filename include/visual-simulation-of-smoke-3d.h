@@ -1,4 +1,14 @@
 /*
+ * This Source Code Form is subject to the terms of the Mozilla Public
+ * License, v. 2.0. If a copy of the MPL was not distributed with this file,
+ * You can obtain one at https://mozilla.org/MPL/2.0/.
+ *
+ * The declarations below (enumerator names and values, struct field order and
+ * types, the twelve prototypes) are those of the reference header
+ * visual-simulation-of-smoke-3d.h:1-162, which is MPL-2.0 licensed; binary
+ * compatibility requires them to be identical, so this header carries the
+ * reference's licence.
+ *
  * C ABI of the smoke step, MI355X implementation.
  *
  * Binary-compatible drop-in for the interface declared by the reference

@@ -90,6 +90,8 @@ def lib():
         L.oracle_noise_bits.restype = C.c_uint32
         L.oracle_f32_benchmark.argtypes = [cfgp, C.POINTER(Options), C.POINTER(abi.PlumeSource), C.c_int, C.c_int,
                                            C.c_int, C.POINTER(C.c_double), C.POINTER(Stats)]
+        L.oracle_f32_steps.argtypes = [cfgp, C.POINTER(Options), C.POINTER(abi.PlumeSource), P, C.c_int, C.c_int, C.c_int,
+                                       P, P, C.POINTER(Stats)]
         L.oracle_f32_step_sample.argtypes = [cfgp, C.POINTER(Options), C.POINTER(abi.PlumeSource), C.c_int, C.c_int,
                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
@@ -240,6 +242,16 @@ def f32_benchmark(cfg, opts: Options, src: abi.PlumeSource | None, warmup: int, 
     lib().oracle_f32_benchmark(C.byref(cfg), C.byref(opts), C.byref(src) if src is not None else None, warmup, steps,
                                threads, C.byref(secs), C.byref(st))
     return secs.value, st
+
+
+def f32_steps(cfg, opts: Options, src: abi.PlumeSource | None, occupancy, warmup: int, steps: int, threads: int):
+    """Whole FP32 host-port steps: (seconds per timed step, pressure iterations per timed step)."""
+    secs = np.zeros(max(steps, 1))
+    its = np.zeros(max(steps, 1), np.int32)
+    occ = None if occupancy is None else np.ascontiguousarray(occupancy, np.uint8)
+    lib().oracle_f32_steps(C.byref(cfg), C.byref(opts), C.byref(src) if src is not None else None,
+                           None if occ is None else _p(occ), warmup, steps, threads, _p(secs), _p(its), C.byref(Stats()))
+    return secs[:steps].tolist(), its[:steps].tolist()
 
 
 def f32_step_sample(cfg, opts: Options, src: abi.PlumeSource | None, cg_iterations: int, threads: int):

@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -1124,6 +1126,35 @@ void set_threads(int threads) {
 }
 
 }  // namespace
+
+// Whole steps of the FP32 host port, each timed with steady_clock: `warmup`
+// untimed steps, then `steps` timed ones (seconds and pressure iterations per
+// timed step), optionally on an occupancy grid (uint8, cells).
+int oracle_f32_steps(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, const uint8_t* occupancy,
+                     int warmup, int steps, int threads, double* step_seconds, int* step_iterations, OracleStats* stats) {
+    set_threads(threads);
+    F32Run run(cfg);
+    if (occupancy) std::copy(occupancy, occupancy + run.n, run.occ.begin());
+    OracleStats local{};
+    float dt = cfg->dt;
+    for (int step = 0; step < warmup + steps; ++step) {
+        run.sources(cfg, src, uint32_t(step));
+        const auto t0 = std::chrono::steady_clock::now();
+        run_stages(cfg, opt, run.f, 0, 4, &local, dt);
+        const double secs = seconds_since(t0);
+        if (step >= warmup) {
+            if (step_seconds) step_seconds[step - warmup] = secs;
+            if (step_iterations) step_iterations[step - warmup] = local.iterations;
+        }
+        if (std::getenv("SMOKE_ORACLE_PROGRESS")) {
+            std::fprintf(stderr, "cpu port step %d%s: %.3f s, %d pressure iterations\n", step, step < warmup ? " (warm-up)" : "", secs,
+                         local.iterations);
+            std::fflush(stderr);
+        }
+    }
+    if (stats) *stats = local;
+    return 0;
+}
 
 int oracle_f32_benchmark(const SmokeSimulationConfig* cfg, const OracleOptions* opt, const OraclePlumeSource* src, int warmup, int steps, int threads, double* seconds, OracleStats* stats) {
     set_threads(threads);

@@ -26,7 +26,28 @@ def rel_l2(a, b) -> float:
     return float(num / den) if den > 0 else float(num)
 
 
-def make_state(cfg: abi.Config, seed: int, amplitude=0.5, solids=0, sources=True, solid_motion=False):
+def user_force(cfg: abi.Config, rng, amplitude=2.0):
+    """A smooth random cell force plus a small per-cell jitter, set in every
+    cell (solid cells included: the reference keeps the user force there,
+    B:902-908, and its scene caller sets it each frame, scene_plume.cpp:321-357)."""
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    zc, yc, xc = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny, (np.arange(nx) + 0.5) / nx, indexing="ij")
+    out = []
+    for _ in range(3):
+        k = rng.integers(1, 4, size=3)
+        ph = rng.uniform(0, 2 * np.pi, size=3)
+        f = np.sin(np.pi * k[0] * xc + ph[0]) * np.cos(np.pi * k[1] * yc + ph[1]) * np.sin(np.pi * k[2] * zc + ph[2])
+        f += 0.1 * rng.uniform(-1.0, 1.0, f.shape)
+        out.append((amplitude * f).astype(np.float32))
+    return out
+
+
+def report(tag: str, errs: dict) -> None:
+    """One parseable line per comparison: the measured errors behind each bound."""
+    print(f"PARITY {tag} " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+def make_state(cfg: abi.Config, seed: int, amplitude=0.5, solids=0, sources=True, solid_motion=False, force=True):
     rng = np.random.default_rng(seed)
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     h = float(np.float32(cfg.cell_size))
@@ -69,7 +90,7 @@ def make_state(cfg: abi.Config, seed: int, amplitude=0.5, solids=0, sources=True
     else:
         st["density_source"] = np.zeros(occ.shape, np.float32)
         st["temperature_source"] = np.zeros(occ.shape, np.float32)
-    st["force"] = [np.zeros(occ.shape, np.float32) for _ in range(3)]
+    st["force"] = user_force(cfg, np.random.default_rng(seed + 1000)) if force else [np.zeros(occ.shape, np.float32) for _ in range(3)]
     return st
 
 

@@ -1,23 +1,28 @@
 """HIP step vs the FP64 oracle, stage by stage and for whole steps, through
 the C ABI (B = backend-cuda.cu).
 
-Tolerances (relative L2 over the field unless stated):
-  * stages without interpolation (forces, RHS, projection): 1e-5 -- FP32
-    rounding against FP64;
-  * semi-Lagrangian stages: 2e-4 -- FP32 floor / limiter decisions can flip a
-    cubic tap near cell boundaries (B:378-392, 442-484);
+Tolerances (relative L2 over the field unless stated), each about 10x the
+error these tests measured on MI355X (every comparison prints a PARITY line
+with its errors):
+  * stages without interpolation (forces, projection): 3e-7 (measured
+    2.5e-8: FP32 rounding against FP64); the RHS 1.5e-5 (1.2e-6);
+  * semi-Lagrangian stages: 1.5e-5 for velocity, 1e-5 for traces of about 3
+    cells, 3e-6 for the scalars (measured <= 1.25e-6, 8.7e-7, 3.0e-7) -- FP32
+    floor / limiter decisions can flip a cubic tap near cell boundaries
+    (B:378-392, 442-484);
   * pressure: the GPU's FP64 solution checked against the ORACLE's operator:
-    ||b - A p|| / ||b|| <= 1e-6, plus 1e-3 on p
-    itself against a 1e-10 oracle solve;
-  * whole step from a shared state: 1e-3 (solve tolerance 1e-6 feeds through
-    the projection).
+    ||b - A p|| / ||b|| <= 1e-6, plus 1e-4 on p
+    itself against a 1e-10 oracle solve (measured <= 8.9e-6);
+  * whole step from a shared state: 1.5e-5, 2e-5 on the ragged grid (measured
+    <= 1.3e-6 and 1.7e-6: the 1e-6 solve tolerance feeds through the
+    projection).
 """
 import numpy as np
 import pytest
 
 import smoke_abi as abi
 import oracle as orc
-from parity import (O, P, W, load_gpu, load_oracle, make_state, one_step_parity, rel_l2, tolerance_options)
+from parity import (O, P, W, load_gpu, load_oracle, make_state, one_step_parity, rel_l2, report, tolerance_options)
 
 pytestmark = pytest.mark.gpu
 
@@ -61,8 +66,9 @@ def test_stage_forces(name):
         ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_FORCES)
         g = ctx.export_state()
     s.run_stages(0, 0, orc.options(mode=0))
-    for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w")):
-        assert rel_l2(g[a], s[b]) < 1e-5, (a, rel_l2(g[a], s[b]))
+    errs = {a: rel_l2(g[a], s[b]) for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w"))}
+    report(f"forces/{name}", errs)
+    assert max(errs.values()) < 3e-7, errs      # measured 2.5e-8
     assert rel_l2(g["temperature"], s["temperature"]) == 0.0
 
 
@@ -75,8 +81,9 @@ def test_stage_velocity_advection(name, cubic):
         ctx.run_stages(abi.STAGE_ADVECT_VELOCITY, abi.STAGE_ADVECT_VELOCITY)
         g = ctx.export_state()
     s.run_stages(1, 1, orc.options(mode=0))
-    for a, b in (("velocity_star_x", "us"), ("velocity_star_y", "vs"), ("velocity_star_z", "ws")):
-        assert rel_l2(g[a], s[b]) < 2e-4, (a, rel_l2(g[a], s[b]))
+    errs = {a: rel_l2(g[a], s[b]) for a, b in (("velocity_star_x", "us"), ("velocity_star_y", "vs"), ("velocity_star_z", "ws"))}
+    report(f"advect_velocity/{name}/cubic={cubic}", errs)
+    assert max(errs.values()) < 1.5e-5, errs    # measured <= 1.25e-6 (mixed, linear)
 
 
 STAGED = (32, 8, 1, 2)      # the LDS-staged form: tiles of 32 x 8 x 2 outputs
@@ -139,8 +146,8 @@ def test_stage_advection_long_traces(name, cubic):
     s2.run_stages(4, 4, orc.options())
     errs["temperature"] = rel_l2(gs["temperature"], s2["temperature"])
     errs["density"] = rel_l2(gs["density"], s2["density"])
-    print(name, cubic, errs, flush=True)
-    assert max(errs.values()) < 2e-4, errs
+    report(f"long_traces/{name}/cubic={cubic}", errs)
+    assert max(errs.values()) < 1e-5, errs      # measured <= 8.7e-7
 
 
 @pytest.mark.parametrize("name", list(CONFIGS))
@@ -154,14 +161,15 @@ def test_stage_pressure_solve(name):
         p64 = ctx.export_pressure_f64()
         stats = ctx.stats()
     s.run_stages(0, 2, orc.options(mode=1, tol=1e-10, inner_reduction=1e-12))
-    assert rel_l2(rhs, s["pressure_rhs"]) < 1e-5
     # the GPU solution against the oracle's operator and the GPU's own b
     r = rhs.astype(np.float64) - orc.pressure_apply(cfg, st["occupancy"], p64)
     true_rel = np.linalg.norm(r) / np.linalg.norm(rhs.astype(np.float64))
+    report(f"pressure/{name}", {"rhs": rel_l2(rhs, s["pressure_rhs"]), "true_residual": true_rel, "p": rel_l2(p64, s["pressure"])})
+    assert rel_l2(rhs, s["pressure_rhs"]) < 1.5e-5      # measured <= 1.2e-6
     assert stats.converged == 1
     assert true_rel <= 1.0e-6 * 1.01, true_rel
     assert abs(true_rel - stats.relative_residual) <= 1e-3 * true_rel + 1e-12
-    assert rel_l2(p64, s["pressure"]) < 1e-3
+    assert rel_l2(p64, s["pressure"]) < 1e-4      # measured <= 8.9e-6 (closed box, 1e-6 solve)
 
 
 @pytest.mark.parametrize("n", [(20, 24, 2), (20, 24, 1), (2, 24, 18), (1, 24, 18), (8, 24, 2)])
@@ -204,8 +212,9 @@ def test_stage_projection(name):
     s["us"], s["vs"], s["ws"] = g0["velocity_star_x"], g0["velocity_star_y"], g0["velocity_star_z"]
     s["pressure"] = st["pressure"]
     s.run_stages(3, 3, orc.options())
-    for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w")):
-        assert rel_l2(g[a], s[b]) < 1e-5, (a, rel_l2(g[a], s[b]))
+    errs = {a: rel_l2(g[a], s[b]) for a, b in (("velocity_x", "u"), ("velocity_y", "v"), ("velocity_z", "w"))}
+    report(f"projection/{name}", errs)
+    assert max(errs.values()) < 3e-7, errs      # measured 2.5e-8
 
 
 @pytest.mark.parametrize("name", list(CONFIGS))
@@ -217,16 +226,18 @@ def test_stage_scalars(name, cubic):
         ctx.run_stages(abi.STAGE_SCALARS, abi.STAGE_SCALARS)
         g = ctx.export_state()
     s.run_stages(4, 4, orc.options())
-    assert rel_l2(g["temperature"], s["temperature"]) < 2e-4
-    assert rel_l2(g["density"], s["density"]) < 2e-4
+    errs = {k: rel_l2(g[k], s[k]) for k in ("temperature", "density")}
+    report(f"scalars/{name}/cubic={cubic}", errs)
+    assert max(errs.values()) < 3e-6, errs      # measured <= 3.0e-7
 
 
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_full_step(name):
     errs = one_step_parity(cfg_for(name), seed=7, solids=2)
+    report(f"full_step/{name}", errs)
     assert errs["solver_relative_residual"] <= 1e-6
     for k in ("u", "v", "w", "density", "temperature"):
-        assert errs[k] < 1e-3, errs
+        assert errs[k] < 1.5e-5, errs            # measured <= 1.3e-6
 
 
 @pytest.mark.parametrize("name", ["closed", "benchmark"])
@@ -234,9 +245,10 @@ def test_full_step_ragged_multi_tile(name):
     # wider than one 64-column tile, taller than one 8-row tile, ragged in every
     # axis and deep enough for several z chunks of the PCG passes
     errs = one_step_parity(cfg_for(name, n=(150, 37, 23)), seed=13, solids=3)
+    report(f"full_step_ragged/{name}", errs)
     assert errs["solver_relative_residual"] <= 1e-6
     for k in ("u", "v", "w", "density", "temperature"):
-        assert errs[k] < 1e-3, errs
+        assert errs[k] < 2e-5, errs              # measured <= 1.7e-6
 
 
 def test_fixed_iteration_cg_matches_reference_semantics():
@@ -248,7 +260,8 @@ def test_fixed_iteration_cg_matches_reference_semantics():
         ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_PRESSURE)
         p = ctx.view("FLOW_PRESSURE")
     s.run_stages(0, 2, orc.options(mode=0))
-    assert rel_l2(p, s["pressure"]) < 1e-4
+    report("fixed_cg", {"p": rel_l2(p, s["pressure"])})
+    assert rel_l2(p, s["pressure"]) < 3e-6      # measured 2.2e-7
 
 
 def test_fixed_mode_graph_step_equals_staged_step():
@@ -400,3 +413,48 @@ def test_full_size_256_plume_properties():
     h = np.float32(cfg.cell_size)
     div_star = (np.diff(g["velocity_star_x"], axis=2) + np.diff(g["velocity_star_y"], axis=1) + np.diff(g["velocity_star_z"], axis=0)) / h
     assert np.abs(div).max() < 1e-3 * np.abs(div_star).max()
+
+
+def test_checkpoint_loads_a_hand_assembled_file_and_rejects_a_bad_one(tmp_path):
+    """SMOKECK1 written byte by byte from the documented layout
+    (tests/checkpoint_format.py) loads into a context: the state it exports is
+    the file's bits.  A file saved by the library parses with the same
+    independent reader.  A truncated file is rejected and leaves the context
+    unchanged (the whole file is checked before any device copy)."""
+    import checkpoint_format as ckf
+    cfg = cfg_for("closed", n=(12, 10, 8))
+    dims = (cfg.nx, cfg.ny, cfg.nz)
+    st = ckf.random_state(*dims, seed=5)
+    st["occupancy"][:] = 0
+    st["occupancy"][2:4, 3:6, 4:7] = 1
+    # identity rows (solids and the anchor, the first fluid cell) of the FP64
+    # solutions read back as +0: loading rebuilds the occupancy-derived data,
+    # which zeroes them
+    for k in ("pressure_f64", "pressure_f64_history"):
+        st[k][st["occupancy"] == 1] = 0.0
+        st[k].reshape(-1)[0] = 0.0
+    good = tmp_path / "hand.ck"
+    good.write_bytes(ckf.assemble(*dims, st, solves_done=1))
+    with abi.SmokeContext(cfg) as ctx:
+        assert ctx.lib.smoke_simulation_ext_load_checkpoint(ctx.handle, str(good).encode()) == 0
+        g = ctx.export_state()
+        for k in ("velocity_x", "velocity_y", "velocity_z", "density", "temperature", "pressure"):
+            np.testing.assert_array_equal(g[k], st[k], err_msg=k)
+        np.testing.assert_array_equal(ctx.export_pressure_f64(), st["pressure_f64"])
+        np.testing.assert_array_equal(ctx.view("OCCUPANCY"), st["occupancy"].astype(np.float32))
+        np.testing.assert_array_equal(ctx.view("DENSITY_SOURCE"), st["density_source"])
+        fx, fy, fz = ctx.view("FORCE")
+        np.testing.assert_array_equal(fy, st["force_y"])
+        np.testing.assert_array_equal(ctx.view("SOLID_TEMPERATURE"), st["solid_temperature"])
+        bad = tmp_path / "bad.ck"
+        bad.write_bytes(good.read_bytes()[:-100])
+        assert ctx.lib.smoke_simulation_ext_load_checkpoint(ctx.handle, str(bad).encode()) == 2
+        after = ctx.export_state()
+        for k in ("velocity_x", "density", "pressure"):
+            np.testing.assert_array_equal(after[k], st[k], err_msg=k)
+        saved = tmp_path / "saved.ck"
+        assert ctx.lib.smoke_simulation_ext_save_checkpoint(ctx.handle, str(saved).encode()) == 0
+    d, back, solves = ckf.parse(saved.read_bytes())
+    assert d == dims and solves == 1
+    for k in st:
+        np.testing.assert_array_equal(back[k], st[k], err_msg=k)

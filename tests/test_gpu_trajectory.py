@@ -15,10 +15,11 @@ feed the nonlinear advection; density, with the sharpest fronts, drifts most
 (a cubic-limiter decision that flips on one side moves a whole tap).
 Measured on MI355X, worst relative L2 over steps 1..99 at 64^3: u, v, w
 <= 1.9e-4, density 7.5e-4, temperature 1.9e-4; at 32^3 over 60 steps: 4.8e-5
-and 4.9e-4; at 256^3 (the benchmark workload) over its first 5 steps:
-<= 1.4e-6 for every field.  The bounds are ~5x those.
-`SMOKE_TRAJECTORY_256=1` adds the 256^3 case (about 2.5 minutes, mostly
-oracle time);
+and 4.9e-4; at 32^3 with per-step user forces over 30 steps: 1.0e-6 and
+3.1e-4; at 256^3 (the benchmark workload) over its first 2 steps: <= 1.4e-6
+for every field.  The bounds are 5-10x those.
+The 256^3 case runs 2 steps by default (about a minute, mostly oracle time),
+5 with `SMOKE_TRAJECTORY_256=1`;
 `SMOKE_TRAJECTORY_LOG=<file>` writes the per-step errors.
 """
 import json
@@ -51,7 +52,32 @@ def plume_config(n: int):
     return cfg, src
 
 
-def run_trajectory(n: int, steps: int, cfg=None, src=None, oracle_tol=1e-10):
+def scene_forces(cfg, seed: int):
+    """Per-step user forces in the pattern of the reference's scene caller
+    (scene_plume.cpp:271-296, 321-357): an emitter weight exp(-2.2 r^2) over an
+    ellipsoid at (0.5, 0.1, 0.5) of the domain, times a force of speed
+    U(0.82, 1.2) * base along a random near-vertical direction, redrawn each step."""
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    h = np.float32(cfg.cell_size)
+    z, y, x = np.meshgrid(np.arange(nz, dtype=np.float32), np.arange(ny, dtype=np.float32), np.arange(nx, dtype=np.float32), indexing="ij")
+    ex, ey, ez = np.float32(nx) * h * np.float32(0.5), np.float32(ny) * h * np.float32(0.1), np.float32(nz) * h * np.float32(0.5)
+    rx, ry, rz = np.float32(nx) * h * np.float32(0.12), np.float32(ny) * h * np.float32(0.08), np.float32(nz) * h * np.float32(0.12)
+    dx = ((x + np.float32(0.5)) * h - ex) / rx
+    dy = ((y + np.float32(0.5)) * h - ey) / ry
+    dz = ((z + np.float32(0.5)) * h - ez) / rz
+    r2 = dx * dx + dy * dy + dz * dz
+    weight = np.where(r2 <= 1.0, np.exp(np.float32(-2.2) * r2), np.float32(0.0)).astype(np.float32)
+    rng = np.random.default_rng(seed)
+
+    def force(step):
+        lx, lz = rng.uniform(-0.28, 0.28, size=2)
+        speed = 2.0 * rng.uniform(0.82, 1.2)
+        norm = np.sqrt(lx * lx + 1.0 + lz * lz)
+        return [(weight * np.float32(speed * d / norm)).astype(np.float32) for d in (lx, 1.0, lz)]
+    return force
+
+
+def run_trajectory(n: int, steps: int, cfg=None, src=None, oracle_tol=1e-10, force=None):
     if cfg is None:
         cfg, src = plume_config(n)
     oracle = orc.OracleState(cfg)
@@ -62,6 +88,10 @@ def run_trajectory(n: int, steps: int, cfg=None, src=None, oracle_tol=1e-10):
         bench.solver_options(ctx, 16)
         for k in range(steps):
             ctx.plume_sources(src, k)
+            if force is not None:
+                fx, fy, fz = force(k)
+                ctx.update("force", fx, fy, fz)
+                oracle["force_user_x"], oracle["force_user_y"], oracle["force_user_z"] = fx, fy, fz
             ctx.step()
             st = ctx.stats()
             rs, ts = orc.plume_sources(cfg, src, k)
@@ -91,6 +121,7 @@ def _worst(rows, first=0):
 def test_plume_trajectory_32_cubed_60_steps():
     rows = run_trajectory(32, 60)
     worst = _worst(rows, first=1)   # step 0 has a zero velocity field
+    print("PARITY trajectory32 " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()), flush=True)
     for on, _ in FIELDS:
         assert worst[on] < (2.5e-3 if on == "density" else 2.5e-4), (on, worst)
     # dt follows the same CFL rule on both sides
@@ -98,19 +129,35 @@ def test_plume_trajectory_32_cubed_60_steps():
         assert abs(r["dt_gpu"] - r["dt_oracle"]) <= 1e-4 * r["dt_oracle"], r
 
 
+def test_plume_trajectory_32_cubed_scene_forces():
+    """A user force set before every step, as the reference's scene caller does
+    (scene_plume.cpp:321-357): 30 steps against the oracle given the same forces."""
+    cfg, src = plume_config(32)
+    rows = run_trajectory(32, 30, cfg=cfg, src=src, force=scene_forces(cfg, seed=4))
+    worst = _worst(rows, first=1)
+    print("PARITY trajectory32_forces " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()), flush=True)
+    # measured: u, v, w, temperature <= 1.0e-6, density 3.1e-4
+    for on, _ in FIELDS:
+        assert worst[on] < (3e-3 if on == "density" else 1e-5), (on, worst)
+
+
 def test_plume_trajectory_64_cubed_100_steps():
     """The 64^3 plume workload in full: 100 steps."""
     rows = run_trajectory(64, 100)
     worst = _worst(rows, first=1)
+    print("PARITY trajectory64 " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()), flush=True)
     for on, _ in FIELDS:
         assert worst[on] < (4e-3 if on == "density" else 1e-3), (on, worst)
 
 
-@pytest.mark.skipif(os.environ.get("SMOKE_TRAJECTORY_256") != "1", reason="long run: set SMOKE_TRAJECTORY_256=1")
-def test_plume_trajectory_256_cubed_5_steps():
+@pytest.mark.timeout(900)
+def test_plume_trajectory_256_cubed_first_steps():
     """The 256^3 benchmark workload (source radius 24 at (128, 32, 128)):
-    error against the FP64 oracle over its first 5 steps."""
-    rows = run_trajectory(256, 5, cfg=bench.workload_config(256), src=bench.plume(256), oracle_tol=1e-9)
+    error against the FP64 oracle over its first steps -- 2 by default (about
+    a minute of oracle time), 5 with SMOKE_TRAJECTORY_256=1."""
+    steps = 5 if os.environ.get("SMOKE_TRAJECTORY_256") == "1" else 2
+    rows = run_trajectory(256, steps, cfg=bench.workload_config(256), src=bench.plume(256), oracle_tol=1e-9)
     worst = _worst(rows, first=1)
+    print("PARITY trajectory256 " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()), flush=True)
     for on, _ in FIELDS:
         assert worst[on] < 1e-5, (on, worst)

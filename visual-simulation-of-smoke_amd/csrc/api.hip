@@ -15,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -280,37 +281,101 @@ void step(Context& c) {
     c.diagnostics_dirty = true;
 }
 
-// Checkpoint layout: "SMOKECK1", u32 version, i32 nx, ny, nz, u32 field count,
-// then per field: char[24] name, u64 byte count, raw little-endian data.
+// Checkpoint layout: "SMOKECK1", u32 version 2, i32 nx, ny, nz, u32 field
+// count, then per field: char[24] name (NUL-padded), u64 byte count, raw
+// little-endian data; after the last field an i32 (solves with a valid
+// warm-start history, 0..2) and nothing else.  The field list and sizes are
+// fixed by the dimensions (checkpoint_layout).
 struct CheckpointField {
     const char* name;
     void* device;
     size_t bytes;
 };
 
-std::vector<CheckpointField> checkpoint_fields(Context& c) {
-    const size_t cells = size_t(c.grid.cells);
+constexpr uint32_t kCheckpointVersion = 2;
+
+std::vector<CheckpointField> checkpoint_layout(int nx, int ny, int nz) {
+    const size_t cells = size_t(nx) * size_t(ny) * size_t(nz);
     const size_t cb = cells * sizeof(float);
+    const size_t fu = size_t(nx + 1) * size_t(ny) * size_t(nz) * sizeof(float);
+    const size_t fv = size_t(nx) * size_t(ny + 1) * size_t(nz) * sizeof(float);
+    const size_t fw = size_t(nx) * size_t(ny) * size_t(nz + 1) * sizeof(float);
     return {
-        {"velocity_x", c.face[0], size_t(c.grid.faces[0]) * sizeof(float)},
-        {"velocity_y", c.face[1], size_t(c.grid.faces[1]) * sizeof(float)},
-        {"velocity_z", c.face[2], size_t(c.grid.faces[2]) * sizeof(float)},
-        {"density", c.density, cb},
-        {"temperature", c.temperature, cb},
-        {"density_source", c.density_source, cb},
-        {"temperature_source", c.temperature_source, cb},
-        {"force_x", c.force_user[0], cb},
-        {"force_y", c.force_user[1], cb},
-        {"force_z", c.force_user[2], cb},
-        {"solid_velocity_x", c.solid_velocity[0], cb},
-        {"solid_velocity_y", c.solid_velocity[1], cb},
-        {"solid_velocity_z", c.solid_velocity[2], cb},
-        {"solid_temperature", c.solid_temperature, cb},
-        {"occupancy", c.occupancy, cells},
-        {"pressure", c.pressure, cb},
-        {"pressure_f64", c.x64[c.x64_current], cells * sizeof(double)},
-        {"pressure_f64_history", c.x64_hist, cells * sizeof(double)},
+        {"velocity_x", nullptr, fu},
+        {"velocity_y", nullptr, fv},
+        {"velocity_z", nullptr, fw},
+        {"density", nullptr, cb},
+        {"temperature", nullptr, cb},
+        {"density_source", nullptr, cb},
+        {"temperature_source", nullptr, cb},
+        {"force_x", nullptr, cb},
+        {"force_y", nullptr, cb},
+        {"force_z", nullptr, cb},
+        {"solid_velocity_x", nullptr, cb},
+        {"solid_velocity_y", nullptr, cb},
+        {"solid_velocity_z", nullptr, cb},
+        {"solid_temperature", nullptr, cb},
+        {"occupancy", nullptr, cells},
+        {"pressure", nullptr, cb},
+        {"pressure_f64", nullptr, cells * sizeof(double)},
+        {"pressure_f64_history", nullptr, cells * sizeof(double)},
     };
+}
+
+std::vector<CheckpointField> checkpoint_fields(Context& c) {
+    auto f = checkpoint_layout(c.config.nx, c.config.ny, c.config.nz);
+    void* dev[] = {c.face[0], c.face[1], c.face[2], c.density, c.temperature, c.density_source, c.temperature_source,
+                   c.force_user[0], c.force_user[1], c.force_user[2], c.solid_velocity[0], c.solid_velocity[1], c.solid_velocity[2],
+                   c.solid_temperature, c.occupancy, c.pressure, c.x64[c.x64_current], c.x64_hist};
+    static_assert(sizeof(dev) / sizeof(dev[0]) == 18);
+    for (size_t i = 0; i < f.size(); ++i) f[i].device = dev[i];
+    return f;
+}
+
+// A whole checkpoint file read and validated on the host (no device call):
+// magic, version, dimensions (positive; equal to `want` when given), the
+// field list of those dimensions in order with matching names and sizes, the
+// warm-start count and the end of the file.  Returns an empty reason on
+// success; `data` then holds every field's bytes.
+struct CheckpointFile {
+    int32_t dims[3] = {0, 0, 0};
+    uint32_t version = 0;
+    int32_t solves_done = 0;
+    std::vector<std::vector<char>> data;
+};
+
+std::string read_checkpoint(const char* path, const int32_t* want, CheckpointFile& out) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return "cannot open the file";
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{f};
+    char magic[8];
+    uint32_t count = 0;
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "SMOKECK1", 8) != 0) return "not a SMOKECK1 file";
+    if (std::fread(&out.version, 4, 1, f) != 1 || out.version != kCheckpointVersion) return "unsupported version";
+    if (std::fread(out.dims, 4, 3, f) != 3 || out.dims[0] <= 0 || out.dims[1] <= 0 || out.dims[2] <= 0) return "bad dimensions";
+    if (int64_t(out.dims[0]) * out.dims[1] * out.dims[2] >= (int64_t(1) << 31) - 1) return "bad dimensions";
+    if (want && (out.dims[0] != want[0] || out.dims[1] != want[1] || out.dims[2] != want[2])) return "dimensions differ from the context";
+    const auto layout = checkpoint_layout(out.dims[0], out.dims[1], out.dims[2]);
+    if (std::fread(&count, 4, 1, f) != 1 || count != layout.size()) return "wrong field count";
+    out.data.assign(layout.size(), {});
+    for (size_t i = 0; i < layout.size(); ++i) {
+        char name[24];
+        uint64_t bytes = 0;
+        if (std::fread(name, 1, 24, f) != 24 || std::fread(&bytes, 8, 1, f) != 1) return "truncated field header";
+        char expect[24] = {};
+        std::strncpy(expect, layout[i].name, sizeof(expect) - 1);
+        if (std::memcmp(name, expect, 24) != 0) return std::string("unexpected field in place of ") + layout[i].name;
+        if (bytes != layout[i].bytes) return std::string("wrong size of ") + layout[i].name;
+        out.data[i].resize(layout[i].bytes);
+        if (std::fread(out.data[i].data(), 1, layout[i].bytes, f) != layout[i].bytes) return std::string("truncated ") + layout[i].name;
+    }
+    if (std::fread(&out.solves_done, 4, 1, f) != 1 || out.solves_done < 0 || out.solves_done > 2) return "bad warm-start count";
+    char extra;
+    if (std::fread(&extra, 1, 1, f) != 0) return "trailing bytes";
+    return {};
 }
 
 }  // namespace
@@ -673,7 +738,7 @@ SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContex
         FILE* f = std::fopen(path, "wb");
         if (!f) throw smoke::HipError("cannot open checkpoint for writing");
         auto fields = smoke::checkpoint_fields(c);
-        const uint32_t version = 2, count = uint32_t(fields.size());
+        const uint32_t version = smoke::kCheckpointVersion, count = uint32_t(fields.size());
         const int32_t dims[3] = {c.config.nx, c.config.ny, c.config.nz};
         bool ok = std::fwrite("SMOKECK1", 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 && std::fwrite(dims, 4, 3, f) == 3 &&
                   std::fwrite(&count, 4, 1, f) == 1;
@@ -697,35 +762,40 @@ SmokeSimulationResult smoke_simulation_ext_save_checkpoint(SmokeSimulationContex
 SmokeSimulationResult smoke_simulation_ext_load_checkpoint(SmokeSimulationContext context, const char* path) {
     if (!path) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) {
+        // the whole file is read and checked before any device buffer
+        // changes: a bad file leaves the context as it was
+        const int32_t want[3] = {c.config.nx, c.config.ny, c.config.nz};
+        smoke::CheckpointFile file;
+        const std::string why = smoke::read_checkpoint(path, want, file);
+        if (!why.empty()) throw smoke::HipError("checkpoint: " + why);
         hip_check(hipStreamSynchronize(c.stream), "sync");
-        FILE* f = std::fopen(path, "rb");
-        if (!f) throw smoke::HipError("cannot open checkpoint");
-        char magic[8];
-        uint32_t version = 0, count = 0;
-        int32_t dims[3];
-        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SMOKECK1", 8) == 0 && std::fread(&version, 4, 1, f) == 1 && version == 2 &&
-                  std::fread(dims, 4, 3, f) == 3 && dims[0] == c.config.nx && dims[1] == c.config.ny && dims[2] == c.config.nz &&
-                  std::fread(&count, 4, 1, f) == 1;
         auto fields = smoke::checkpoint_fields(c);
-        ok = ok && count == fields.size();
-        std::vector<char> host;
-        for (auto& fd : fields) {
-            if (!ok) break;
-            char name[24];
-            uint64_t bytes = 0;
-            ok = std::fread(name, 1, 24, f) == 24 && std::fread(&bytes, 8, 1, f) == 1 && bytes == fd.bytes && std::strncmp(name, fd.name, 24) == 0;
-            if (!ok) break;
-            host.resize(fd.bytes);
-            ok = std::fread(host.data(), 1, fd.bytes, f) == fd.bytes && hipMemcpy(fd.device, host.data(), fd.bytes, hipMemcpyHostToDevice) == hipSuccess;
-        }
-        int32_t solves_done = 0;
-        ok = ok && std::fread(&solves_done, 4, 1, f) == 1 && solves_done >= 0 && solves_done <= 2;
-        std::fclose(f);
-        if (!ok) throw smoke::HipError("checkpoint read failed");
+        for (size_t i = 0; i < fields.size(); ++i)
+            hip_check(hipMemcpy(fields[i].device, file.data[i].data(), fields[i].bytes, hipMemcpyHostToDevice), "checkpoint upload");
         smoke::enqueue_occupancy_derived(c);
-        c.solves_done = solves_done;
+        c.solves_done = file.solves_done;
         hip_check(hipStreamSynchronize(c.stream), "sync");
     });
+}
+
+SmokeSimulationResult smoke_simulation_ext_inspect_checkpoint(const char* path, SmokeSimulationCheckpointInfo* out_info) {
+    if (!path || !out_info) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    try {
+        smoke::CheckpointFile file;
+        const std::string why = smoke::read_checkpoint(path, nullptr, file);
+        *out_info = SmokeSimulationCheckpointInfo{};
+        std::strncpy(out_info->error, why.c_str(), sizeof(out_info->error) - 1);
+        if (!why.empty()) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+        out_info->version = file.version;
+        for (int k = 0; k < 3; ++k) out_info->dims[k] = file.dims[k];
+        out_info->field_count = uint32_t(file.data.size());
+        out_info->solves_done = file.solves_done;
+        return SMOKE_SIMULATION_RESULT_OK;
+    } catch (const std::bad_alloc&) {
+        return SMOKE_SIMULATION_RESULT_OUT_OF_MEMORY;
+    } catch (...) {
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
 }
 
 SmokeSimulationResult smoke_simulation_ext_set_kernel_timing(SmokeSimulationContext context, int32_t enable) {

@@ -43,6 +43,7 @@ struct SmokeNnPreconditioner {
     float* z = nullptr;     // network output (channel 0, FP32)
     float* q = nullptr;     // loss residual A z - r
     double* sums = nullptr; // per volume: sum r^2, ||r||^2 over fluid rows, ||q||^2
+    double* partials = nullptr;   // per volume and block of the sum-of-squares pass: its partial sum of r^2
 };
 
 namespace smoke {
@@ -71,6 +72,7 @@ constexpr LayerSpec kLayers[] = {
     {SMOKE_NN_CONV_SAME, 16, 16, 0, 0, 0, 16, 1},   // out
 };
 constexpr uint8_t kIdentity = 0x40;
+constexpr int kSumBlocks = 256;   // blocks of the sum-of-squares pass per volume (fixed: the sum's order does not depend on the size)
 
 struct Dims {
     int n[3][3];   // level -> (W, H, D)
@@ -116,6 +118,7 @@ void free_workspace(Net& net) {
     std::memset(net.G, 0, sizeof(net.G));
     net.rn = net.z = net.q = nullptr;
     net.sums = nullptr;
+    net.partials = nullptr;
     net.ws_batch = 0;
 }
 
@@ -135,6 +138,7 @@ void ensure_workspace(Net& net, int batch, const int dims[3], bool train) {
     net.rn = dalloc<float>(net, size_t(voxels(d, 0, batch)));
     net.z = dalloc<float>(net, size_t(voxels(d, 0, batch)));
     net.sums = dalloc<double>(net, size_t(3 * batch));
+    net.partials = dalloc<double>(net, size_t(kSumBlocks) * batch);
     net.ws_batch = batch;
     std::memcpy(net.ws_dims, dims, sizeof(net.ws_dims));
     net.ws_train = train;
@@ -153,8 +157,12 @@ __global__ void zero_sums_kernel(double* __restrict__ p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
-// per volume (blockIdx.y): sum of r^2
-__global__ __launch_bounds__(256) void sum_squares_kernel(const float* __restrict__ r, int64_t cells, double* __restrict__ sums) {
+// per volume (blockIdx.y): sum of r^2 in a fixed order -- each of the
+// kSumBlocks blocks sums a strided share of the cells (wave shuffles, then the
+// block's waves in order) into its partial, and finalize_sums_kernel adds the
+// partials in block order -- so the normalisation s, and with it every
+// bfloat16 input, is the same on every run
+__global__ __launch_bounds__(256) void sum_squares_kernel(const float* __restrict__ r, int64_t cells, double* __restrict__ partials) {
     const int b = blockIdx.y;
     double s = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < cells; i += int64_t(gridDim.x) * 256) {
@@ -162,7 +170,19 @@ __global__ __launch_bounds__(256) void sum_squares_kernel(const float* __restric
         s += v * v;
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&sums[3 * b], s);
+    __shared__ double wave[4];
+    if ((threadIdx.x & 63) == 0) wave[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[int64_t(b) * gridDim.x + blockIdx.x] = ((wave[0] + wave[1]) + wave[2]) + wave[3];
+}
+
+// one block per volume: sums[3 b] = the volume's partials added in block order
+__global__ __launch_bounds__(64) void finalize_sums_kernel(const double* __restrict__ partials, int blocks, double* __restrict__ sums) {
+    const int b = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < blocks; i += 64) s += partials[int64_t(b) * blocks + i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) sums[3 * b] = s;
 }
 
 // input channels (r / s, identity mask, zeros), the normalised target rn and
@@ -342,8 +362,9 @@ void backward(Net& net, int batch, const Dims& d, hipStream_t s) {
 
 void prepare_input(Net& net, const float* r, const uint8_t* flags, int batch, int64_t cells, hipStream_t s) {
     zero_sums_kernel<<<1, 64, 0, s>>>(net.sums, 3 * batch);
+    sum_squares_kernel<<<dim3(kSumBlocks, batch), 256, 0, s>>>(r, cells, net.partials);
+    finalize_sums_kernel<<<batch, 64, 0, s>>>(net.partials, kSumBlocks, net.sums);
     const dim3 grid(grid_for(cells), batch);
-    sum_squares_kernel<<<grid, 256, 0, s>>>(r, cells, net.sums);
     input_kernel<<<grid, 256, 0, s>>>(r, flags, cells, net.sums, net.T[kIn], net.rn);
     hip_check(hipGetLastError(), "preconditioner input");
 }

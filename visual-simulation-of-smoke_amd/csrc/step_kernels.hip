@@ -445,11 +445,17 @@ __global__ void nearest_fluid_kernel(Grid g, const uint8_t* __restrict__ occ, in
     pairs[2 * slot + 1] = best;
 }
 
-__global__ void zero_identity_rows_kernel(const uint8_t* __restrict__ flags, double* __restrict__ x0, double* __restrict__ x1, int64_t n) {
+// Identity rows (solid cells, the anchor) hold exactly +0 in every FP64
+// solution buffer, the warm-start history included: the PCG passes rely on
+// p = 0 there, and an extrapolated start 2 x_n - x_(n-1) keeps the zeros only
+// if both terms have them.
+__global__ void zero_identity_rows_kernel(const uint8_t* __restrict__ flags, double* __restrict__ x0, double* __restrict__ x1,
+                                          double* __restrict__ hist, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n && (flags[i] & 0x40)) {
         x0[i] = 0.0;
         x1[i] = 0.0;
+        hist[i] = 0.0;
     }
 }
 
@@ -1056,7 +1062,7 @@ void enqueue_occupancy_derived(Context& c) {
     anchor_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.occupancy, c.anchor, g.cells);
     stencil_flags_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.occupancy, c.anchor, c.stencil_flags);
     nearest_fluid_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.occupancy, c.solid_pairs, c.solid_count);
-    zero_identity_rows_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.stencil_flags, c.x64[0], c.x64[1], g.cells);
+    zero_identity_rows_kernel<<<grid1(g.cells), kBlock, 0, c.stream>>>(c.stencil_flags, c.x64[0], c.x64[1], c.x64_hist, g.cells);
     hip_check(hipGetLastError(), "occupancy-derived kernels");
     c.diagnostics_dirty = true;
 }

@@ -149,6 +149,11 @@ class PlumeSource(C.Structure):
                 ("temperature_noise", C.c_float), ("seed", C.c_uint32)]
 
 
+class CheckpointInfo(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("dims", C.c_int32 * 3), ("field_count", C.c_uint32), ("solves_done", C.c_int32),
+                ("error", C.c_char * 96)]
+
+
 class BatchCreateDesc(C.Structure):
     _fields_ = [("plumes", C.c_void_p), ("count", C.c_uint32), ("lanes", C.c_int32)]
 
@@ -290,6 +295,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_run_stages": [C.c_void_p, C.c_int32, C.c_int32],
         "smoke_simulation_ext_save_checkpoint": [C.c_void_p, C.c_char_p],
         "smoke_simulation_ext_load_checkpoint": [C.c_void_p, C.c_char_p],
+        "smoke_simulation_ext_inspect_checkpoint": [C.c_char_p, C.POINTER(CheckpointInfo)],
         "smoke_simulation_ext_set_kernel_timing": [C.c_void_p, C.c_int32],
         "smoke_simulation_ext_get_kernel_timing": [C.c_void_p, C.POINTER(KernelTiming)],
         "smoke_simulation_ext_measure_copy_bandwidth": [C.c_uint64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
@@ -607,6 +613,13 @@ class LearnedPreconditioner:
         check(self.lib.smoke_nn_preconditioner_train_step(self.handle, d_r, d_flags, batch, (C.c_int32 * 3)(*dims), lr, C.byref(loss),
                                                           None, None), "train step")
         return loss.value
+
+
+def inspect_checkpoint(path) -> tuple[int, CheckpointInfo]:
+    """Host-only validation of a SMOKECK1 file: (result code, info)."""
+    info = CheckpointInfo()
+    rc = load_library().smoke_simulation_ext_inspect_checkpoint(str(path).encode(), C.byref(info))
+    return rc, info
 
 
 def measure_copy_bandwidth(nbytes: int, reps: int):
