@@ -129,6 +129,23 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_set_loss(Smok
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_apply(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags,
                                                                          const int32_t dims[3], float* z, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the same z = M(r) as smoke_nn_preconditioner_apply, with
+   the same kernels and the same bits, but the inference's workspace
+   (activations, normalised residual, output, sums) lives in a static 16 MiB
+   device array of the library instead of buffers the network allocates on
+   its first call for a volume size; it calls no hipMalloc / hipFree.  The
+   network's weights were packed at create or by an earlier call.  Limit: the
+   workspace of a W x H x D volume is about 188 bytes per voxel (plus 256-byte
+   alignment per buffer), so volumes up to about 89,000 voxels fit (44^3,
+   64 x 48 x 28); a larger volume returns BACKEND_FAILURE and prints the
+   reason to stderr.  Every GPU holds its own array until the process ends;
+   calls on one GPU share it, so two calls that overlap on one GPU (two
+   streams, or two host threads) race and must be ordered by the caller. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_preconditioner_apply_static(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags,
+                                                                                const int32_t dims[3], float* z, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

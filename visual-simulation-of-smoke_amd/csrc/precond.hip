@@ -8,11 +8,19 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
 using smoke::HipError;
 using smoke::hip_check;
+
+// Workspace of smoke_nn_preconditioner_apply_static: the activations, the
+// normalised residual, the output and the sums of one inference live in this
+// array instead of buffers the network allocates.  Each GPU holds its own copy
+// until the process ends; calls on one GPU share it.
+constexpr size_t kStaticWorkspaceBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
 
 struct SmokeNnPreconditioner {
     struct Layer {
@@ -452,6 +460,92 @@ void preconditioner_apply(SmokeNnPreconditioner& net, const float* r, const uint
     hip_check(hipGetLastError(), "preconditioner output");
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (the host-side symbol is not a device address).
+void* static_workspace() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex m;
+    static void* ptr[kMaxDevices] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw HipError("device index out of range");
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
+// Offsets of one inference's buffers in the static workspace (256-byte
+// aligned): the activations, rn, z, the sums and the sum partials.
+struct StaticLayout {
+    size_t tensor[kTensorCount];
+    size_t rn, z, sums, partials, total;
+};
+
+StaticLayout static_layout(const int dims[3]) {
+    const Dims d = level_dims(dims);
+    const int level_of[kTensorCount] = {0, 0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
+    const int chan_of[kTensorCount] = {8, 16, 16, 32, 32, 64, 64, 32, 32, 16, 16};
+    StaticLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    for (int t = 0; t < kTensorCount; ++t) L.tensor[t] = take(size_t(voxels(d, level_of[t], 1)) * chan_of[t] * sizeof(uint16_t));
+    L.rn = take(size_t(voxels(d, 0, 1)) * sizeof(float));
+    L.z = take(size_t(voxels(d, 0, 1)) * sizeof(float));
+    L.sums = take(3 * sizeof(double));
+    L.partials = take(size_t(kSumBlocks) * sizeof(double));
+    L.total = off;
+    return L;
+}
+
+// The same kernels as preconditioner_apply, on buffers in the static
+// workspace: no device memory is allocated.
+void preconditioner_apply_static(SmokeNnPreconditioner& net, const float* r, const uint8_t* flags, const int dims[3], float* z, hipStream_t s) {
+    const StaticLayout L = static_layout(dims);
+    if (L.total > kStaticWorkspaceBytes) throw HipError("volume too large for the static workspace");
+    unsigned char* base = static_cast<unsigned char*>(static_workspace());
+    // the network's own workspace stands aside for this call (kernel
+    // arguments are taken at launch, so it is restored right after)
+    uint16_t* saved_T[kTensorCount];
+    std::memcpy(saved_T, net.T, sizeof(saved_T));
+    float* saved_rn = net.rn;
+    float* saved_z = net.z;
+    double* saved_sums = net.sums;
+    double* saved_partials = net.partials;
+    auto restore = [&] {
+        std::memcpy(net.T, saved_T, sizeof(saved_T));
+        net.rn = saved_rn;
+        net.z = saved_z;
+        net.sums = saved_sums;
+        net.partials = saved_partials;
+    };
+    for (int t = 0; t < kTensorCount; ++t) net.T[t] = reinterpret_cast<uint16_t*>(base + L.tensor[t]);
+    net.rn = reinterpret_cast<float*>(base + L.rn);
+    net.z = reinterpret_cast<float*>(base + L.z);
+    net.sums = reinterpret_cast<double*>(base + L.sums);
+    net.partials = reinterpret_cast<double*>(base + L.partials);
+    try {
+        pack_all(net, s);
+        const Dims d = level_dims(dims);
+        const int64_t cells = voxels(d, 0, 1);
+        prepare_input(net, r, flags, 1, cells, s);
+        forward(net, 1, d, s);
+        output_kernel<<<grid_for(cells), 256, 0, s>>>(net.z, flags, cells, net.sums, z);
+        hip_check(hipGetLastError(), "preconditioner output");
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
+}
+
 void preconditioner_train_step(SmokeNnPreconditioner& net, const float* r, const uint8_t* flags, int batch, const int dims[3], float lr, double* loss,
                                float* host_grads, hipStream_t s) {
     ensure_workspace(net, batch, dims, true);
@@ -572,6 +666,17 @@ SmokeSimulationResult smoke_nn_preconditioner_apply(SmokeNnPreconditioner* net, 
                                                     void* stream) {
     if (!net || !r || !flags || !z || !dims || dims[0] < 4 || dims[1] < 4 || dims[2] < 4) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     SMOKE_NN_GUARD(smoke::preconditioner_apply(*net, r, flags, dims, z, static_cast<hipStream_t>(stream)))
+}
+
+SmokeSimulationResult smoke_nn_preconditioner_apply_static(SmokeNnPreconditioner* net, const float* r, const uint8_t* flags, const int32_t dims[3],
+                                                           float* z, void* stream) {
+    if (!net || !r || !flags || !z || !dims || dims[0] < 4 || dims[1] < 4 || dims[2] < 4) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    if (smoke::static_layout(dims).total > kStaticWorkspaceBytes) {
+        std::fprintf(stderr, "smoke_nn_preconditioner_apply_static: a %dx%dx%d volume needs more than the %zu-byte static workspace\n", dims[0],
+                     dims[1], dims[2], kStaticWorkspaceBytes);
+        return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    }
+    SMOKE_NN_GUARD(smoke::preconditioner_apply_static(*net, r, flags, dims, z, static_cast<hipStream_t>(stream)))
 }
 
 }  // extern "C"

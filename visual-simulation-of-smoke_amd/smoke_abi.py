@@ -568,6 +568,7 @@ class LearnedPreconditioner:
         L.smoke_nn_preconditioner_load.argtypes = [V, C.c_char_p]
         L.smoke_nn_preconditioner_train_step.argtypes = [V, V, V, C.c_int32, C.POINTER(C.c_int32), C.c_float, C.POINTER(C.c_double), V, V]
         L.smoke_nn_preconditioner_apply.argtypes = [V, V, V, C.POINTER(C.c_int32), V, V]
+        L.smoke_nn_preconditioner_apply_static.argtypes = [V, V, V, C.POINTER(C.c_int32), V, V]
         L.smoke_nn_preconditioner_set_loss.argtypes = [V, C.c_uint32]
         self.handle = V()
         check(L.smoke_nn_preconditioner_create(seed, C.byref(self.handle)), "preconditioner create")
@@ -606,6 +607,10 @@ class LearnedPreconditioner:
     def apply(self, d_r: int, d_flags: int, dims, d_z: int, stream=None):
         """z = M(r) on one device volume of dims (W, H, D)."""
         check(self.lib.smoke_nn_preconditioner_apply(self.handle, d_r, d_flags, (C.c_int32 * 3)(*dims), d_z, stream), "apply")
+
+    def apply_static(self, d_r: int, d_flags: int, dims, d_z: int, stream=None) -> int:
+        """The allocation-free z = M(r) (static device workspace); returns the result code."""
+        return self.lib.smoke_nn_preconditioner_apply_static(self.handle, d_r, d_flags, (C.c_int32 * 3)(*dims), d_z, stream)
 
     def train_step(self, d_r: int, d_flags: int, batch: int, dims, lr: float) -> float:
         """One Adam step on `batch` device volumes (dims = (W, H, D)); returns the loss."""
