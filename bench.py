@@ -195,44 +195,48 @@ def cpu_baseline(n: int, scene: str):
     }
 
 
-def batch_levels(cfg: abi.Config) -> int:
-    """V-cycle depth whose coarsest level keeps at least 4 cells per axis."""
-    m, levels = min(cfg.nx, cfg.ny, cfg.nz), 1
-    while m >= 8 and levels < 7:
-        m, levels = (m + 1) // 2, levels + 1
-    return levels
-
-
-def run_batch(count: int, steps: int, warmup: int, lanes_max: int = 4, modes=("fixed", "tolerance")):
+def run_batch(count: int, steps: int, warmup: int, lanes: int = 16, modes=("fixed", "tolerance"), paths=("fused", "per_plume")):
     """The batched-plume workload (batch_plumes.plume_set(count, seed=3)):
-    steps/s of the whole batch on `lanes_max` lanes and on 1, in the reference's
-    fixed-iteration mode (64 plain CG iterations, one captured graph per
-    plume) and to tolerance 1e-6 (FPCG + V(4,4)-cycle)."""
+    steps/s of the whole batch, fused (one launch per kernel over all plumes;
+    fixed mode as one captured graph) against each plume's own step on
+    `lanes` streams, in the reference's fixed-iteration mode (64 plain CG
+    iterations) and to tolerance 1e-6 with Jacobi PCG (the plain method)."""
     import batch_plumes as bp
     plumes = bp.plume_set(count, seed=3)
     cells = bp.total_cells(plumes)
-    out = {"plumes": count, "cells": cells, "grids": sorted({f"{c.nx}x{c.ny}x{c.nz}" for c, _ in plumes}), "steps": steps}
+    out = {"plumes": count, "cells": cells, "grids": sorted({f"{c.nx}x{c.ny}x{c.nz}" for c, _ in plumes}), "steps": steps,
+           "lanes": lanes}
     for mode in modes:
-        for lanes in (lanes_max, 1):
+        for fused in [p == "fused" for p in paths]:
             with abi.SmokeBatch([c for c, _ in plumes], lanes=lanes) as batch:
-                for (cfg, _), p in zip(plumes, batch.plumes):
+                batch.set_fusion(fused)
+                for p in batch.plumes:
                     if mode == "tolerance":
-                        p.set_multilevel_options(levels=batch_levels(cfg), pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0,
-                                                 coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=8)
-                        solver_options(p, 32, preconditioner=abi.PRECONDITIONER_MULTILEVEL)
+                        solver_options(p, 32, warm_start=2)
                 srcs = [s for _, s in plumes]
                 for k in range(warmup):
                     batch.step(srcs, k)
                 batch.synchronize()
+                iters = 0
                 t0 = time.perf_counter()
                 for k in range(warmup, warmup + steps):
                     batch.step(srcs, k)
+                    if mode == "tolerance":
+                        iters += sum(p.stats().iterations for p in batch.plumes)
                 batch.synchronize()
                 dt = time.perf_counter() - t0
-            out[f"{mode}_lanes{lanes}"] = {"batch_steps_per_s": round(steps / dt, 3), "mcells_per_s": round(cells * steps / dt / 1e6, 1)}
-        many = out[f"{mode}_lanes{lanes_max}"]
-        out[f"{mode}_lane_speedup"] = round(many["batch_steps_per_s"] / out[f"{mode}_lanes1"]["batch_steps_per_s"], 3)
-        print(f"batch {mode}: {many} on {lanes_max} lanes, {out[f'{mode}_lanes1']} on 1", file=sys.stderr, flush=True)
+                info = batch.info()
+            key = f"{mode}_{'fused' if fused else 'per_plume'}"
+            out[key] = {"batch_steps_per_s": round(steps / dt, 3), "mcells_per_s": round(cells * steps / dt / 1e6, 1)}
+            if mode == "tolerance":
+                out[key]["pcg_iterations_per_plume_step"] = round(iters / (steps * count), 1)
+            if fused:
+                out[key]["launches_per_step"] = int(info.launches_per_step)
+                out[key]["launches_per_cg_iteration"] = int(info.launches_per_iteration)
+                out[key]["graph"] = bool(info.graph)
+            print(f"batch {key}: {out[key]}", file=sys.stderr, flush=True)
+        if f"{mode}_fused" in out and f"{mode}_per_plume" in out:
+            out[f"{mode}_fusion_speedup"] = round(out[f"{mode}_fused"]["batch_steps_per_s"] / out[f"{mode}_per_plume"]["batch_steps_per_s"], 3)
     return out
 
 
@@ -270,6 +274,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-lanes", type=int, default=16)
     ap.add_argument("--batch-modes", default="fixed,tolerance", help="comma-separated, from fixed, tolerance")
+    ap.add_argument("--batch-paths", default="fused,per_plume",
+                    help="comma-separated, from fused (one launch per kernel over all plumes) and per_plume (each plume's own step)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -375,7 +381,8 @@ def main():
     }
     if args.batch_plumes > 0:
         modes = tuple(m.strip() for m in args.batch_modes.split(",") if m.strip())
-        out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2, args.batch_lanes, modes)
+        paths = tuple(p.strip() for p in args.batch_paths.split(",") if p.strip())
+        out["batched_plumes"] = run_batch(args.batch_plumes, args.batch_steps, 2, args.batch_lanes, modes, paths)
     if not args.no_cpu_baseline and n == 256:
         print("bench: CPU baseline (one whole 256^3 step on the host cores)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.scene)

@@ -407,20 +407,37 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_destroy_batch(Sm
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_batch_context(SmokeSimulationBatch batch, uint32_t index, SmokeSimulationContext* out_context,
                                                                             int32_t* out_lane);
 /* One step of every plume; with `sources` (one per plume, or NULL) each
-   plume's plume_sources(sources[i], step) is enqueued just before its step.
-   When every plume is in FIXED_ITERATIONS mode (kernel timing off), the
-   lanes run each plume's step up to its CG iterations, a solve stream of the
-   batch runs the iterations of all plumes as one launch per pass for each
-   group of plumes with the same launch shapes and iteration count, and the
-   lanes finish the steps; the call returns without waiting.  When any plume
-   solves to a tolerance (its step waits on its solve), one host thread per
-   lane drives that lane's plumes and the call returns when every lane has
-   enqueued its last step.  A learned network
-   attached to plumes on two lanes would run on two streams at once: such a
-   batch fails here (attach one network per lane). */
+   plume's plume_sources(sources[i], step) comes first.
+   When every plume is in FIXED_ITERATIONS mode, or every plume solves to a
+   tolerance with Jacobi or no preconditioner (no network, kernel timing off),
+   the step is fused: each kernel of the step is one launch over all plumes on
+   a stream of the batch (after the lanes' earlier work; the lanes wait for it),
+   in fixed mode as one captured graph (the call returns without waiting), in
+   tolerance mode with every plume's solve stopping on the device and the
+   refinement passes over the plumes still open (the call returns once every
+   solve has finished).  Otherwise each plume's own step is enqueued in turn on
+   its lane from the calling thread.  Either way a plume's results are bitwise
+   those of a context created alone and driven the same way. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_step_batch(SmokeSimulationBatch batch, const SmokeSimulationPlumeSource* sources,
                                                                          uint32_t step);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_synchronize_batch(SmokeSimulationBatch batch);
+
+/* How the last step_batch ran: fused = 0 (per-plume steps), 1 (fused, fixed
+   iterations, one graph), 2 (fused, tolerance solves); the fused step's
+   launches (stages other than the solve's iterations, right-hand side
+   included) and launches per CG iteration, each over every plume. */
+typedef struct SmokeSimulationBatchInfo {
+    int32_t fused;
+    int32_t graph;
+    uint64_t steps;
+    uint64_t launches_per_step;
+    uint64_t launches_per_iteration;
+} SmokeSimulationBatchInfo;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_batch_info(SmokeSimulationBatch batch, SmokeSimulationBatchInfo* out_info);
+/* enable = 0 steps every plume's own context in turn even where the fused
+   step applies (for comparison); 1 (the default) fuses where it can. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_batch_fusion(SmokeSimulationBatch batch, int32_t enable);
 /* The lane assignment rule above, on the host (no device needed):
    out_lane[i] for plume i of `count` plumes with cells[i] cells. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_batch_lanes(const uint64_t* cells, uint32_t count, int32_t lanes, int32_t* out_lane);

@@ -1,10 +1,12 @@
 """Batched plumes (csrc/batch.hip) through the C ABI: every plume of a batch
-of uneven grids, stepped on several lanes, equals bitwise a context created
-alone from the same description and driven the same way -- in the reference's
-fixed-iteration mode (the CG iterations of all plumes batched into one launch
-per pass and group of plumes), in the tolerance mode (one host thread per
-lane) with Jacobi PCG and with the multilevel preconditioner -- and a network
-attached on two lanes is refused."""
+of uneven grids equals bitwise a context created alone from the same
+description and driven the same way -- in the reference's fixed-iteration
+mode (the fused step: one launch per kernel over all plumes, captured as one
+graph), in the tolerance mode with Jacobi PCG (fused: per-plume convergence on
+the device, gated refinement passes) and with the multilevel preconditioner
+(each plume's own step in turn) -- on five small uneven grids and on the
+64-plume set of the batched workload; and a network attached on two lanes is
+refused."""
 import numpy as np
 import pytest
 
@@ -49,11 +51,10 @@ def _configure(ctx, mode):
         tolerance_options(ctx, preconditioner=abi.PRECONDITIONER_MULTILEVEL)
 
 
-@pytest.mark.parametrize("mode,lanes", [("fixed", 4), ("tolerance", 3), ("multilevel", 2)])
-def test_batch_plumes_equal_standalone_contexts(mode, lanes):
-    scenes = _scenes()
-    steps = 3
-    print(f"batch: mode {mode} lanes {lanes}", flush=True)
+FUSED = {"fixed": 1, "tolerance": 2, "multilevel": 0}
+
+
+def _run_batch(scenes, mode, lanes, steps):
     with abi.SmokeBatch([c for c, _ in scenes], lanes=lanes) as batch:
         cells = [c.nx * c.ny * c.nz for c, _ in scenes]
         assert batch.lanes == abi.batch_lanes(cells, lanes)
@@ -62,8 +63,22 @@ def test_batch_plumes_equal_standalone_contexts(mode, lanes):
         for k in range(steps):
             batch.step([s for _, s in scenes], k)
         batch.synchronize()
+        info = batch.info()
         got = [p.export_state() for p in batch.plumes]
         stats = [p.stats() for p in batch.plumes]
+    assert info.fused == FUSED[mode] and info.steps == steps
+    assert info.graph == (1 if mode == "fixed" else 0)
+    if info.fused:
+        print(f"batch {mode}: {info.launches_per_step} launches per step, {info.launches_per_iteration} per CG iteration", flush=True)
+    return got, stats
+
+
+@pytest.mark.parametrize("mode,lanes", [("fixed", 4), ("tolerance", 3), ("multilevel", 2)])
+def test_batch_plumes_equal_standalone_contexts(mode, lanes):
+    scenes = _scenes()
+    steps = 3
+    print(f"batch: mode {mode} lanes {lanes}", flush=True)
+    got, stats = _run_batch(scenes, mode, lanes, steps)
     for i, (cfg, src) in enumerate(scenes):
         with abi.SmokeContext(cfg) as ctx:
             _configure(ctx, mode)
@@ -92,6 +107,31 @@ def test_batch_refuses_a_network_on_two_lanes():
             p.set_learned_preconditioner(None)
         batch.step([s for _, s in scenes], 0)
         batch.synchronize()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", ["fixed", "tolerance"])
+def test_plume_set_64_fused_step_equals_lone_contexts(mode):
+    """The 64-plume set of the batched workload (sizes 32^3 to 128^3, seed 3):
+    two fused batch steps, each plume bitwise equal to its own context driven
+    the same way."""
+    plumes = bp.plume_set(64, seed=3)
+    steps = 2
+    print(f"plume set 64: mode {mode}", flush=True)
+    got, stats = _run_batch(plumes, mode, 16, steps)
+    for i, (cfg, src) in enumerate(plumes):
+        with abi.SmokeContext(cfg) as ctx:
+            _configure(ctx, mode)
+            for k in range(steps):
+                ctx.plume_sources(src, k)
+                ctx.step()
+            ref = ctx.export_state()
+            st = ctx.stats()
+        for f in FIELDS:
+            np.testing.assert_array_equal(got[i][f], ref[f], err_msg=f"plume {i} {f}")
+        assert stats[i].iterations == st.iterations and stats[i].refinements == st.refinements
+        if mode == "tolerance":
+            assert stats[i].relative_residual == st.relative_residual and st.relative_residual <= 1e-6
 
 
 def test_plume_set_batch_runs_to_tolerance():

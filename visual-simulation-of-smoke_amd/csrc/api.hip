@@ -380,23 +380,6 @@ std::string read_checkpoint(const char* path, const int32_t* want, CheckpointFil
 
 }  // namespace
 
-// A fixed-iteration step split around its K CG iterations, for the batched
-// solve of many contexts: together with the iterations they enqueue exactly
-// what the step's graph holds (stages 0-1, the right-hand side and the CG
-// start; then stages 3-4) and keep the step's statistics.
-void enqueue_fixed_step_before_iterations(Context& c) {
-    enqueue_stages(c, 0, 1);
-    enqueue_pressure_rhs(c);
-    enqueue_pcg_init(c, false);
-}
-
-void enqueue_fixed_step_after_iterations(Context& c) {
-    enqueue_stages(c, 3, 4);
-    c.stats.iterations = c.config.pressure_iterations;
-    c.stats.steps += 1;
-    c.diagnostics_dirty = true;
-}
-
 }  // namespace smoke
 
 using smoke::Context;

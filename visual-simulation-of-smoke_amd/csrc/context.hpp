@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "batched.hpp"
 #include "device_common.hpp"
 
 struct SmokeNnPreconditioner;
@@ -178,6 +179,16 @@ struct Context {
     hipEvent_t timing_events[2][6] = {};
 };
 
+// Batched stages (step_kernels.hip): the launches of the stages other than
+// the pressure solve for a batch of contexts, one launch per kernel (and per
+// launch form of the plumes' advection kernels), tables in `arena`.
+struct BatchedStages {
+    std::vector<BatchedLaunch> time_step, forces, advect, project, scalars;
+};
+void build_batched_stages(const std::vector<Context*>& plumes, TableArena& arena, BatchedStages& out);
+void build_batched_sources(const std::vector<Context*>& plumes, const SmokeSimulationPlumeSource* sources, uint32_t step, TableArena& arena,
+                           std::vector<BatchedLaunch>& out);
+
 // Launch helpers implemented in the .hip translation units.
 void enqueue_fill(Context& c, float* p, float value, int64_t n);
 void enqueue_fill_u8(Context& c, uint8_t* p, uint8_t value, int64_t n);
@@ -200,16 +211,20 @@ void enqueue_fixed_pcg(Context& c);                        // reference: K plain
 void run_tolerance_pcg(Context& c);                        // host-polled tolerance solve
 void enqueue_true_residual(Context& c);                    // rr_true, bb for the FP32 pressure
 
-// Batched fixed-iteration solves (pcg.hip, api.hip): contexts in
-// FIXED_ITERATIONS mode with the same stencil launch shapes and iteration
-// count run their K CG iterations as one launch per pass.  The table image
-// (fill_batched_table) goes to device memory of batched_table_bytes(n); the
-// step around the iterations is enqueue_fixed_step_before/after_iterations.
-size_t batched_table_bytes(size_t count);
-void fill_batched_table(const std::vector<Context*>& plumes, std::vector<unsigned char>& image);
-void enqueue_batched_iterations(const std::vector<Context*>& plumes, const void* table, hipStream_t s);
-void enqueue_fixed_step_before_iterations(Context& c);
-void enqueue_fixed_step_after_iterations(Context& c);
+// Batched solves (pcg.hip): the pressure stage of a batch of contexts in
+// Jacobi / plain-CG modes, one launch per pass (and per launch shape of the
+// contexts' stencil passes).  Static launches are built once per batch plan;
+// the warm start, residual and pressure-from-x64 launches name the contexts'
+// current FP64 buffers and are built per solve phase (the warm start rotates
+// the buffer roles as run_tolerance_pcg does).
+struct BatchedSolve {
+    std::vector<BatchedLaunch> rhs, iteration, fixed_start, gather;
+};
+void build_batched_solve(const std::vector<Context*>& plumes, PcgScalars* gathered, TableArena& arena, BatchedSolve& out);
+void build_batched_warm_start(const std::vector<Context*>& plumes, TableArena& arena, std::vector<BatchedLaunch>& out);
+void build_batched_residual(const std::vector<Context*>& plumes, bool refine, TableArena& arena, std::vector<BatchedLaunch>& out);
+void build_batched_zero_x64(const std::vector<Context*>& plumes, TableArena& arena, std::vector<BatchedLaunch>& out);
+void build_batched_pressure_from_x64(const std::vector<Context*>& plumes, TableArena& arena, std::vector<BatchedLaunch>& out);
 void configure_pcg(Context& c);
 double measure_copy(uint64_t bytes, int reps, double* ms_per_copy);
 

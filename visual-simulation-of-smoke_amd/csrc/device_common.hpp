@@ -24,16 +24,35 @@ constexpr unsigned kXcds = 8;
 
 inline unsigned xcd_blocks(const dim3& tiles) { return (tiles.x * tiles.y * tiles.z + kXcds - 1) / kXcds * kXcds; }
 
-__device__ __forceinline__ bool xcd_tile(const uint3& tiles, uint3& tile, unsigned& id) {
+// Block index of a kernel body: the hardware's in an ordinary launch, a
+// grid's own numbering inside a batched launch over many grids (batched.hpp).
+struct VBlock {
+    uint3 idx;   // block index in the grid's own launch
+    uint3 dim;   // the grid's own launch dimensions
+};
+
+__device__ __forceinline__ VBlock hw_block() {
+    VBlock b;
+    b.idx = make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    b.dim = make_uint3(gridDim.x, gridDim.y, gridDim.z);
+    return b;
+}
+
+__device__ __forceinline__ bool xcd_tile_of(unsigned block, const uint3& tiles, uint3& tile, unsigned& id) {
     const unsigned total = tiles.x * tiles.y * tiles.z;
     const unsigned per = (total + kXcds - 1) / kXcds;
-    id = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    id = (block % kXcds) * per + block / kXcds;
     if (id >= total) return false;
     tile.x = id % tiles.x;
     const unsigned rest = id / tiles.x;
     tile.y = rest % tiles.y;
     tile.z = rest / tiles.y;
     return true;
+}
+
+__device__ __forceinline__ bool xcd_tile(const uint3& tiles, uint3& tile, unsigned& id) { return xcd_tile_of(blockIdx.x, tiles, tile, id); }
+__device__ __forceinline__ bool xcd_tile(const VBlock& vb, const uint3& tiles, uint3& tile, unsigned& id) {
+    return xcd_tile_of(vb.idx.x, tiles, tile, id);
 }
 
 // Boundary kinds (values of the ABI enums).

@@ -76,18 +76,6 @@ struct PcgArgs {
     double* partials;
 };
 
-// One context of a batched fixed-iteration solve: its arguments with the
-// tiles of the pass-2 and pass-3 launch shapes, its vectors, and where its
-// blocks start in each batched launch.
-struct BatchedPlume {
-    PcgArgs a[2];
-    float* p;
-    float* e;
-    float* r;
-    int update_first, update_blocks;
-    int stencil_first[2], stencil_blocks[2];
-};
-
 // OUTFLOW faces of cell (x, y, z) (each adds 1 to the diagonal of a fluid row).
 __device__ __forceinline__ int outflow_faces(const PcgArgs& a, int x, int y, int z) {
     if (!a.any_outflow) return 0;
@@ -563,44 +551,48 @@ __device__ __forceinline__ void beta_body(const PcgArgs& a, int nblocks) {
 __global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_kernel(PcgArgs a, int nblocks) { alpha_body(a, nblocks); }
 __global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_kernel(PcgArgs a, int nblocks) { beta_body(a, nblocks); }
 
-// ---- batched fixed-iteration solves (batched plumes)
-// One launch per pass for many contexts that share the stencil launch shapes
-// and the iteration count.  A block finds its context in the table by the
-// context's first block in the launch and runs the pass body with the
-// context's own block numbering, so every context computes exactly what its
-// own launches would.
-template <int W>   // 0: update pass; 1, 2: stencil passes 2 and 3
-__device__ __forceinline__ int plume_of(const BatchedPlume* __restrict__ t, int count, int b) {
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int first = W == 0 ? t[mid].update_first : t[mid].stencil_first[W - 1];
-        if (first <= b) lo = mid;
-        else hi = mid - 1;
+// ---- the pass bodies as functors of batched launches over many contexts
+// (batched.hpp): each context's blocks run with its own numbering
+template <bool JACOBI>
+struct update_p_k {
+    __device__ __forceinline__ static void run(const VBlock& vb, PcgArgs a, const float* r, float* p) { update_p_body<JACOBI>(a, r, p, int(vb.idx.x)); }
+};
+
+template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
+struct stencil_k {
+    __device__ __forceinline__ static void run(const VBlock& vb, PcgArgs a, const float* p, float* e, float* r) {
+        stencil_body<TXV, TY, ZR, VEC, MODE, JACOBI>(a, p, e, r, nullptr, int(vb.idx.x));
     }
-    return lo;
-}
+};
 
-__global__ __launch_bounds__(256) void pcg_update_p_batched_kernel(const BatchedPlume* __restrict__ t, int count) {
-    const BatchedPlume& q = t[plume_of<0>(t, count, int(blockIdx.x))];
-    update_p_body<false>(q.a[0], q.r, q.p, int(blockIdx.x) - q.update_first);
-}
+struct alpha_k {
+    __device__ __forceinline__ static void run(const VBlock&, PcgArgs a, int nblocks) { alpha_body(a, nblocks); }
+};
 
-template <int TXV, int TY, int ZR, int VEC, int MODE>
-__global__ __launch_bounds__(TXV* TY) void pcg_stencil_batched_kernel(const BatchedPlume* __restrict__ t, int count) {
-    const BatchedPlume& q = t[plume_of<MODE + 1>(t, count, int(blockIdx.x))];
-    stencil_body<TXV, TY, ZR, VEC, MODE, false>(q.a[MODE], q.p, q.e, q.r, nullptr, int(blockIdx.x) - q.stencil_first[MODE]);
-}
+struct beta_k {
+    __device__ __forceinline__ static void run(const VBlock&, PcgArgs a, int nblocks) { beta_body(a, nblocks); }
+};
 
-__global__ __launch_bounds__(kFinalizeThreads) void pcg_alpha_batched_kernel(const BatchedPlume* __restrict__ t) {
-    const BatchedPlume& q = t[blockIdx.x];
-    alpha_body(q.a[0], q.stencil_blocks[0]);
-}
+// the scalar block of one context into the batch's array (one thread)
+struct gather_scalars_k {
+    __device__ __forceinline__ static void run(const VBlock&, const PcgScalars* src, PcgScalars* dst) {
+        if (threadIdx.x == 0) *dst = *src;
+    }
+};
 
-__global__ __launch_bounds__(kFinalizeThreads) void pcg_beta_batched_kernel(const BatchedPlume* __restrict__ t) {
-    const BatchedPlume& q = t[blockIdx.x];
-    beta_body(q.a[1], q.stencil_blocks[1]);
-}
+struct zero64_k {
+    __device__ __forceinline__ static void run(const VBlock& vb, double* x, int64_t n) {
+        const int64_t i = int64_t(vb.idx.x) * 256 + threadIdx.x;
+        if (i < n) x[i] = 0.0;
+    }
+};
+
+struct copy64_k {
+    __device__ __forceinline__ static void run(const VBlock& vb, const double* src, double* dst, int64_t n) {
+        const int64_t i = int64_t(vb.idx.x) * 256 + threadIdx.x;
+        if (i < n) dst[i] = src[i];
+    }
+};
 
 // ---- flexible PCG (learned preconditioner): z is a stored vector
 // p <- z (first) or z + beta p
@@ -659,14 +651,14 @@ __global__ __launch_bounds__(kFinalizeThreads) void fpcg_beta_kernel(PcgArgs a, 
 
 // r = b - A x' with x' = x_old + e_add (either may be absent) evaluated in
 // FP64; optionally stores x' (FP64) and starts a fresh inner solve.
-__global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
+__device__ __forceinline__ void residual_body(const VBlock& vb, PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
                                                        const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
                                                        const PcgScalars* __restrict__ S) {
     // tiles of 64 (x) x 4 (y) columns by kResidualPlanes z planes, in the
     // XCD-aware order (neighbouring tiles' rows are read through one L2)
     uint3 tile;
     unsigned id;
-    if (!xcd_tile(tiles, tile, id)) return;
+    if (!xcd_tile(vb, tiles, tile, id)) return;
     const int nblocks = int(tiles.x * tiles.y * tiles.z);
     const int bid = int(id);
     const int x = int(tile.x) * 64 + (threadIdx.x & 63);
@@ -714,18 +706,27 @@ __global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, uint3 tiles, c
     store_block_partials<3, 4>(v, a.partials, nblocks, bid);
 }
 
+__global__ __launch_bounds__(256) void residual_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
+                                                       const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
+                                                       const PcgScalars* __restrict__ S) { residual_body(hw_block(), a, tiles, b, x_old, e_add, x_new, r, S); }
+
+struct residual_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { residual_body(vb, a...); }
+};
+
 // The same pass with float4 rows (nx % 4 == 0): a thread owns 4 consecutive
 // x cells of a row and marches kResidualPlanes z planes, keeping the rows at
 // z - 1, z and z + 1 of x' in registers; the rows at y +- 1 are vector loads,
 // the cells at x - 1 and x + 4 scalar loads.  Every cell subtracts its
 // neighbours in the order of residual_kernel, so r and x' are the same; only
 // the grouping of the FP64 partial sums differs.
-__global__ __launch_bounds__(256) void residual4_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
+__device__ __forceinline__ void residual4_body(const VBlock& vb, PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
                                                         const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
                                                         const PcgScalars* __restrict__ S) {
     uint3 tile;
     unsigned id;
-    if (!xcd_tile(tiles, tile, id)) return;
+    if (!xcd_tile(vb, tiles, tile, id)) return;
     const int nblocks = int(tiles.x * tiles.y * tiles.z);
     const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
     const int x0 = (int(tile.x) * 64 + (threadIdx.x & 63)) * 4;
@@ -818,9 +819,18 @@ __global__ __launch_bounds__(256) void residual4_kernel(PcgArgs a, uint3 tiles, 
     store_block_partials<3, 4>(v, a.partials, nblocks, int(id));
 }
 
+__global__ __launch_bounds__(256) void residual4_kernel(PcgArgs a, uint3 tiles, const float* __restrict__ b, const double* __restrict__ x_old,
+                                                        const float* __restrict__ e_add, double* __restrict__ x_new, float* __restrict__ r,
+                                                        const PcgScalars* __restrict__ S) { residual4_body(hw_block(), a, tiles, b, x_old, e_add, x_new, r, S); }
+
+struct residual4_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { residual4_body(vb, a...); }
+};
+
 // Totals of the residual kernel: true ||r||^2, r.z, ||b||^2; starts a fresh
 // inner solve (p = z at the next iteration) with its stop threshold.
-__global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(PcgArgs a, int nblocks, int compute_bb, int reset_iterations) {
+__device__ __forceinline__ void residual_finalize_body(const VBlock& vb, PcgArgs a, int nblocks, int compute_bb, int reset_iterations) {
     PcgScalars* __restrict__ S = a.s;
     double tot[3];
     sum_partials<3>(a.partials, nblocks, tot);
@@ -840,16 +850,37 @@ __global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(Pcg
     }
 }
 
+__global__ __launch_bounds__(kFinalizeThreads) void residual_finalize_kernel(PcgArgs a, int nblocks, int compute_bb, int reset_iterations) { residual_finalize_body(hw_block(), a, nblocks, compute_bb, reset_iterations); }
+
+struct residual_finalize_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { residual_finalize_body(vb, a...); }
+};
+
 // Warm start extrapolated from the two previous solutions: g = 2 x_n - x_{n-1}.
-__global__ void extrapolate_kernel(const double* __restrict__ x, const double* __restrict__ prev, double* __restrict__ g, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void extrapolate_body(const VBlock& vb, const double* __restrict__ x, const double* __restrict__ prev, double* __restrict__ g, int64_t n) {
+    const int64_t i = int64_t(vb.idx.x) * blockDim.x + threadIdx.x;
     if (i < n) g[i] = 2.0 * x[i] - prev[i];
 }
 
-__global__ void pressure_from_x64_kernel(const double* __restrict__ x, float* __restrict__ p, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ void extrapolate_kernel(const double* __restrict__ x, const double* __restrict__ prev, double* __restrict__ g, int64_t n) { extrapolate_body(hw_block(), x, prev, g, n); }
+
+struct extrapolate_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { extrapolate_body(vb, a...); }
+};
+
+__device__ __forceinline__ void pressure_from_x64_body(const VBlock& vb, const double* __restrict__ x, float* __restrict__ p, int64_t n) {
+    const int64_t i = int64_t(vb.idx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = float(x[i]);
 }
+
+__global__ void pressure_from_x64_kernel(const double* __restrict__ x, float* __restrict__ p, int64_t n) { pressure_from_x64_body(hw_block(), x, p, n); }
+
+struct pressure_from_x64_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { pressure_from_x64_body(vb, a...); }
+};
 
 __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -857,11 +888,11 @@ __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__
 }
 
 // b = -(h^2/dt) div(u*) + sum of OUTFLOW face pressures, 0 on identity rows (B:1304-1328).
-__global__ void pressure_rhs_kernel(Grid g, const float* __restrict__ us, const float* __restrict__ vs, const float* __restrict__ ws,
+__device__ __forceinline__ void pressure_rhs_body(const VBlock& vb, Grid g, const float* __restrict__ us, const float* __restrict__ vs, const float* __restrict__ ws,
                                     const uint8_t* __restrict__ flags, const float* __restrict__ dt_ptr, float* __restrict__ rhs) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const int x = vb.idx.x * 64 + (threadIdx.x & 63);
+    const int y = vb.idx.y * 4 + (threadIdx.x >> 6);
+    const int z = vb.idx.z;
     if (x >= g.n[0] || y >= g.n[1]) return;
     const int64_t i = cidx(g, x, y, z);
     if (flags[i] & kIdentity) {
@@ -881,6 +912,14 @@ __global__ void pressure_rhs_kernel(Grid g, const float* __restrict__ us, const 
     if (z == g.n[2] - 1 && g.flow_type[5] == kOutflow) bsum += g.flow_pressure[5];
     rhs[i] = -(h * h / dt) * div + bsum;
 }
+
+__global__ void pressure_rhs_kernel(Grid g, const float* __restrict__ us, const float* __restrict__ vs, const float* __restrict__ ws,
+                                    const uint8_t* __restrict__ flags, const float* __restrict__ dt_ptr, float* __restrict__ rhs) { pressure_rhs_body(hw_block(), g, us, vs, ws, flags, dt_ptr, rhs); }
+
+struct pressure_rhs_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { pressure_rhs_body(vb, a...); }
+};
 
 PcgArgs make_args(const Context& c) {
     PcgArgs a{};
@@ -1402,83 +1441,190 @@ double measure_copy(uint64_t bytes, int reps, double* ms_per_copy) {
     return per > 0 ? 2.0 * double(n4) * 16.0 / (per * 1e-3) / 1e9 : 0.0;
 }
 
-size_t batched_table_bytes(size_t count) { return count * sizeof(BatchedPlume); }
+// ---- batched solves (batch.hip): the pressure stage of many contexts as
+// one launch per pass.  Static launches (right-hand side, the iteration
+// passes, the fixed-mode start, the scalar gather) are built once per batch
+// plan; the launches that name a context's current FP64 solution buffers are
+// built per solve phase.
+namespace {
 
-void fill_batched_table(const std::vector<Context*>& plumes, std::vector<unsigned char>& image) {
-    image.assign(plumes.size() * sizeof(BatchedPlume), 0);
-    int update = 0, stencil[2] = {0, 0};
-    for (size_t i = 0; i < plumes.size(); ++i) {
-        Context& c = *plumes[i];
-        BatchedPlume q{};
-        for (int m = 0; m < 2; ++m) {
-            const StencilShape& sh = kStencilShapes[c.pcg_shape[m]];
-            PcgArgs a = make_args(c);
-            a.tiles[0] = (c.grid.n[0] + sh.txv * sh.vec - 1) / (sh.txv * sh.vec);
-            a.tiles[1] = (c.grid.n[1] + sh.ty - 1) / sh.ty;
-            a.tiles[2] = (c.grid.n[2] + sh.zr - 1) / sh.zr;
-            a.tile_div[0] = make_divisor(unsigned(a.tiles[2]));
-            a.tile_div[1] = make_divisor(unsigned(a.tiles[1]));
-            q.a[m] = a;
-            q.stencil_blocks[m] = a.tiles[0] * a.tiles[1] * a.tiles[2];
-            q.stencil_first[m] = stencil[m];
-            stencil[m] += q.stencil_blocks[m];
-        }
-        q.p = c.pcg_p;
-        q.e = c.pressure;   // fixed-iteration mode accumulates the solution in the pressure
-        q.r = c.pcg_r;
-        q.update_blocks = int((c.grid.cells + 1023) / 1024);
-        q.update_first = update;
-        update += q.update_blocks;
-        std::memcpy(image.data() + i * sizeof(BatchedPlume), &q, sizeof(q));
+template <int TXV, int TY, int ZR, int VEC, int MODE, bool JACOBI>
+void stencil_group(const std::vector<Context*>& P, const std::vector<unsigned>& members, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<stencil_k<TXV, TY, ZR, VEC, MODE, JACOBI>, TXV * TY, PcgArgs, const float*, float*, float*> L;
+    for (unsigned p : members) {
+        const Context& c = *P[p];
+        PcgArgs a = make_args(c);
+        a.tiles[0] = (c.grid.n[0] + TXV * VEC - 1) / (TXV * VEC);
+        a.tiles[1] = (c.grid.n[1] + TY - 1) / TY;
+        a.tiles[2] = (c.grid.n[2] + ZR - 1) / ZR;
+        a.tile_div[0] = make_divisor(unsigned(a.tiles[2]));
+        a.tile_div[1] = make_divisor(unsigned(a.tiles[1]));
+        float* e = c.options.pressure_mode == SMOKE_SIMULATION_PRESSURE_TOLERANCE ? c.pcg_e : c.pressure;
+        L.add(p, dim3(unsigned(a.tiles[0] * a.tiles[1] * a.tiles[2])), a, c.pcg_p, e, c.pcg_r);
     }
+    if (!L.empty()) out.push_back(L.finish(A, dim3(TXV * TY)));
 }
 
-template <int TXV, int TY, int ZR, int VEC, int MODE>
-void stencil_batched(unsigned blocks, const BatchedPlume* t, int count, hipStream_t s) {
-    pcg_stencil_batched_kernel<TXV, TY, ZR, VEC, MODE><<<blocks, TXV * TY, 0, s>>>(t, count);
-}
-
-template <int MODE>
-void launch_stencils_batched(int shape, unsigned blocks, const BatchedPlume* t, int count, hipStream_t s) {
-    switch (shape) {
-        case 0: stencil_batched<64, 4, 2, 4, MODE>(blocks, t, count, s); break;
-        case 1: stencil_batched<64, 8, 2, 4, MODE>(blocks, t, count, s); break;
-        case 2: stencil_batched<32, 8, 2, 4, MODE>(blocks, t, count, s); break;
-        case 3: stencil_batched<64, 4, 4, 4, MODE>(blocks, t, count, s); break;
-        case 4: stencil_batched<16, 16, 2, 4, MODE>(blocks, t, count, s); break;
-        case 5: stencil_batched<16, 8, 2, 4, MODE>(blocks, t, count, s); break;
-        case 6: stencil_batched<64, 4, 2, 1, MODE>(blocks, t, count, s); break;
-        case 7: stencil_batched<64, 8, 2, 1, MODE>(blocks, t, count, s); break;
-        case 8: stencil_batched<32, 8, 2, 1, MODE>(blocks, t, count, s); break;
-        default: throw HipError("invalid stencil shape");
-    }
-}
-
-void enqueue_batched_iterations(const std::vector<Context*>& plumes, const void* table, hipStream_t s) {
-    if (plumes.empty()) return;
-    const Context& c0 = *plumes.front();
-    const int count = int(plumes.size());
-    int64_t update = 0, stencil[2] = {0, 0};
-    for (const Context* c : plumes) {
-        if (c->options.pressure_mode != SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS || c->pcg_shape[0] != c0.pcg_shape[0] ||
-            c->pcg_shape[1] != c0.pcg_shape[1] || c->config.pressure_iterations != c0.config.pressure_iterations)
-            throw HipError("batched solve: contexts differ in mode, launch shapes or iterations");
-        update += (c->grid.cells + 1023) / 1024;
-        for (int m = 0; m < 2; ++m) {
-            const StencilShape& sh = kStencilShapes[c->pcg_shape[m]];
-            stencil[m] += int64_t((c->grid.n[0] + sh.txv * sh.vec - 1) / (sh.txv * sh.vec)) * ((c->grid.n[1] + sh.ty - 1) / sh.ty) *
-                          ((c->grid.n[2] + sh.zr - 1) / sh.zr);
+template <int MODE, bool JACOBI>
+void stencil_groups(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    for (int shape = 0; shape < kStencilShapeCount; ++shape) {
+        std::vector<unsigned> m;
+        for (unsigned p = 0; p < P.size(); ++p)
+            if (P[p]->pcg_shape[MODE] == shape && jacobi_on(*P[p]) == JACOBI) m.push_back(p);
+        if (m.empty()) continue;
+        switch (shape) {
+            case 0: stencil_group<64, 4, 2, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 1: stencil_group<64, 8, 2, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 2: stencil_group<32, 8, 2, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 3: stencil_group<64, 4, 4, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 4: stencil_group<16, 16, 2, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 5: stencil_group<16, 8, 2, 4, MODE, JACOBI>(P, m, A, out); break;
+            case 6: stencil_group<64, 4, 2, 1, MODE, JACOBI>(P, m, A, out); break;
+            case 7: stencil_group<64, 8, 2, 1, MODE, JACOBI>(P, m, A, out); break;
+            case 8: stencil_group<32, 8, 2, 1, MODE, JACOBI>(P, m, A, out); break;
+            default: throw HipError("invalid stencil shape");
         }
     }
-    const BatchedPlume* t = static_cast<const BatchedPlume*>(table);
-    for (int k = 0; k < c0.config.pressure_iterations; ++k) {
-        pcg_update_p_batched_kernel<<<unsigned(update), 256, 0, s>>>(t, count);
-        launch_stencils_batched<0>(c0.pcg_shape[0], unsigned(stencil[0]), t, count, s);
-        pcg_alpha_batched_kernel<<<unsigned(count), kFinalizeThreads, 0, s>>>(t);
-        launch_stencils_batched<1>(c0.pcg_shape[1], unsigned(stencil[1]), t, count, s);
-        pcg_beta_batched_kernel<<<unsigned(count), kFinalizeThreads, 0, s>>>(t);
+}
+
+template <bool JACOBI>
+void update_group(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<update_p_k<JACOBI>, 256, PcgArgs, const float*, float*> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        if (jacobi_on(c) != JACOBI) continue;
+        L.add(p, dim3(unsigned((c.grid.cells + 1023) / 1024)), make_args(c), c.pcg_r, c.pcg_p);
     }
-    hip_check(hipGetLastError(), "batched pcg iterations");
+    if (!L.empty()) out.push_back(L.finish(A, dim3(256)));
+}
+
+int stencil_blocks_of(const Context& c, int mode) {
+    return int(stencil_tiles(c, kStencilShapes[c.pcg_shape[mode]]));
+}
+
+// r = b - A (x_old + e_add) in FP64 (x_new = that sum when given), then the
+// totals; per context the residual kernel of its row width
+void residual_launches(const std::vector<Context*>& P, const std::vector<const double*>& x_old, const std::vector<const float*>& e_add,
+                       const std::vector<double*>& x_new, bool bb, bool reset, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<residual4_k, 256, PcgArgs, uint3, const float*, const double*, const float*, double*, float*, const PcgScalars*> R4;
+    LaunchBuilder<residual_k, 256, PcgArgs, uint3, const float*, const double*, const float*, double*, float*, const PcgScalars*> R1;
+    LaunchBuilder<residual_finalize_k, kFinalizeThreads, PcgArgs, int, int, int> F;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        const PcgArgs a = make_args(c);
+        const bool rows4 = c.grid.n[0] % 4 == 0;
+        const dim3 grid((c.grid.n[0] + (rows4 ? 255 : 63)) / (rows4 ? 256 : 64), (c.grid.n[1] + 3) / 4,
+                        (c.grid.n[2] + kResidualPlanes - 1) / kResidualPlanes);
+        const uint3 tiles = make_uint3(grid.x, grid.y, grid.z);
+        if (rows4) R4.add(p, dim3(xcd_blocks(grid)), a, tiles, c.pressure_rhs, x_old[p], e_add[p], x_new[p], c.pcg_r, c.scalars);
+        else R1.add(p, dim3(xcd_blocks(grid)), a, tiles, c.pressure_rhs, x_old[p], e_add[p], x_new[p], c.pcg_r, c.scalars);
+        F.add(p, dim3(1), a, int(grid.x * grid.y * grid.z), bb ? 1 : 0, reset ? 1 : 0);
+    }
+    if (!R4.empty()) out.push_back(R4.finish(A, dim3(256)));
+    if (!R1.empty()) out.push_back(R1.finish(A, dim3(256)));
+    out.push_back(F.finish(A, dim3(kFinalizeThreads)));
+}
+
+}  // namespace
+
+void build_batched_solve(const std::vector<Context*>& P, PcgScalars* gathered, TableArena& A, BatchedSolve& out) {
+    out = BatchedSolve{};
+    {
+        LaunchBuilder<pressure_rhs_k, 256, Grid, const float*, const float*, const float*, const uint8_t*, const float*, float*> L;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            L.add(p, dim3((c.grid.n[0] + 63) / 64, (c.grid.n[1] + 3) / 4, c.grid.n[2]), c.grid, c.face_star[0], c.face_star[1], c.face_star[2],
+                  c.stencil_flags, c.dt_device, c.pressure_rhs);
+        }
+        out.rhs.push_back(L.finish(A, dim3(256)));
+    }
+    // one iteration: p update, p.Ap, alpha, e / r update, beta
+    update_group<false>(P, A, out.iteration);
+    update_group<true>(P, A, out.iteration);
+    stencil_groups<0, false>(P, A, out.iteration);
+    stencil_groups<0, true>(P, A, out.iteration);
+    {
+        LaunchBuilder<alpha_k, kFinalizeThreads, PcgArgs, int> L;
+        for (unsigned p = 0; p < P.size(); ++p) L.add(p, dim3(1), make_args(*P[p]), stencil_blocks_of(*P[p], 0));
+        out.iteration.push_back(L.finish(A, dim3(kFinalizeThreads)));
+    }
+    stencil_groups<1, false>(P, A, out.iteration);
+    stencil_groups<1, true>(P, A, out.iteration);
+    {
+        LaunchBuilder<beta_k, kFinalizeThreads, PcgArgs, int> L;
+        for (unsigned p = 0; p < P.size(); ++p) L.add(p, dim3(1), make_args(*P[p]), stencil_blocks_of(*P[p], 1));
+        out.iteration.push_back(L.finish(A, dim3(kFinalizeThreads)));
+    }
+    // fixed mode starts from p = 0: r = b
+    {
+        const size_t n = P.size();
+        residual_launches(P, std::vector<const double*>(n, nullptr), std::vector<const float*>(n, nullptr), std::vector<double*>(n, nullptr), true,
+                          true, A, out.fixed_start);
+    }
+    {
+        LaunchBuilder<gather_scalars_k, 64, const PcgScalars*, PcgScalars*> L;
+        for (unsigned p = 0; p < P.size(); ++p) L.add(p, dim3(1), P[p]->scalars, gathered + p);
+        out.gather.push_back(L.finish(A, dim3(64)));
+    }
+}
+
+void build_batched_warm_start(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    // the per-context logic of run_tolerance_pcg's start, buffer roles included
+    LaunchBuilder<zero64_k, 256, double*, int64_t> Z;
+    LaunchBuilder<extrapolate_k, 256, const double*, const double*, double*, int64_t> E;
+    LaunchBuilder<copy64_k, 256, const double*, double*, int64_t> Cp;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        Context& c = *P[p];
+        const int64_t n = c.grid.cells;
+        const dim3 grid(unsigned((n + 255) / 256));
+        if (!c.options.warm_start) {
+            Z.add(p, grid, c.x64[c.x64_current], n);
+        } else if (c.options.warm_start == 2) {
+            if (c.solves_done >= 2) {
+                double* cur = c.x64[c.x64_current];
+                double* spare = c.x64[c.x64_current ^ 1];
+                E.add(p, grid, cur, c.x64_hist, spare, n);
+                c.x64[c.x64_current ^ 1] = c.x64_hist;
+                c.x64_hist = cur;
+                c.x64[c.x64_current] = spare;
+            } else if (c.solves_done == 1) {
+                Cp.add(p, grid, c.x64[c.x64_current], c.x64_hist, n);
+            }
+        }
+    }
+    if (!Z.empty()) out.push_back(Z.finish(A, dim3(256)));
+    if (!E.empty()) out.push_back(E.finish(A, dim3(256)));
+    if (!Cp.empty()) out.push_back(Cp.finish(A, dim3(256)));
+}
+
+void build_batched_residual(const std::vector<Context*>& P, bool refine, TableArena& A, std::vector<BatchedLaunch>& out) {
+    const size_t n = P.size();
+    std::vector<const double*> xo(n);
+    std::vector<const float*> e(n, nullptr);
+    std::vector<double*> xn(n, nullptr);
+    for (size_t p = 0; p < n; ++p) {
+        xo[p] = P[p]->x64[P[p]->x64_current];
+        if (refine) {
+            e[p] = P[p]->pcg_e;
+            xn[p] = P[p]->x64[P[p]->x64_current ^ 1];
+        }
+    }
+    residual_launches(P, xo, e, xn, !refine, !refine, A, out);
+}
+
+void build_batched_zero_x64(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<zero64_k, 256, double*, int64_t> Z;
+    for (unsigned p = 0; p < P.size(); ++p) Z.add(p, dim3(unsigned((P[p]->grid.cells + 255) / 256)), P[p]->x64[P[p]->x64_current], P[p]->grid.cells);
+    out.push_back(Z.finish(A, dim3(256)));
+}
+
+void build_batched_pressure_from_x64(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<pressure_from_x64_k, 256, const double*, float*, int64_t> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        L.add(p, dim3(unsigned((c.grid.cells + 255) / 256)), c.x64[c.x64_current], c.pressure, c.grid.cells);
+    }
+    out.push_back(L.finish(A, dim3(256)));
 }
 
 }  // namespace smoke

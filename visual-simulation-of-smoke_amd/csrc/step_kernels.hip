@@ -6,6 +6,7 @@
 // enforce / project passes (B:1014-1189, B:1425-1600): OUTFLOW copies the
 // finished neighbour face, walls take their prescribed normal velocity, a
 // periodic face n copies face 0.
+#include "batched.hpp"
 #include "context.hpp"
 
 namespace smoke {
@@ -55,10 +56,10 @@ __device__ __forceinline__ float abs_max(const float* __restrict__ f, int64_t n,
     return m;
 }
 
-__global__ void max_speed_kernel(const float* __restrict__ u, int64_t nu, const float* __restrict__ v, int64_t nv, const float* __restrict__ w,
+__device__ __forceinline__ void max_speed_body(const VBlock& vb, const float* __restrict__ u, int64_t nu, const float* __restrict__ v, int64_t nv, const float* __restrict__ w,
                                  int64_t nw, unsigned* __restrict__ max_bits) {
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = int64_t(vb.dim.x) * blockDim.x;
+    const int64_t first = int64_t(vb.idx.x) * blockDim.x + threadIdx.x;
     float m = abs_max(u, nu, first, stride, 0.0f);
     m = abs_max(v, nv, first, stride, m);
     m = abs_max(w, nw, first, stride, m);
@@ -74,8 +75,16 @@ __global__ void max_speed_kernel(const float* __restrict__ u, int64_t nu, const 
     }
 }
 
+__global__ void max_speed_kernel(const float* __restrict__ u, int64_t nu, const float* __restrict__ v, int64_t nv, const float* __restrict__ w,
+                                 int64_t nw, unsigned* __restrict__ max_bits) { max_speed_body(hw_block(), u, nu, v, nv, w, nw, max_bits); }
+
+struct max_speed_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { max_speed_body(vb, a...); }
+};
+
 // dt = min(dt_max, cfl h / max|u|) (adaptive) or the configured dt.
-__global__ void time_step_kernel(unsigned* __restrict__ max_bits, float* __restrict__ max_speed, float* __restrict__ dt, int adaptive, float cfl,
+__device__ __forceinline__ void time_step_body(const VBlock& vb, unsigned* __restrict__ max_bits, float* __restrict__ max_speed, float* __restrict__ dt, int adaptive, float cfl,
                                  float h, float dt_max, float dt_fixed) {
     const float m = __uint_as_float(*max_bits);
     *max_speed = m;
@@ -87,6 +96,14 @@ __global__ void time_step_kernel(unsigned* __restrict__ max_bits, float* __restr
     const float cand = m > 0.0f ? cfl * h / m : dt_max;
     *dt = fminf(dt_max, cand);
 }
+
+__global__ void time_step_kernel(unsigned* __restrict__ max_bits, float* __restrict__ max_speed, float* __restrict__ dt, int adaptive, float cfl,
+                                 float h, float dt_max, float dt_fixed) { time_step_body(hw_block(), max_bits, max_speed, dt, adaptive, cfl, h, dt_max, dt_fixed); }
+
+struct time_step_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { time_step_body(vb, a...); }
+};
 
 // arrays of the grid (faces included) are indexable in 32 bits
 __device__ __forceinline__ bool index32(const Grid& g) {
@@ -102,12 +119,12 @@ __device__ __forceinline__ int cell_stride(const Grid& g) {
 // ------------------------------------------------------------ forces (stage 0)
 // B:2001-2003: solid temperature, then omega and |omega| from the
 // cell-centred velocity with the flow ghost rules (B:870-900).
-__global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+__device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                  const uint8_t* __restrict__ occ, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
                                  float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const int x = vb.idx.x * blockDim.x + threadIdx.x;
+    const int y = vb.idx.y * blockDim.y + threadIdx.y;
+    const int z = vb.idx.z * blockDim.z + threadIdx.z;
     if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
     const int64_t i = cidx(g, x, y, z);
     if (occ[i] != 0) {
@@ -131,15 +148,24 @@ __global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const floa
     om[i] = sqrtf(fmaf(wz, wz, fmaf(wx, wx, wy * wy)));
 }
 
+__global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                 const uint8_t* __restrict__ occ, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature) { vorticity_body(hw_block(), g, u, v, w, occ, ox, oy, oz, om, temperature, solid_temperature); }
+
+struct vorticity_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { vorticity_body(vb, a...); }
+};
+
 // B:902-948: user force, buoyancy (+y) and vorticity confinement in fluid cells.
-__global__ void cell_force_kernel(Grid g, const float* __restrict__ fux, const float* __restrict__ fuy, const float* __restrict__ fuz,
+__device__ __forceinline__ void cell_force_body(const VBlock& vb, Grid g, const float* __restrict__ fux, const float* __restrict__ fuy, const float* __restrict__ fuz,
                                   const float* __restrict__ rho, const float* __restrict__ T, const uint8_t* __restrict__ occ,
                                   const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
                                   const float* __restrict__ om, float a, float b, float tamb, float eps, float* __restrict__ fx,
                                   float* __restrict__ fy, float* __restrict__ fz) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const int x = vb.idx.x * blockDim.x + threadIdx.x;
+    const int y = vb.idx.y * blockDim.y + threadIdx.y;
+    const int z = vb.idx.z * blockDim.z + threadIdx.z;
     if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
     const int64_t i = cidx(g, x, y, z);
     float sx = fux[i], sy = fuy[i], sz = fuz[i];
@@ -164,14 +190,25 @@ __global__ void cell_force_kernel(Grid g, const float* __restrict__ fux, const f
     fz[i] = sz;
 }
 
+__global__ void cell_force_kernel(Grid g, const float* __restrict__ fux, const float* __restrict__ fuy, const float* __restrict__ fuz,
+                                  const float* __restrict__ rho, const float* __restrict__ T, const uint8_t* __restrict__ occ,
+                                  const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
+                                  const float* __restrict__ om, float a, float b, float tamb, float eps, float* __restrict__ fx,
+                                  float* __restrict__ fy, float* __restrict__ fz) { cell_force_body(hw_block(), g, fux, fuy, fuz, rho, T, occ, ox, oy, oz, om, a, b, tamb, eps, fx, fy, fz); }
+
+struct cell_force_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { cell_force_body(vb, a...); }
+};
+
 // B:950-1005 + solid part of B:1014-1168: u_face += dt * mean of the
 // in-domain neighbour cell forces (no wrap), then the solid override.
 template <int C>
-__global__ void face_force_kernel(Grid g, float* __restrict__ f, const float* __restrict__ force, const uint8_t* __restrict__ occ,
+__device__ __forceinline__ void face_force_body(const VBlock& vb, Grid g, float* __restrict__ f, const float* __restrict__ force, const uint8_t* __restrict__ occ,
                                   const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    const int i = vb.idx.x * blockDim.x + threadIdx.x;
+    const int j = vb.idx.y * blockDim.y + threadIdx.y;
+    const int k = vb.idx.z * blockDim.z + threadIdx.z;
     if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
     const int c[3] = {i, j, k};
     if (boundary_face(g, C, c[C])) return;
@@ -208,13 +245,23 @@ __global__ void face_force_kernel(Grid g, float* __restrict__ f, const float* __
     f[fi] += *dt_ptr * (sum / weight);
 }
 
+template <int C>
+__global__ void face_force_kernel(Grid g, float* __restrict__ f, const float* __restrict__ force, const uint8_t* __restrict__ occ,
+                                  const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) { face_force_body<C>(hw_block(), g, f, force, occ, solid_vel, dt_ptr); }
+
+template <int C>
+struct face_force_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { face_force_body<C>(vb, a...); }
+};
+
 // Domain end faces of component C, one thread per line along axis C; both
 // ends are read before either is written.
 template <int C>
-__global__ void boundary_faces_kernel(Grid g, float* __restrict__ f) {
+__device__ __forceinline__ void boundary_faces_body(const VBlock& vb, Grid g, float* __restrict__ f) {
     constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
-    const int q1 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q2 = blockIdx.y * blockDim.y + threadIdx.y;
+    const int q1 = vb.idx.x * blockDim.x + threadIdx.x;
+    const int q2 = vb.idx.y * blockDim.y + threadIdx.y;
     if (q1 >= g.n[A1] || q2 >= g.n[A2]) return;
     auto at = [&](int idx) {
         int c[3];
@@ -237,6 +284,15 @@ __global__ void boundary_faces_kernel(Grid g, float* __restrict__ f) {
     f[at(n)] = hi;
 }
 
+template <int C>
+__global__ void boundary_faces_kernel(Grid g, float* __restrict__ f) { boundary_faces_body<C>(hw_block(), g, f); }
+
+template <int C>
+struct boundary_faces_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { boundary_faces_body<C>(vb, a...); }
+};
+
 // ------------------------------------------------------------ velocity advection (stage 1)
 // B:1191-1219 + the enforce that follows (B:2025-2030): every interior face is
 // traced and sampled unless a solid override replaces it.
@@ -254,12 +310,12 @@ __device__ __forceinline__ float advect_face_value(const Grid& g, const float* _
 }
 
 template <int C>
-__global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+__device__ __forceinline__ void advect_face_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                    float* __restrict__ dst, const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
                                    const float* __restrict__ dt_ptr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    const int i = vb.idx.x * blockDim.x + threadIdx.x;
+    const int j = vb.idx.y * blockDim.y + threadIdx.y;
+    const int k = vb.idx.z * blockDim.z + threadIdx.z;
     if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
     const int c[3] = {i, j, k};
     if (boundary_face(g, C, c[C])) return;
@@ -272,15 +328,26 @@ __global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const fl
     dst[fi] = advect_face_value<C>(g, u, v, w, occ, *dt_ptr, i, j, k);
 }
 
+template <int C>
+__global__ void advect_face_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                   float* __restrict__ dst, const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
+                                   const float* __restrict__ dt_ptr) { advect_face_body<C>(hw_block(), g, u, v, w, dst, occ, solid_vel, dt_ptr); }
+
+template <int C>
+struct advect_face_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_body<C>(vb, a...); }
+};
+
 // ------------------------------------------------------------ projection (stage 3)
 // B:1425-1600 + enforce + copy back (B:2032-2043): the projected faces are
 // written straight into the state array.
 template <int C>
-__global__ void project_kernel(Grid g, const float* __restrict__ ustar, float* __restrict__ out, const float* __restrict__ p,
+__device__ __forceinline__ void project_body(const VBlock& vb, Grid g, const float* __restrict__ ustar, float* __restrict__ out, const float* __restrict__ p,
                                const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    const int i = vb.idx.x * blockDim.x + threadIdx.x;
+    const int j = vb.idx.y * blockDim.y + threadIdx.y;
+    const int k = vb.idx.z * blockDim.z + threadIdx.z;
     if (i >= g.n[0] + (C == 0) || j >= g.n[1] + (C == 1) || k >= g.n[2] + (C == 2)) return;
     const int c[3] = {i, j, k};
     if (boundary_face(g, C, c[C])) return;
@@ -310,27 +377,46 @@ __global__ void project_kernel(Grid g, const float* __restrict__ ustar, float* _
     out[fi] = result;
 }
 
+template <int C>
+__global__ void project_kernel(Grid g, const float* __restrict__ ustar, float* __restrict__ out, const float* __restrict__ p,
+                               const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel, const float* __restrict__ dt_ptr) { project_body<C>(hw_block(), g, ustar, out, p, occ, solid_vel, dt_ptr); }
+
+template <int C>
+struct project_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { project_body<C>(vb, a...); }
+};
+
 // ------------------------------------------------------------ scalars (stage 4)
 // B:852-857: tmp = cur + dt * src for both scalars.
-__global__ void scalar_sources_kernel(const float* __restrict__ T, const float* __restrict__ ST, const float* __restrict__ rho,
+__device__ __forceinline__ void scalar_sources_body(const VBlock& vb, const float* __restrict__ T, const float* __restrict__ ST, const float* __restrict__ rho,
                                       const float* __restrict__ Srho, float* __restrict__ Ttmp, float* __restrict__ rtmp,
                                       const float* __restrict__ dt_ptr, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t i = int64_t(vb.idx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float dt = *dt_ptr;
     Ttmp[i] = T[i] + dt * ST[i];
     rtmp[i] = rho[i] + dt * Srho[i];
 }
 
+__global__ void scalar_sources_kernel(const float* __restrict__ T, const float* __restrict__ ST, const float* __restrict__ rho,
+                                      const float* __restrict__ Srho, float* __restrict__ Ttmp, float* __restrict__ rtmp,
+                                      const float* __restrict__ dt_ptr, int64_t n) { scalar_sources_body(hw_block(), T, ST, rho, Srho, Ttmp, rtmp, dt_ptr, n); }
+
+struct scalar_sources_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { scalar_sources_body(vb, a...); }
+};
+
 // B:1221-1241: one trace per fluid cell shared by temperature and density;
 // solid cells take the solid temperature and a density filled afterwards.
-__global__ void advect_scalars_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+__device__ __forceinline__ void advect_scalars_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                       const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                       float* __restrict__ T, float* __restrict__ rho, const float* __restrict__ solid_T,
                                       const float* __restrict__ dt_ptr) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const int x = vb.idx.x * blockDim.x + threadIdx.x;
+    const int y = vb.idx.y * blockDim.y + threadIdx.y;
+    const int z = vb.idx.z * blockDim.z + threadIdx.z;
     if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
     const int64_t i = cidx(g, x, y, z);
     if (occ[i] != 0) {
@@ -346,14 +432,31 @@ __global__ void advect_scalars_kernel(Grid g, const float* __restrict__ u, const
     rho[i] = rv;
 }
 
+__global__ void advect_scalars_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                      const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                      float* __restrict__ T, float* __restrict__ rho, const float* __restrict__ solid_T,
+                                      const float* __restrict__ dt_ptr) { advect_scalars_body(hw_block(), g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt_ptr); }
+
+struct advect_scalars_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_scalars_body(vb, a...); }
+};
+
 // B:1243-1290 as a gather through the nearest-fluid map built at occupancy update.
-__global__ void density_fill_kernel(float* __restrict__ rho, const int64_t* __restrict__ pairs, const unsigned* __restrict__ count) {
+__device__ __forceinline__ void density_fill_body(const VBlock& vb, float* __restrict__ rho, const int64_t* __restrict__ pairs, const unsigned* __restrict__ count) {
     const unsigned n = *count;
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    for (unsigned q = vb.idx.x * blockDim.x + threadIdx.x; q < n; q += vb.dim.x * blockDim.x) {
         const int64_t cell = pairs[2 * q], src = pairs[2 * q + 1];
         rho[cell] = src >= 0 ? rho[src] : 0.0f;
     }
 }
+
+__global__ void density_fill_kernel(float* __restrict__ rho, const int64_t* __restrict__ pairs, const unsigned* __restrict__ count) { density_fill_body(hw_block(), rho, pairs, count); }
+
+struct density_fill_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { density_fill_body(vb, a...); }
+};
 
 // ------------------------------------------------------------ occupancy-derived data
 __global__ void occupancy_float_kernel(const uint8_t* __restrict__ occ, float* __restrict__ out, int64_t n) {
@@ -490,10 +593,10 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-__global__ void plume_kernel(Grid g, SmokeSimulationPlumeSource src, uint32_t step, float* __restrict__ rs, float* __restrict__ ts) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void plume_body(const VBlock& vb, Grid g, SmokeSimulationPlumeSource src, uint32_t step, float* __restrict__ rs, float* __restrict__ ts) {
+    const int x = vb.idx.x * blockDim.x + threadIdx.x;
+    const int y = vb.idx.y * blockDim.y + threadIdx.y;
+    const int z = vb.idx.z * blockDim.z + threadIdx.z;
     if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
     const int64_t i = cidx(g, x, y, z);
     const float dx = (float(x) + 0.5f) - src.center_x;
@@ -510,6 +613,13 @@ __global__ void plume_kernel(Grid g, SmokeSimulationPlumeSource src, uint32_t st
     rs[i] = src.density_rate;
     ts[i] = __builtin_fmaf(src.temperature_noise, unit, src.temperature_rate);
 }
+
+__global__ void plume_kernel(Grid g, SmokeSimulationPlumeSource src, uint32_t step, float* __restrict__ rs, float* __restrict__ ts) { plume_body(hw_block(), g, src, step, rs, ts); }
+
+struct plume_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { plume_body(vb, a...); }
+};
 
 template <int C>
 void launch_face_force(Context& c) {
@@ -712,7 +822,7 @@ __device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* _
     rho[i] = rv;
 }
 
-__global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                            const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                                            const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                                                            float* __restrict__ T, float* __restrict__ rho,
@@ -728,7 +838,7 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
     __shared__ uint8_t so[kO];
     uint3 tile;
     unsigned id;
-    if (!xcd_tile(tiles, tile, id)) return;
+    if (!xcd_tile(vb, tiles, tile, id)) return;
     const int tid = threadIdx.x;
     const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
     const LdsBox bs{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
@@ -819,8 +929,21 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid 
     }
 }
 
+__global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                           const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                                           const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                                                           float* __restrict__ T, float* __restrict__ rho,
+                                                                           const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
+                                                                           int* __restrict__ fallback, unsigned* __restrict__ fallback_count,
+                                                                           uint3 tiles) { advect_scalars_staged_body(hw_block(), g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt_ptr, fallback, fallback_count, tiles); }
+
+struct advect_scalars_staged_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_scalars_staged_body(vb, a...); }
+};
+
 // the cells the staged kernel could not advect from LDS, by the global path
-__global__ __launch_bounds__(256) void advect_scalars_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__device__ __forceinline__ void advect_scalars_fallback_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                       const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                                       const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                                                       float* __restrict__ T, float* __restrict__ rho,
@@ -828,14 +951,33 @@ __global__ __launch_bounds__(256) void advect_scalars_fallback_kernel(Grid g, co
                                                                       const int* __restrict__ fallback, const unsigned* __restrict__ fallback_count) {
     const unsigned n = *fallback_count;
     const float dt = *dt_ptr;
-    for (unsigned q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+    for (unsigned q = vb.idx.x * 256 + threadIdx.x; q < n; q += vb.dim.x * 256) {
         const int i = fallback[q];
         const int x = i % g.n[0], y = (i / g.n[0]) % g.n[1], z = i / (g.n[0] * g.n[1]);
         advect_scalar_cell(g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt, x, y, z);
     }
 }
 
-__global__ void zero_count_kernel(unsigned* __restrict__ count) { *count = 0u; }
+__global__ __launch_bounds__(256) void advect_scalars_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                      const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                                      const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
+                                                                      float* __restrict__ T, float* __restrict__ rho,
+                                                                      const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
+                                                                      const int* __restrict__ fallback, const unsigned* __restrict__ fallback_count) { advect_scalars_fallback_body(hw_block(), g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt_ptr, fallback, fallback_count); }
+
+struct advect_scalars_fallback_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_scalars_fallback_body(vb, a...); }
+};
+
+__device__ __forceinline__ void zero_count_body(const VBlock& vb, unsigned* __restrict__ count) { *count = 0u; }
+
+__global__ void zero_count_kernel(unsigned* __restrict__ count) { zero_count_body(hw_block(), count); }
+
+struct zero_count_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { zero_count_body(vb, a...); }
+};
 
 // sample_cubic of one staged field (the stencil must lie in the box)
 __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx, const Stencil& s) {
@@ -865,7 +1007,7 @@ __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx
 // test and the solid-face test).  Faces whose trace or foot stencil leaves the
 // staged boxes go to the fallback list.
 template <int C>
-__global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                         const float* __restrict__ w, float* __restrict__ dst,
                                                                         const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
                                                                         const float* __restrict__ dt_ptr, int* __restrict__ fallback,
@@ -879,7 +1021,7 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, 
     __shared__ uint8_t so[kO];
     uint3 tile;
     unsigned id;
-    if (!xcd_tile(tiles, tile, id)) return;
+    if (!xcd_tile(vb, tiles, tile, id)) return;
     const int tid = threadIdx.x;
     const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
     const LdsBox bf{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
@@ -972,21 +1114,46 @@ __global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, 
     }
 }
 
+template <int C>
+__global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                        const float* __restrict__ w, float* __restrict__ dst,
+                                                                        const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
+                                                                        const float* __restrict__ dt_ptr, int* __restrict__ fallback,
+                                                                        unsigned* __restrict__ fallback_count, uint3 tiles) { advect_face_staged_body<C>(hw_block(), g, u, v, w, dst, occ, solid_vel, dt_ptr, fallback, fallback_count, tiles); }
+
+template <int C>
+struct advect_face_staged_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_staged_body<C>(vb, a...); }
+};
+
 // the faces the staged kernel could not advect from LDS, by the global path
 template <int C>
-__global__ __launch_bounds__(256) void advect_face_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__device__ __forceinline__ void advect_face_fallback_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                    const float* __restrict__ w, float* __restrict__ dst, const uint8_t* __restrict__ occ,
                                                                    const float* __restrict__ dt_ptr, const int* __restrict__ fallback,
                                                                    const unsigned* __restrict__ fallback_count) {
     const unsigned n = *fallback_count;
     const float dt = *dt_ptr;
     const int d0 = g.n[0] + (C == 0), d1 = g.n[1] + (C == 1);
-    for (unsigned q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+    for (unsigned q = vb.idx.x * 256 + threadIdx.x; q < n; q += vb.dim.x * 256) {
         const int fi = fallback[q];
         const int i = fi % d0, j = (fi / d0) % d1, k = fi / (d0 * d1);
         dst[fi] = advect_face_value<C>(g, u, v, w, occ, dt, i, j, k);
     }
 }
+
+template <int C>
+__global__ __launch_bounds__(256) void advect_face_fallback_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                   const float* __restrict__ w, float* __restrict__ dst, const uint8_t* __restrict__ occ,
+                                                                   const float* __restrict__ dt_ptr, const int* __restrict__ fallback,
+                                                                   const unsigned* __restrict__ fallback_count) { advect_face_fallback_body<C>(hw_block(), g, u, v, w, dst, occ, dt_ptr, fallback, fallback_count); }
+
+template <int C>
+struct advect_face_fallback_k {
+    template <class... A>
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_fallback_body<C>(vb, a...); }
+};
 
 template <int C>
 void launch_advect(Context& c, int block_index) {
@@ -1134,6 +1301,223 @@ void enqueue_diagnostics(Context& c) {
 void enqueue_plume_sources(Context& c, const SmokeSimulationPlumeSource& src, uint32_t step) {
     plume_kernel<<<cell_grid(c.grid), kCellBlock, 0, c.stream>>>(c.grid, src, step, c.density_source, c.temperature_source);
     hip_check(hipGetLastError(), "plume_kernel");
+}
+
+// ---- batched stages: one launch per kernel over the plumes of a batch
+// (batched.hpp), with the launch shapes of the single-context path above.
+namespace {
+
+constexpr int kCellThreads = 256;   // kCellBlock
+
+template <int C>
+void batch_boundaries(const std::vector<Context*>& P, float* (Context::*faces)[3], TableArena& A, std::vector<BatchedLaunch>& out) {
+    constexpr int A1 = (C + 1) % 3, A2 = (C + 2) % 3;
+    const dim3 block(64, 4, 1);
+    LaunchBuilder<boundary_faces_k<C>, 256, Grid, float*> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        L.add(p, grid3(c.grid.n[A1], c.grid.n[A2], 1, block), c.grid, (c.*faces)[C]);
+    }
+    if (!L.empty()) out.push_back(L.finish(A, block));
+}
+
+void batch_all_boundaries(const std::vector<Context*>& P, float* (Context::*faces)[3], TableArena& A, std::vector<BatchedLaunch>& out) {
+    batch_boundaries<0>(P, faces, A, out);
+    batch_boundaries<1>(P, faces, A, out);
+    batch_boundaries<2>(P, faces, A, out);
+}
+
+// `blocks` of the fallback passes: their loops stride over the list, so any
+// count gives the same results; a plume needs no more than its cells / 256
+unsigned fallback_blocks(const Context& c) { return unsigned(std::min<int64_t>(1024, (c.grid.cells + 255) / 256)); }
+
+template <int C>
+void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<zero_count_k, 64, unsigned*> Z;
+    LaunchBuilder<advect_face_staged_k<C>, kSaX * kSaY, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
+                  const float*, int*, unsigned*, uint3>
+        S;
+    LaunchBuilder<advect_face_fallback_k<C>, 256, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*, const int*,
+                  const unsigned*>
+        F;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        if (c.advect_block[C] != kAdvectStaged) continue;
+        const Grid& g = c.grid;
+        const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
+        Z.add(p, dim3(1), c.advect_fallback_count);
+        S.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C], c.dt_device,
+              c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        F.add(p, dim3(fallback_blocks(c)), g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.dt_device, c.advect_fallback,
+              c.advect_fallback_count);
+    }
+    if (!S.empty()) {
+        out.push_back(Z.finish(A, dim3(1)));
+        out.push_back(S.finish(A, dim3(kSaX * kSaY)));
+        out.push_back(F.finish(A, dim3(256)));
+    }
+    for (int b = 0; b < kAdvectBlockCount; ++b) {
+        const dim3 block = kAdvectBlocks[b];
+        // the gather blocks go up to 512 threads (8 x 8 x 8)
+        LaunchBuilder<advect_face_k<C>, 1024, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*, const float*> G;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            if (c.advect_block[C] != b) continue;
+            const Grid& g = c.grid;
+            G.add(p, grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), block), g, c.face[0], c.face[1], c.face[2], c.face_star[C],
+                  c.occupancy, c.solid_velocity[C], c.dt_device);
+        }
+        if (!G.empty()) out.push_back(G.finish(A, block));
+    }
+}
+
+void batch_advect_scalars(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<zero_count_k, 64, unsigned*> Z;
+    LaunchBuilder<advect_scalars_staged_k, kSaX * kSaY, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*,
+                  float*, float*, const float*, const float*, int*, unsigned*, uint3>
+        S;
+    LaunchBuilder<advect_scalars_fallback_k, 256, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*, float*,
+                  float*, const float*, const float*, const int*, const unsigned*>
+        F;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        if (c.advect_block[3] != kAdvectStaged) continue;
+        const Grid& g = c.grid;
+        const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
+        Z.add(p, dim3(1), c.advect_fallback_count);
+        S.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, c.temperature, c.density,
+              c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        F.add(p, dim3(fallback_blocks(c)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, c.temperature, c.density,
+              c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count);
+    }
+    if (!S.empty()) {
+        out.push_back(Z.finish(A, dim3(1)));
+        out.push_back(S.finish(A, dim3(kSaX * kSaY)));
+        out.push_back(F.finish(A, dim3(256)));
+    }
+    for (int b = 0; b < kAdvectBlockCount; ++b) {
+        const dim3 block = kAdvectBlocks[b];
+        LaunchBuilder<advect_scalars_k, 1024, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*, float*, float*,
+                      const float*, const float*>
+            G;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            if (c.advect_block[3] != b) continue;
+            const Grid& g = c.grid;
+            G.add(p, grid3(g.n[0], g.n[1], g.n[2], block), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp,
+                  c.temperature, c.density, c.solid_temperature, c.dt_device);
+        }
+        if (!G.empty()) out.push_back(G.finish(A, block));
+    }
+}
+
+template <int C>
+void batch_face_force(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<face_force_k<C>, kCellThreads, Grid, float*, const float*, const uint8_t*, const float*, const float*> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        const Grid& g = c.grid;
+        L.add(p, grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock), g, c.face[C], c.cell_force[C], c.occupancy,
+              c.solid_velocity[C], c.dt_device);
+    }
+    out.push_back(L.finish(A, kCellBlock));
+}
+
+template <int C>
+void batch_project(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<project_k<C>, kCellThreads, Grid, const float*, float*, const float*, const uint8_t*, const float*, const float*> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        const Grid& g = c.grid;
+        L.add(p, grid3(g.n[0] + (C == 0), g.n[1] + (C == 1), g.n[2] + (C == 2), kCellBlock), g, c.face_star[C], c.face[C], c.pressure, c.occupancy,
+              c.solid_velocity[C], c.dt_device);
+    }
+    out.push_back(L.finish(A, kCellBlock));
+}
+
+}  // namespace
+
+void build_batched_stages(const std::vector<Context*>& P, TableArena& A, BatchedStages& out) {
+    out = BatchedStages{};
+    // time step: the running max |u_face| of the adaptive plumes, then dt
+    {
+        LaunchBuilder<max_speed_k, kBlock, const float*, int64_t, const float*, int64_t, const float*, int64_t, unsigned*> M;
+        LaunchBuilder<time_step_k, 64, unsigned*, float*, float*, int, float, float, float, float> T;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            const Grid& g = c.grid;
+            if (c.options.adaptive_dt) {
+                const int64_t total = g.faces[0] + g.faces[1] + g.faces[2];
+                const unsigned blocks = unsigned(std::min<int64_t>(2048, (total + kBlock - 1) / kBlock));
+                M.add(p, dim3(blocks), c.face[0], g.faces[0], c.face[1], g.faces[1], c.face[2], g.faces[2], c.max_speed_bits);
+            }
+            T.add(p, dim3(1), c.max_speed_bits, c.max_speed, c.dt_device, c.options.adaptive_dt ? 1 : 0, c.options.cfl, g.h, c.options.dt_max,
+                  c.config.dt);
+        }
+        if (!M.empty()) out.time_step.push_back(M.finish(A, dim3(kBlock)));
+        out.time_step.push_back(T.finish(A, dim3(1)));
+    }
+    // forces
+    {
+        LaunchBuilder<vorticity_k, kCellThreads, Grid, const float*, const float*, const float*, const uint8_t*, float*, float*, float*, float*, float*,
+                      const float*>
+            V;
+        LaunchBuilder<cell_force_k, kCellThreads, Grid, const float*, const float*, const float*, const float*, const float*, const uint8_t*,
+                      const float*, const float*, const float*, const float*, float, float, float, float, float*, float*, float*>
+            F;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            Context& c = *P[p];
+            const Grid& g = c.grid;
+            V.add(p, cell_grid(g), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
+                  c.vorticity_magnitude, c.temperature, c.solid_temperature);
+            F.add(p, cell_grid(g), g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0],
+                  c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor,
+                  c.config.ambient_temperature, c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2]);
+        }
+        out.forces.push_back(V.finish(A, kCellBlock));
+        out.forces.push_back(F.finish(A, kCellBlock));
+        batch_face_force<0>(P, A, out.forces);
+        batch_face_force<1>(P, A, out.forces);
+        batch_face_force<2>(P, A, out.forces);
+        batch_all_boundaries(P, &Context::face, A, out.forces);
+    }
+    // velocity advection
+    batch_advect_face<0>(P, A, out.advect);
+    batch_advect_face<1>(P, A, out.advect);
+    batch_advect_face<2>(P, A, out.advect);
+    batch_all_boundaries(P, &Context::face_star, A, out.advect);
+    // projection
+    batch_project<0>(P, A, out.project);
+    batch_project<1>(P, A, out.project);
+    batch_project<2>(P, A, out.project);
+    batch_all_boundaries(P, &Context::face, A, out.project);
+    // scalars
+    {
+        LaunchBuilder<scalar_sources_k, kBlock, const float*, const float*, const float*, const float*, float*, float*, const float*, int64_t> S;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            S.add(p, dim3(grid1(c.grid.cells)), c.temperature, c.temperature_source, c.density, c.density_source, c.temperature_tmp, c.density_tmp,
+                  c.dt_device, c.grid.cells);
+        }
+        out.scalars.push_back(S.finish(A, dim3(kBlock)));
+        batch_advect_scalars(P, A, out.scalars);
+        LaunchBuilder<density_fill_k, kBlock, float*, const int64_t*, const unsigned*> D;
+        for (unsigned p = 0; p < P.size(); ++p) {
+            const Context& c = *P[p];
+            D.add(p, dim3(256), c.density, c.solid_pairs, c.solid_count);
+        }
+        out.scalars.push_back(D.finish(A, dim3(kBlock)));
+    }
+}
+
+void build_batched_sources(const std::vector<Context*>& P, const SmokeSimulationPlumeSource* src, uint32_t step, TableArena& A,
+                           std::vector<BatchedLaunch>& out) {
+    LaunchBuilder<plume_k, kCellThreads, Grid, SmokeSimulationPlumeSource, uint32_t, float*, float*> L;
+    for (unsigned p = 0; p < P.size(); ++p) {
+        const Context& c = *P[p];
+        L.add(p, cell_grid(c.grid), c.grid, src[p], step, c.density_source, c.temperature_source);
+    }
+    out.push_back(L.finish(A, kCellBlock));
 }
 
 void table_advect_blocks(Context& c) {

@@ -154,6 +154,11 @@ class CheckpointInfo(C.Structure):
                 ("error", C.c_char * 96)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("fused", C.c_int32), ("graph", C.c_int32), ("steps", C.c_uint64), ("launches_per_step", C.c_uint64),
+                ("launches_per_iteration", C.c_uint64)]
+
+
 class BatchCreateDesc(C.Structure):
     _fields_ = [("plumes", C.c_void_p), ("count", C.c_uint32), ("lanes", C.c_int32)]
 
@@ -319,6 +324,8 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_batch_context": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)],
         "smoke_simulation_ext_step_batch": [C.c_void_p, C.c_void_p, C.c_uint32],
         "smoke_simulation_ext_synchronize_batch": [C.c_void_p],
+        "smoke_simulation_ext_get_batch_info": [C.c_void_p, C.POINTER(BatchInfo)],
+        "smoke_simulation_ext_set_batch_fusion": [C.c_void_p, C.c_int32],
         "smoke_simulation_ext_batch_lanes": [C.POINTER(C.c_uint64), C.c_uint32, C.c_int32, C.POINTER(C.c_int32)],
     }
     for name, args in sigs.items():
@@ -695,6 +702,15 @@ class SmokeBatch:
 
     def synchronize(self):
         check(self.lib.smoke_simulation_ext_synchronize_batch(self.handle), "synchronize_batch")
+
+    def set_fusion(self, enable: bool):
+        check(self.lib.smoke_simulation_ext_set_batch_fusion(self.handle, 1 if enable else 0), "set_batch_fusion")
+
+    def info(self) -> BatchInfo:
+        """How the last step ran (fused path, graph, launches per step and per CG iteration)."""
+        out = BatchInfo()
+        check(self.lib.smoke_simulation_ext_get_batch_info(self.handle, C.byref(out)), "get_batch_info")
+        return out
 
     def close(self):
         if self.handle:
