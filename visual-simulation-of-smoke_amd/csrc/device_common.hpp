@@ -252,21 +252,47 @@ __device__ inline float load_face(const Grid& g, const float* __restrict__ f, in
     return f[fidx(g, comp, c[0], c[1], c[2])];
 }
 
-// B:378-392
-__device__ __forceinline__ float monotonic_cubic(float p0, float p1, float p2, float p3, float t) {
-    const float delta = p2 - p1;
-    float m1 = 0.5f * (p2 - p0);
-    float m2 = 0.5f * (p3 - p1);
-    if (fabsf(delta) < 1.0e-6f) {
-        m1 = 0.0f;
-        m2 = 0.0f;
-    } else {
-        if (m1 * delta <= 0.0f) m1 = 0.0f;
-        if (m2 * delta <= 0.0f) m2 = 0.0f;
-    }
+// B:378-392: the monotone cubic of p0..p3 at t in [0, 1] -- the Hermite
+// basis h00 p1 + h10 m1 + h01 p2 + h11 m2 (evaluated as p1 + h01 (p2 - p1) +
+// h10 m1 + h11 m2, h00 = 1 - h01) with slopes m1 = (p2 - p0) / 2,
+// m2 = (p3 - p1) / 2, each zeroed when its sign differs from that of
+// delta = p2 - p1 (or is 0), and both zeroed when |delta| < 1e-6.
+// The basis of a fraction t is computed once (Hermite) and shared by every
+// row sampled at that fraction; the slopes' factor 1/2 is carried by the
+// basis (exact: a power of two), so a row costs its three differences, the
+// limiter and four multiply-adds.  The limiter is one clamp per slope: the
+// slope clamped to the interval between 0 and +-inf on delta's side (to 0
+// when |delta| < 1e-6) -- the same slopes as the sign tests, except a slope
+// whose product with delta underflows to 0, which the sign tests would zero
+// and the clamp keeps (its contribution is then below 2^-126 |h10|).
+struct Hermite {
+    float h00, h10, h01, h11;   // h10 and h11 times 1/2
+};
+
+__device__ __forceinline__ Hermite hermite(float t) {
     const float t2 = t * t;
     const float t3 = t2 * t;
-    return (2.0f * t3 - 3.0f * t2 + 1.0f) * p1 + (t3 - 2.0f * t2 + t) * m1 + (-2.0f * t3 + 3.0f * t2) * p2 + (t3 - t2) * m2;
+    Hermite w;
+    // every multiply-add written out, so that the rounding does not depend
+    // on how the compiler contracts the expression where it is inlined
+    w.h00 = fmaf(2.0f, t3, fmaf(-3.0f, t2, 1.0f));
+    w.h10 = 0.5f * fmaf(-2.0f, t2, t3 + t);
+    w.h01 = fmaf(-2.0f, t3, 3.0f * t2);
+    w.h11 = 0.5f * (t3 - t2);
+    return w;
+}
+
+__device__ __forceinline__ float monotonic_cubic(float p0, float p1, float p2, float p3, const Hermite& w) {
+    const float delta = p2 - p1;
+    const float d1 = p2 - p0, d2 = p3 - p1;   // twice the slopes
+    const float side = fabsf(delta) < 1.0e-6f ? 0.0f : copysignf(__builtin_inff(), delta);
+    const float s1 = __builtin_amdgcn_fmed3f(d1, 0.0f, side);
+    const float s2 = __builtin_amdgcn_fmed3f(d2, 0.0f, side);
+    return fmaf(w.h11, s2, fmaf(w.h10, s1, fmaf(w.h01, delta, p1)));
+}
+
+__device__ __forceinline__ float monotonic_cubic(float p0, float p1, float p2, float p3, float t) {
+    return monotonic_cubic(p0, p1, p2, p3, hermite(t));
 }
 
 // fmodf(p, extent) then + extent when negative (the periodic wrap of the
@@ -282,9 +308,12 @@ __device__ __forceinline__ float wrap_position(float p, float extent) {
     return p < 0.0f ? p + extent : p;
 }
 
-// Sampling position -> base index and fraction on each axis.  `comp` is the
-// staggered axis of a face component, -1 for cell-centred data; `periodic`
-// selects the fmod wrap (B:394-410 and the other samplers' preambles).
+// Sampling position -> base index and fraction on each axis.  Positions are
+// in cell units (world position / h: the face of component C with index i
+// lies at i on axis C and at j + 1/2 on the others).  `comp` is the staggered
+// axis of a face component, -1 for cell-centred data; `periodic` selects the
+// fmod wrap (B:394-410 and the other samplers' preambles, there in world
+// units: the wrap extent is n cells).
 struct Stencil {
     int b[3];
     float t[3];
@@ -296,12 +325,12 @@ __device__ __forceinline__ Stencil locate(const Grid& g, const int periodic[3], 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (periodic[a]) {
-            p[a] = wrap_position(p[a], float(g.n[a]) * g.h);
+            p[a] = wrap_position(p[a], float(g.n[a]));
         }
         // Beyond two cells outside the domain every tap is the same ghost
         // value, so clamping there changes no result; it keeps b +- 2 (and a
         // non-finite position) inside int range for the bounds tests below.
-        const float gc = fminf(fmaxf(a == comp ? p[a] / g.h : p[a] / g.h - 0.5f, -4.0f), float(g.n[a]) + 4.0f);
+        const float gc = __builtin_amdgcn_fmed3f(a == comp ? p[a] : p[a] - 0.5f, -4.0f, float(g.n[a]) + 4.0f);
         s.b[a] = int(floorf(gc));
         s.t[a] = gc - float(s.b[a]);
     }
@@ -326,6 +355,20 @@ __device__ __forceinline__ bool stencil_inside(const Grid& g, const Stencil& s, 
     return ok;
 }
 
+// Trilinear combination of the 8 taps c[(dz << 2) | (dy << 1) | dx], lerp
+// order x, y, z (B:394-440), each lerp one written-out multiply-add.
+__device__ __forceinline__ float lerp1(float a, float b, float t) { return fmaf(b - a, t, a); }
+
+__device__ __forceinline__ float trilinear(const float (&c)[8], const Stencil& s) {
+    const float c00 = lerp1(c[0], c[1], s.t[0]);
+    const float c10 = lerp1(c[2], c[3], s.t[0]);
+    const float c01 = lerp1(c[4], c[5], s.t[0]);
+    const float c11 = lerp1(c[6], c[7], s.t[0]);
+    const float c0 = lerp1(c00, c10, s.t[1]);
+    const float c1 = lerp1(c01, c11, s.t[1]);
+    return lerp1(c0, c1, s.t[2]);
+}
+
 // B:394-440 / 486-530 etc. -- trilinear, lerp order x, y, z.
 template <int COMP>
 __device__ inline float sample_linear(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, const Stencil& s) {
@@ -346,19 +389,14 @@ __device__ inline float sample_linear(const Grid& g, const ScalarBC& bc, const f
 #pragma unroll
         for (int q = 0; q < 8; ++q) c[q] = tap<COMP>(g, bc, f, s.b[0] + (q & 1), s.b[1] + ((q >> 1) & 1), s.b[2] + (q >> 2));
     }
-    const float c00 = c[0] + (c[1] - c[0]) * s.t[0];
-    const float c10 = c[2] + (c[3] - c[2]) * s.t[0];
-    const float c01 = c[4] + (c[5] - c[4]) * s.t[0];
-    const float c11 = c[6] + (c[7] - c[6]) * s.t[0];
-    const float c0 = c00 + (c10 - c00) * s.t[1];
-    const float c1 = c01 + (c11 - c01) * s.t[1];
-    return c0 + (c1 - c0) * s.t[2];
+    return trilinear(c, s);
 }
 
 // B:442-484 / 532-573 etc. -- 4x4x4 taps, monotone cubic along x, then y, then z.
 template <int COMP>
 __device__ inline float sample_cubic(const Grid& g, const ScalarBC& bc, const float* __restrict__ f, const Stencil& s) {
     float zs[4];
+    const Hermite wx = hermite(s.t[0]), wy = hermite(s.t[1]), wz = hermite(s.t[2]);
     if (stencil_inside<COMP>(g, s, -1, 2)) {
         const int d0 = g.n[0] + (COMP == 0), d1 = g.n[1] + (COMP == 1);
         const int64_t sy = d0, sz = int64_t(d0) * d1;
@@ -369,9 +407,9 @@ __device__ inline float sample_cubic(const Grid& g, const ScalarBC& bc, const fl
 #pragma unroll
             for (int dy = 0; dy < 4; ++dy) {
                 const float* row = f + base + dz * sz + dy * sy;
-                ys[dy] = monotonic_cubic(row[0], row[1], row[2], row[3], s.t[0]);
+                ys[dy] = monotonic_cubic(row[0], row[1], row[2], row[3], wx);
             }
-            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], wy);
         }
     } else {
         for (int dz = 0; dz < 4; ++dz) {
@@ -382,12 +420,12 @@ __device__ inline float sample_cubic(const Grid& g, const ScalarBC& bc, const fl
                 const float p1 = tap<COMP>(g, bc, f, s.b[0], yy, zz);
                 const float p2 = tap<COMP>(g, bc, f, s.b[0] + 1, yy, zz);
                 const float p3 = tap<COMP>(g, bc, f, s.b[0] + 2, yy, zz);
-                ys[dy] = monotonic_cubic(p0, p1, p2, p3, s.t[0]);
+                ys[dy] = monotonic_cubic(p0, p1, p2, p3, wx);
             }
-            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+            zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], wy);
         }
     }
-    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], s.t[2]);
+    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], wz);
 }
 
 template <int COMP>
@@ -418,6 +456,7 @@ __device__ inline void sample_scalar2(const Grid& g, const ScalarBC& bca, const 
     }
     const int64_t sy = g.n[0], sz = int64_t(g.n[0]) * g.n[1];
     const int64_t base = (int64_t(sa.b[2] - 1) * g.n[1] + (sa.b[1] - 1)) * g.n[0] + (sa.b[0] - 1);
+    const Hermite wx = hermite(sa.t[0]), wy = hermite(sa.t[1]), wz = hermite(sa.t[2]);
     float za[4], zb[4];
 #pragma unroll
     for (int dz = 0; dz < 4; ++dz) {
@@ -427,56 +466,112 @@ __device__ inline void sample_scalar2(const Grid& g, const ScalarBC& bca, const 
             const int64_t o = base + dz * sz + dy * sy;
             const float a0 = fa[o], a1 = fa[o + 1], a2 = fa[o + 2], a3 = fa[o + 3];
             const float b0 = fb[o], b1 = fb[o + 1], b2 = fb[o + 2], b3 = fb[o + 3];
-            ya[dy] = monotonic_cubic(a0, a1, a2, a3, sa.t[0]);
-            yb[dy] = monotonic_cubic(b0, b1, b2, b3, sa.t[0]);
+            ya[dy] = monotonic_cubic(a0, a1, a2, a3, wx);
+            yb[dy] = monotonic_cubic(b0, b1, b2, b3, wx);
         }
-        za[dz] = monotonic_cubic(ya[0], ya[1], ya[2], ya[3], sa.t[1]);
-        zb[dz] = monotonic_cubic(yb[0], yb[1], yb[2], yb[3], sa.t[1]);
+        za[dz] = monotonic_cubic(ya[0], ya[1], ya[2], ya[3], wy);
+        zb[dz] = monotonic_cubic(yb[0], yb[1], yb[2], yb[3], wy);
     }
-    va = monotonic_cubic(za[0], za[1], za[2], za[3], sa.t[2]);
-    vb = monotonic_cubic(zb[0], zb[1], zb[2], zb[3], sa.t[2]);
+    va = monotonic_cubic(za[0], za[1], za[2], za[3], wz);
+    vb = monotonic_cubic(zb[0], zb[1], zb[2], zb[3], wz);
 }
 
-// Hit test of B:785-794 / 819-828.
+// Hit test of B:785-794 / 819-828, position in cell units.
 __device__ __forceinline__ bool hits_solid(const Grid& g, const uint8_t* __restrict__ occ, float qx, float qy, float qz) {
     float q[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]) * g.h);
+        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]));
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        if (q[a] < 0.0f || q[a] > float(g.n[a]) * g.h) return true;
+        if (q[a] < 0.0f || q[a] > float(g.n[a])) return true;
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        c[a] = int(floorf(q[a] / g.h));
+        c[a] = int(floorf(q[a]));
         if (c[a] == g.n[a]) c[a] = g.n[a] - 1;
     }
     if (c[0] < 0 || c[0] >= g.n[0] || c[1] < 0 || c[1] >= g.n[1] || c[2] < 0 || c[2] >= g.n[2]) return true;
     return occ[cidx(g, c[0], c[1], c[2])] != 0;
 }
 
+// The linear velocity samples at the start of a trace.  A trace starts at a
+// cell centre (scalars) or at a face (velocity), where every sample fraction
+// is 0 or 1/2: the trilinear form then reduces to one or two half-way lerps of
+// taps next to the start, bit for bit (a lerp at fraction 0 returns its first
+// value; only a zero's sign can differ, and the positions built from the
+// samples cannot), so these read 2 (cell centre) or 4 taps instead of 8.
+// `tap(comp, i, j, k)` returns a face value with the ghost rules.
+__device__ __forceinline__ float half_lerp(float a, float b) { return lerp1(a, b, 0.5f); }
+
+// component D at the start of a trace from face (i, j, k) of component C
+// (D != C): fraction 1/2 on axis C (taps idx_C - 1, idx_C), 1/2 on axis D
+// (taps idx_D, idx_D + 1), 0 on the third axis; lerps in x, y, z order
+template <int C, int D, class Tap>
+__device__ __forceinline__ float face_start_component(const Tap& tap, int i, int j, int k) {
+    constexpr int A1 = C < D ? C : D, A2 = C < D ? D : C;
+    auto at = [&](int o1, int o2) {
+        int c[3] = {i, j, k};
+        c[A1] += o1;
+        c[A2] += o2;
+        return tap(D, c[0], c[1], c[2]);
+    };
+    constexpr int lo1 = A1 == C ? -1 : 0, lo2 = A2 == C ? -1 : 0;   // lower tap on each axis
+    return half_lerp(half_lerp(at(lo1, lo2), at(lo1 + 1, lo2)), half_lerp(at(lo1, lo2 + 1), at(lo1 + 1, lo2 + 1)));
+}
+
+// (v0x, v0y, v0z) at the start of a trace from face (i, j, k) of component C,
+// or (C = -1) from the centre of cell (i, j, k)
+template <int C, class Tap>
+__device__ __forceinline__ float3 start_velocity(const Tap& tap, int i, int j, int k) {
+    if constexpr (C < 0) {
+        return make_float3(half_lerp(tap(0, i, j, k), tap(0, i + 1, j, k)), half_lerp(tap(1, i, j, k), tap(1, i, j + 1, k)),
+                           half_lerp(tap(2, i, j, k), tap(2, i, j, k + 1)));
+    } else {
+        float v[3];
+        v[C] = tap(C, i, j, k);
+        if constexpr (C != 0) v[0] = face_start_component<C, 0>(tap, i, j, k);
+        if constexpr (C != 1) v[1] = face_start_component<C, 1>(tap, i, j, k);
+        if constexpr (C != 2) v[2] = face_start_component<C, 2>(tap, i, j, k);
+        return make_float3(v[0], v[1], v[2]);
+    }
+}
+
 // B:756-838 -- RK2 midpoint backtrace with linear velocity samples, bisection
-// clipping against walls and solids; the result is not wrapped.
-__device__ inline float3 trace_rk2(const Grid& g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
-                                   const uint8_t* __restrict__ occ, float dt, float sx, float sy, float sz) {
-    const float v0x = sample_face<0>(g, u, sx, sy, sz, false);
-    const float v0y = sample_face<1>(g, v, sx, sy, sz, false);
-    const float v0z = sample_face<2>(g, w, sx, sy, sz, false);
-    const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
+// clipping against walls and solids; the result is not wrapped.  In cell
+// units: a velocity v moves dt v / h cells, so `s` = dt / h scales the
+// samples (the reference divides each world position by h instead).  This
+// is the rest of the trace from its start (sx, sy, sz) and start velocity v0.
+__device__ inline float3 trace_rk2_from(const Grid& g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                        const uint8_t* __restrict__ occ, float s, float sx, float sy, float sz, float3 v0) {
+    const float hs = -0.5f * s;
+    const float mx = fmaf(hs, v0.x, sx), my = fmaf(hs, v0.y, sy), mz = fmaf(hs, v0.z, sz);
     const float v1x = sample_face<0>(g, u, mx, my, mz, false);
     const float v1y = sample_face<1>(g, v, mx, my, mz, false);
     const float v1z = sample_face<2>(g, w, mx, my, mz, false);
-    const float tx = sx - dt * v1x, ty = sy - dt * v1y, tz = sz - dt * v1z;
+    const float tx = fmaf(-s, v1x, sx), ty = fmaf(-s, v1y, sy), tz = fmaf(-s, v1z, sz);
     if (!hits_solid(g, occ, tx, ty, tz)) return make_float3(tx, ty, tz);
     float lo = 0.0f, hi = 1.0f;
     for (int it = 0; it < 10; ++it) {
         const float mt = 0.5f * (lo + hi);
-        if (hits_solid(g, occ, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tz - sz) * mt)) hi = mt;
+        if (hits_solid(g, occ, fmaf(tx - sx, mt, sx), fmaf(ty - sy, mt, sy), fmaf(tz - sz, mt, sz))) hi = mt;
         else lo = mt;
     }
-    return make_float3(sx + (tx - sx) * lo, sy + (ty - sy) * lo, sz + (tz - sz) * lo);
+    return make_float3(fmaf(tx - sx, lo, sx), fmaf(ty - sy, lo, sy), fmaf(tz - sz, lo, sz));
+}
+
+// RK2 trace from face (i, j, k) of component C (C = -1: from the centre of
+// cell (i, j, k)), ghost-rule taps from the global arrays
+template <int C>
+__device__ inline float3 trace_rk2_at(const Grid& g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                      const uint8_t* __restrict__ occ, float s, int i, int j, int k) {
+    auto tap = [&](int comp, int a, int b, int c) { return load_face(g, comp == 0 ? u : (comp == 1 ? v : w), comp, a, b, c); };
+    const float3 v0 = start_velocity<C>(tap, i, j, k);
+    const float sx = C == 0 ? float(i) : float(i) + 0.5f;
+    const float sy = C == 1 ? float(j) : float(j) + 0.5f;
+    const float sz = C == 2 ? float(k) : float(k) + 0.5f;
+    return trace_rk2_from(g, u, v, w, occ, s, sx, sy, sz, v0);
 }
 
 // B:1007-1012 with the face-neighbour logic of B:282-304: mean solid velocity

@@ -21,16 +21,17 @@ const dim3 kCellBlock(64, 4, 1);
 // Candidate launches of the advection kernels: the blocks of the gather form
 // (threads along x, y, z; every thread computes one output independently, so
 // the block shape changes only the speed), then the LDS-staged form
-// (kAdvectStaged).  Measured on MI355X at 256^3 (plume state, cubic
-// advection): the best gather blocks take 0.61 ms for u (16 x 16 x 1), 1.13,
-// 1.14 and 2.24 ms for v, w and the scalars (128 x 2 x 1); the staged form
-// 0.86, 0.85, 0.84 and 1.26 ms.  The table takes the faster of the two.
+// (kAdvectStaged).  Measured on MI355X at 256^3 (12-sphere plume state after
+// 30 steps, cubic advection): the gather form's 16 x 4 x 4 block takes
+// 0.50 ms for u (16 x 16 x 1: 0.55 ms), the staged form 0.62, 0.62 and
+// 0.96 ms for v, w and the scalars (the best gather blocks about twice
+// that).  The table takes the faster of the two.
 const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
                               dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
 constexpr int kAdvectStaged = kAdvectBlockCount;
 constexpr int kAdvectCandidates = kAdvectBlockCount + 1;
-constexpr int kAdvectTable[4] = {2, kAdvectStaged, kAdvectStaged, kAdvectStaged};
+constexpr int kAdvectTable[4] = {8, kAdvectStaged, kAdvectStaged, kAdvectStaged};   // u: 16 x 4 x 4
 
 __global__ void fill_kernel(float* __restrict__ p, float value, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -299,12 +300,9 @@ struct boundary_faces_k {
 // face (i,j,k) of component C traced back and sampled by the global path
 template <int C>
 __device__ __forceinline__ float advect_face_value(const Grid& g, const float* __restrict__ u, const float* __restrict__ v,
-                                                   const float* __restrict__ w, const uint8_t* __restrict__ occ, float dt, int i, int j, int k) {
-    const float h = g.h;
-    const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
-    const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
-    const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
-    const float3 p = trace_rk2(g, u, v, w, occ, dt, sx, sy, sz);
+                                                   const float* __restrict__ w, const uint8_t* __restrict__ occ, float s, int i, int j, int k) {
+    // positions in cell units, s = dt / h
+    const float3 p = trace_rk2_at<C>(g, u, v, w, occ, s, i, j, k);
     const float* src = C == 0 ? u : (C == 1 ? v : w);
     return sample_face<C>(g, src, p.x, p.y, p.z, g.cubic != 0);
 }
@@ -325,7 +323,7 @@ __device__ __forceinline__ void advect_face_body(const VBlock& vb, Grid g, const
         dst[fi] = value;
         return;
     }
-    dst[fi] = advect_face_value<C>(g, u, v, w, occ, *dt_ptr, i, j, k);
+    dst[fi] = advect_face_value<C>(g, u, v, w, occ, *dt_ptr / g.h, i, j, k);
 }
 
 template <int C>
@@ -424,8 +422,7 @@ __device__ __forceinline__ void advect_scalars_body(const VBlock& vb, Grid g, co
         rho[i] = 0.0f;
         return;
     }
-    const float h = g.h;
-    const float3 p = trace_rk2(g, u, v, w, occ, *dt_ptr, (float(x) + 0.5f) * h, (float(y) + 0.5f) * h, (float(z) + 0.5f) * h);
+    const float3 p = trace_rk2_at<-1>(g, u, v, w, occ, *dt_ptr / g.h, x, y, z);
     float tv, rv;
     sample_scalar2(g, g.temperature, Ttmp, g.density, rtmp, p.x, p.y, p.z, g.cubic != 0, tv, rv);
     T[i] = tv;
@@ -657,6 +654,12 @@ void launch_boundaries(Context& c, float* const f[3]) {
 // advects with the global path, so the common path carries none of the
 // ghost-rule code and keeps its register budget.
 constexpr int kSaX = 32, kSaY = 8, kSaZ = 2;
+// threads of a staged block: each owns an (x, y) column of the tile and the
+// z planes tz0, tz0 + kSaZStep, ... of it
+constexpr int kSaThreads = 512;
+constexpr int kSaPlane = kSaX * kSaY;
+constexpr int kSaZStep = kSaThreads / kSaPlane;
+static_assert(kSaThreads % kSaPlane == 0 && kSaZ % kSaZStep == 0, "staged tile and block");
 
 struct LdsBox {
     int o[3];   // origin (global index of the first staged entry)
@@ -709,19 +712,14 @@ __device__ __forceinline__ float staged_linear(const float* lds, const LdsBox& b
     c[5] = lds[base + sz + 1];
     c[6] = lds[base + sz + sy];
     c[7] = lds[base + sz + sy + 1];
-    const float c00 = c[0] + (c[1] - c[0]) * s.t[0];
-    const float c10 = c[2] + (c[3] - c[2]) * s.t[0];
-    const float c01 = c[4] + (c[5] - c[4]) * s.t[0];
-    const float c11 = c[6] + (c[7] - c[6]) * s.t[0];
-    const float c0 = c00 + (c10 - c00) * s.t[1];
-    const float c1 = c01 + (c11 - c01) * s.t[1];
-    return c0 + (c1 - c0) * s.t[2];
+    return trilinear(c, s);
 }
 
 // sample_cubic of two staged fields at one stencil (which must lie in the box)
 __device__ __forceinline__ void staged_cubic2(const float* la, const float* lb, const LdsBox& bx, const Stencil& s, float& va, float& vb) {
     const int sy = bx.d[0], sz = bx.d[0] * bx.d[1];
     const int base = box_base(bx, s.b[0] - 1, s.b[1] - 1, s.b[2] - 1);
+    const Hermite wx = hermite(s.t[0]), wy = hermite(s.t[1]), wz = hermite(s.t[2]);
     float za[4], zb[4];
 #pragma unroll
     for (int dz = 0; dz < 4; ++dz) {
@@ -729,14 +727,14 @@ __device__ __forceinline__ void staged_cubic2(const float* la, const float* lb, 
 #pragma unroll
         for (int dy = 0; dy < 4; ++dy) {
             const int o = base + dz * sz + dy * sy;
-            ya[dy] = monotonic_cubic(la[o], la[o + 1], la[o + 2], la[o + 3], s.t[0]);
-            yb[dy] = monotonic_cubic(lb[o], lb[o + 1], lb[o + 2], lb[o + 3], s.t[0]);
+            ya[dy] = monotonic_cubic(la[o], la[o + 1], la[o + 2], la[o + 3], wx);
+            yb[dy] = monotonic_cubic(lb[o], lb[o + 1], lb[o + 2], lb[o + 3], wx);
         }
-        za[dz] = monotonic_cubic(ya[0], ya[1], ya[2], ya[3], s.t[1]);
-        zb[dz] = monotonic_cubic(yb[0], yb[1], yb[2], yb[3], s.t[1]);
+        za[dz] = monotonic_cubic(ya[0], ya[1], ya[2], ya[3], wy);
+        zb[dz] = monotonic_cubic(yb[0], yb[1], yb[2], yb[3], wy);
     }
-    va = monotonic_cubic(za[0], za[1], za[2], za[3], s.t[2]);
-    vb = monotonic_cubic(zb[0], zb[1], zb[2], zb[3], s.t[2]);
+    va = monotonic_cubic(za[0], za[1], za[2], za[3], wz);
+    vb = monotonic_cubic(zb[0], zb[1], zb[2], zb[3], wz);
 }
 
 // hits_solid (B:785-794) against the staged occupancy: 1 hit, 0 free,
@@ -745,15 +743,15 @@ __device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, co
     float q[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]) * g.h);
+        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]));
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        if (q[a] < 0.0f || q[a] > float(g.n[a]) * g.h) return 1;
+        if (q[a] < 0.0f || q[a] > float(g.n[a])) return 1;
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        c[a] = int(floorf(q[a] / g.h));
+        c[a] = int(floorf(q[a]));
         if (c[a] == g.n[a]) c[a] = g.n[a] - 1;
     }
     if (c[0] < 0 || c[0] >= g.n[0] || c[1] < 0 || c[1] >= g.n[1] || c[2] < 0 || c[2] >= g.n[2]) return 1;
@@ -764,24 +762,32 @@ __device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, co
     return locc[box_base(bx, c[0], c[1], c[2])] != 0 ? 1 : 0;
 }
 
-// trace_rk2 (B:756-838) from (sx, sy, sz) with linear velocity samples of
-// the staged components and the staged hit test: false (nothing written) when
-// a sample stencil or a hit-test cell lies outside its staged box
+// trace_rk2 (B:756-838) from face (i, j, k) of component C (C = -1: the
+// centre of cell (i, j, k)) with linear velocity samples of the staged
+// components and the staged hit test: false (nothing written) when a sample
+// stencil or a hit-test cell lies outside its staged box.  The start samples
+// read the taps next to the start (start_velocity), which every box holds.
+template <int C>
 __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, const LdsBox& bu, const float* lv, const LdsBox& bv, const float* lw,
-                                             const LdsBox& bw, const uint8_t* locc, const LdsBox& bo, float dt, float sx, float sy, float sz,
-                                             float& px, float& py, float& pz) {
-    const Stencil a0 = locate(g, g.flow_periodic, 0, sx, sy, sz);
-    const Stencil a1 = locate(g, g.flow_periodic, 1, sx, sy, sz);
-    const Stencil a2 = locate(g, g.flow_periodic, 2, sx, sy, sz);
-    if (!(in_box<0, 1>(bu, a0) && in_box<0, 1>(bv, a1) && in_box<0, 1>(bw, a2))) return false;
-    const float v0x = staged_linear(lu, bu, a0), v0y = staged_linear(lv, bv, a1), v0z = staged_linear(lw, bw, a2);
-    const float mx = sx - 0.5f * dt * v0x, my = sy - 0.5f * dt * v0y, mz = sz - 0.5f * dt * v0z;
+                                             const LdsBox& bw, const uint8_t* locc, const LdsBox& bo, float s, int i, int j, int k, float& px,
+                                             float& py, float& pz) {
+    auto tap = [&](int comp, int a, int b, int c) {
+        const float* l = comp == 0 ? lu : (comp == 1 ? lv : lw);
+        const LdsBox& bx = comp == 0 ? bu : (comp == 1 ? bv : bw);
+        return l[box_base(bx, a, b, c)];
+    };
+    const float3 v0 = start_velocity<C>(tap, i, j, k);
+    const float sx = C == 0 ? float(i) : float(i) + 0.5f;
+    const float sy = C == 1 ? float(j) : float(j) + 0.5f;
+    const float sz = C == 2 ? float(k) : float(k) + 0.5f;
+    const float hs = -0.5f * s;
+    const float mx = fmaf(hs, v0.x, sx), my = fmaf(hs, v0.y, sy), mz = fmaf(hs, v0.z, sz);
     const Stencil m0 = locate(g, g.flow_periodic, 0, mx, my, mz);
     const Stencil m1 = locate(g, g.flow_periodic, 1, mx, my, mz);
     const Stencil m2 = locate(g, g.flow_periodic, 2, mx, my, mz);
     if (!(in_box<0, 1>(bu, m0) && in_box<0, 1>(bv, m1) && in_box<0, 1>(bw, m2))) return false;
     const float v1x = staged_linear(lu, bu, m0), v1y = staged_linear(lv, bv, m1), v1z = staged_linear(lw, bw, m2);
-    const float tx = sx - dt * v1x, ty = sy - dt * v1y, tz = sz - dt * v1z;
+    const float tx = fmaf(-s, v1x, sx), ty = fmaf(-s, v1y, sy), tz = fmaf(-s, v1z, sz);
     px = tx;
     py = ty;
     pz = tz;
@@ -791,14 +797,14 @@ __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, con
         float lo = 0.0f, hi = 1.0f;
         for (int it = 0; it < 10; ++it) {
             const float mt = 0.5f * (lo + hi);
-            const int hm = hit_staged(g, locc, bo, sx + (tx - sx) * mt, sy + (ty - sy) * mt, sz + (tz - sz) * mt);
+            const int hm = hit_staged(g, locc, bo, fmaf(tx - sx, mt, sx), fmaf(ty - sy, mt, sy), fmaf(tz - sz, mt, sz));
             if (hm < 0) return false;
             if (hm > 0) hi = mt;
             else lo = mt;
         }
-        px = sx + (tx - sx) * lo;
-        py = sy + (ty - sy) * lo;
-        pz = sz + (tz - sz) * lo;
+        px = fmaf(tx - sx, lo, sx);
+        py = fmaf(ty - sy, lo, sy);
+        pz = fmaf(tz - sz, lo, sz);
     }
     return true;
 }
@@ -807,15 +813,14 @@ __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, con
 __device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* __restrict__ u, const float* __restrict__ v,
                                                    const float* __restrict__ w, const uint8_t* __restrict__ occ, const float* __restrict__ Ttmp,
                                                    const float* __restrict__ rtmp, float* __restrict__ T, float* __restrict__ rho,
-                                                   const float* __restrict__ solid_T, float dt, int x, int y, int z) {
+                                                   const float* __restrict__ solid_T, float s, int x, int y, int z) {
     const int64_t i = cidx(g, x, y, z);
     if (occ[i] != 0) {
         T[i] = solid_T[i];
         rho[i] = 0.0f;
         return;
     }
-    const float h = g.h;
-    const float3 p = trace_rk2(g, u, v, w, occ, dt, (float(x) + 0.5f) * h, (float(y) + 0.5f) * h, (float(z) + 0.5f) * h);
+    const float3 p = trace_rk2_at<-1>(g, u, v, w, occ, s, x, y, z);
     float tv, rv;
     sample_scalar2(g, g.temperature, Ttmp, g.density, rtmp, p.x, p.y, p.z, g.cubic != 0, tv, rv);
     T[i] = tv;
@@ -853,28 +858,28 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     const bool inside = index32(g) && box_inside(bs, n0 - 1, n1 - 1, n2 - 1) && box_inside(bu, n0, n1 - 1, n2 - 1) &&
                         box_inside(bv, n0 - 1, n1, n2 - 1) && box_inside(bw, n0 - 1, n1 - 1, n2);
     if (inside) {
-        for (int q = tid; q < kS; q += kSaX * kSaY) {
+        for (int q = tid; q < kS; q += kSaThreads) {
             const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
             const int c = (k * n1 + j) * n0 + i;
             sT[q] = Ttmp[c];
             sR[q] = rtmp[c];
         }
-        stage_inside(su, bu, u, n0 + 1, n1, tid, kSaX * kSaY);
-        stage_inside(sv, bv, v, n0, n1 + 1, tid, kSaX * kSaY);
-        stage_inside(sw, bw, w, n0, n1, tid, kSaX * kSaY);
-        for (int q = tid; q < kO; q += kSaX * kSaY) {   // bo lies inside bs
+        stage_inside(su, bu, u, n0 + 1, n1, tid, kSaThreads);
+        stage_inside(sv, bv, v, n0, n1 + 1, tid, kSaThreads);
+        stage_inside(sw, bw, w, n0, n1, tid, kSaThreads);
+        for (int q = tid; q < kO; q += kSaThreads) {   // bo lies inside bs
             const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
             so[q] = occ[(k * n1 + j) * n0 + i];
         }
     } else {
-        for (int q = tid; q < kS; q += kSaX * kSaY) {
+        for (int q = tid; q < kS; q += kSaThreads) {
             const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
             sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
             sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
         }
         auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
             const int n = box_size(bx);
-            for (int q = tid; q < n; q += kSaX * kSaY) {
+            for (int q = tid; q < n; q += kSaThreads) {
                 const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
                 lds[q] = load_face(g, f, comp, i, j, k);
             }
@@ -882,21 +887,20 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
         stage_face(0, u, su, bu);
         stage_face(1, v, sv, bv);
         stage_face(2, w, sw, bw);
-        for (int q = tid; q < kO; q += kSaX * kSaY) {
+        for (int q = tid; q < kO; q += kSaThreads) {
             const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
             // entries outside the domain are never read (the hit test returns first)
             so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
         }
     }
     __syncthreads();
-    const int x = X0 + tid % kSaX, y = Y0 + tid / kSaX;
+    const int x = X0 + tid % kSaX, y = Y0 + (tid / kSaX) % kSaY;
     if (x >= g.n[0] || y >= g.n[1]) return;
-    const float h = g.h;
-    const float dt = *dt_ptr;
+    const float cps = *dt_ptr / g.h;   // cells moved per unit velocity and time step
     const bool cubic = g.cubic != 0;
     const bool same_periodic = g.temperature.periodic[0] == g.density.periodic[0] && g.temperature.periodic[1] == g.density.periodic[1] &&
                                g.temperature.periodic[2] == g.density.periodic[2];
-    for (int tz = 0; tz < kSaZ; ++tz) {
+    for (int tz = tid / kSaPlane; tz < kSaZ; tz += kSaZStep) {
         const int z = Z0 + tz;
         if (z >= g.n[2]) break;
         const int64_t i = cidx(g, x, y, z);
@@ -905,9 +909,8 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
             rho[i] = 0.0f;
             continue;
         }
-        const float sx = (float(x) + 0.5f) * h, sy = (float(y) + 0.5f) * h, sz = (float(z) + 0.5f) * h;
         float px, py, pz;
-        bool ok = same_periodic && staged_trace(g, su, bu, sv, bv, sw, bw, so, bo, dt, sx, sy, sz, px, py, pz);
+        bool ok = same_periodic && staged_trace<-1>(g, su, bu, sv, bv, sw, bw, so, bo, cps, x, y, z, px, py, pz);
         Stencil st{};
         if (ok) {
             st = locate(g, g.temperature.periodic, -1, px, py, pz);
@@ -929,7 +932,7 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     }
 }
 
-__global__ __launch_bounds__(kSaX* kSaY) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__global__ __launch_bounds__(kSaThreads) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                            const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                                            const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                                                            float* __restrict__ T, float* __restrict__ rho,
@@ -950,11 +953,11 @@ __device__ __forceinline__ void advect_scalars_fallback_body(const VBlock& vb, G
                                                                       const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
                                                                       const int* __restrict__ fallback, const unsigned* __restrict__ fallback_count) {
     const unsigned n = *fallback_count;
-    const float dt = *dt_ptr;
+    const float cps = *dt_ptr / g.h;
     for (unsigned q = vb.idx.x * 256 + threadIdx.x; q < n; q += vb.dim.x * 256) {
         const int i = fallback[q];
         const int x = i % g.n[0], y = (i / g.n[0]) % g.n[1], z = i / (g.n[0] * g.n[1]);
-        advect_scalar_cell(g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt, x, y, z);
+        advect_scalar_cell(g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, cps, x, y, z);
     }
 }
 
@@ -983,6 +986,7 @@ struct zero_count_k {
 __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx, const Stencil& s) {
     const int sy = bx.d[0], sz = bx.d[0] * bx.d[1];
     const int base = box_base(bx, s.b[0] - 1, s.b[1] - 1, s.b[2] - 1);
+    const Hermite wx = hermite(s.t[0]), wy = hermite(s.t[1]), wz = hermite(s.t[2]);
     float zs[4];
 #pragma unroll
     for (int dz = 0; dz < 4; ++dz) {
@@ -990,11 +994,11 @@ __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx
 #pragma unroll
         for (int dy = 0; dy < 4; ++dy) {
             const int o = base + dz * sz + dy * sy;
-            ys[dy] = monotonic_cubic(lds[o], lds[o + 1], lds[o + 2], lds[o + 3], s.t[0]);
+            ys[dy] = monotonic_cubic(lds[o], lds[o + 1], lds[o + 2], lds[o + 3], wx);
         }
-        zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], s.t[1]);
+        zs[dz] = monotonic_cubic(ys[0], ys[1], ys[2], ys[3], wy);
     }
-    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], s.t[2]);
+    return monotonic_cubic(zs[0], zs[1], zs[2], zs[3], wz);
 }
 
 // ---- LDS-staged velocity advection (the same results as advect_face_kernel).
@@ -1038,17 +1042,17 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     };
     const bool inside = index32(g) && inside_for(C, bf) && inside_for(D0, bg) && inside_for(D1, bg) && box_inside(bo, n0 - 1, n1 - 1, n2 - 1);
     if (inside) {
-        stage_inside(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), tid, kSaX * kSaY);
-        stage_inside(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), tid, kSaX * kSaY);
-        stage_inside(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), tid, kSaX * kSaY);
-        for (int q = tid; q < kO; q += kSaX * kSaY) {
+        stage_inside(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), tid, kSaThreads);
+        stage_inside(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), tid, kSaThreads);
+        stage_inside(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), tid, kSaThreads);
+        for (int q = tid; q < kO; q += kSaThreads) {
             const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
             so[q] = occ[(k * n1 + j) * n0 + i];
         }
     } else {
         auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
             const int n = box_size(bx);
-            for (int q = tid; q < n; q += kSaX * kSaY) {
+            for (int q = tid; q < n; q += kSaThreads) {
                 const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
                 lds[q] = load_face(g, f, comp, i, j, k);
             }
@@ -1056,14 +1060,14 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
         stage_face(C, fc, sf, bf);
         stage_face(D0, f0, sg0, bg);
         stage_face(D1, f1, sg1, bg);
-        for (int q = tid; q < kO; q += kSaX * kSaY) {
+        for (int q = tid; q < kO; q += kSaThreads) {
             const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
             // entries outside the domain are never read (both tests check the domain first)
             so[q] = (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) ? occ[cidx(g, i, j, k)] : uint8_t(1);
         }
     }
     __syncthreads();
-    const int i = X0 + tid % kSaX, j = Y0 + tid / kSaX;
+    const int i = X0 + tid % kSaX, j = Y0 + (tid / kSaX) % kSaY;
     if (i >= g.n[0] || j >= g.n[1]) return;
     // staged copies of u, v, w in component order
     const float* lu = C == 0 ? sf : sg0;
@@ -1072,10 +1076,9 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     const LdsBox& bv = C == 1 ? bf : bg;
     const float* lw = C == 2 ? sf : sg1;
     const LdsBox& bw = C == 2 ? bf : bg;
-    const float h = g.h;
-    const float dt = *dt_ptr;
+    const float cps = *dt_ptr / g.h;   // cells moved per unit velocity and time step
     const bool cubic = g.cubic != 0;
-    for (int tz = 0; tz < kSaZ; ++tz) {
+    for (int tz = tid / kSaPlane; tz < kSaZ; tz += kSaZStep) {
         const int k = Z0 + tz;
         if (k >= g.n[2]) break;
         const int c3[3] = {i, j, k};
@@ -1096,11 +1099,8 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
                 continue;
             }
         }
-        const float sx = C == 0 ? float(i) * h : (float(i) + 0.5f) * h;
-        const float sy = C == 1 ? float(j) * h : (float(j) + 0.5f) * h;
-        const float sz = C == 2 ? float(k) * h : (float(k) + 0.5f) * h;
         float px, py, pz;
-        bool ok = staged_trace(g, lu, bu, lv, bv, lw, bw, so, bo, dt, sx, sy, sz, px, py, pz);
+        bool ok = staged_trace<C>(g, lu, bu, lv, bv, lw, bw, so, bo, cps, i, j, k, px, py, pz);
         Stencil st{};
         if (ok) {
             st = locate(g, g.flow_periodic, C, px, py, pz);
@@ -1115,7 +1115,7 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
 }
 
 template <int C>
-__global__ __launch_bounds__(kSaX* kSaY) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+__global__ __launch_bounds__(kSaThreads) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                         const float* __restrict__ w, float* __restrict__ dst,
                                                                         const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
                                                                         const float* __restrict__ dt_ptr, int* __restrict__ fallback,
@@ -1134,12 +1134,12 @@ __device__ __forceinline__ void advect_face_fallback_body(const VBlock& vb, Grid
                                                                    const float* __restrict__ dt_ptr, const int* __restrict__ fallback,
                                                                    const unsigned* __restrict__ fallback_count) {
     const unsigned n = *fallback_count;
-    const float dt = *dt_ptr;
+    const float cps = *dt_ptr / g.h;
     const int d0 = g.n[0] + (C == 0), d1 = g.n[1] + (C == 1);
     for (unsigned q = vb.idx.x * 256 + threadIdx.x; q < n; q += vb.dim.x * 256) {
         const int fi = fallback[q];
         const int i = fi % d0, j = (fi / d0) % d1, k = fi / (d0 * d1);
-        dst[fi] = advect_face_value<C>(g, u, v, w, occ, dt, i, j, k);
+        dst[fi] = advect_face_value<C>(g, u, v, w, occ, cps, i, j, k);
     }
 }
 
@@ -1162,7 +1162,7 @@ void launch_advect(Context& c, int block_index) {
         // tiles over face indices below n (at most `cells` faces reach the list)
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_face_staged_kernel<C><<<xcd_blocks(tiles), kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C],
+        advect_face_staged_kernel<C><<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C],
                                                                                       c.occupancy, c.solid_velocity[C], c.dt_device,
                                                                                       c.advect_fallback, c.advect_fallback_count,
                                                                                       make_uint3(tiles.x, tiles.y, tiles.z));
@@ -1183,7 +1183,7 @@ void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
     if (block_index == kAdvectStaged) {
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_scalars_staged_kernel<<<xcd_blocks(tiles), kSaX * kSaY, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
+        advect_scalars_staged_kernel<<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
                                                                                       c.temperature_tmp, c.density_tmp, T, rho, c.solid_temperature,
                                                                                       c.dt_device, c.advect_fallback, c.advect_fallback_count,
                                                                                       make_uint3(tiles.x, tiles.y, tiles.z));
@@ -1334,7 +1334,7 @@ unsigned fallback_blocks(const Context& c) { return unsigned(std::min<int64_t>(1
 template <int C>
 void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
     LaunchBuilder<zero_count_k, 64, unsigned*> Z;
-    LaunchBuilder<advect_face_staged_k<C>, kSaX * kSaY, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
+    LaunchBuilder<advect_face_staged_k<C>, kSaThreads, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
                   const float*, int*, unsigned*, uint3>
         S;
     LaunchBuilder<advect_face_fallback_k<C>, 256, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*, const int*,
@@ -1353,7 +1353,7 @@ void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vecto
     }
     if (!S.empty()) {
         out.push_back(Z.finish(A, dim3(1)));
-        out.push_back(S.finish(A, dim3(kSaX * kSaY)));
+        out.push_back(S.finish(A, dim3(kSaThreads)));
         out.push_back(F.finish(A, dim3(256)));
     }
     for (int b = 0; b < kAdvectBlockCount; ++b) {
@@ -1373,7 +1373,7 @@ void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vecto
 
 void batch_advect_scalars(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
     LaunchBuilder<zero_count_k, 64, unsigned*> Z;
-    LaunchBuilder<advect_scalars_staged_k, kSaX * kSaY, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*,
+    LaunchBuilder<advect_scalars_staged_k, kSaThreads, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*,
                   float*, float*, const float*, const float*, int*, unsigned*, uint3>
         S;
     LaunchBuilder<advect_scalars_fallback_k, 256, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*, float*,
@@ -1392,7 +1392,7 @@ void batch_advect_scalars(const std::vector<Context*>& P, TableArena& A, std::ve
     }
     if (!S.empty()) {
         out.push_back(Z.finish(A, dim3(1)));
-        out.push_back(S.finish(A, dim3(kSaX * kSaY)));
+        out.push_back(S.finish(A, dim3(kSaThreads)));
         out.push_back(F.finish(A, dim3(256)));
     }
     for (int b = 0; b < kAdvectBlockCount; ++b) {
@@ -1527,7 +1527,9 @@ void table_advect_blocks(Context& c) {
 void describe_advect_block(int index, SmokeSimulationLaunchShape& out) {
     if (index < 0 || index >= kAdvectCandidates) throw HipError("invalid advection launch");
     if (index == kAdvectStaged) {
-        out = {kSaX, kSaY, 1, kSaZ};
+        // the staged form's block: kSaX x kSaY x kSaZStep threads, each with
+        // kSaZ / kSaZStep outputs of the kSaX x kSaY x kSaZ tile
+        out = {kSaX, kSaY, kSaZStep, kSaZ / kSaZStep};
         return;
     }
     const dim3 b = kAdvectBlocks[index];
