@@ -1,6 +1,8 @@
 """Builds the MI355X smoke library: every csrc/*.hip compiled by hipcc for
 gfx950 (in parallel), linked into lib/libvisual-simulation-of-smoke.so, and the
-HIP counterpart of the reference benchmark CLI (bench/benchmark.hip)."""
+two C-ABI callers: the HIP counterpart of the reference benchmark CLI
+(cli/benchmark.hip) and the reference visualizer's scene driver
+(cli/scene_plume.hip)."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -16,6 +18,8 @@ BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libvisual-simulation-of-smoke.so"
 BENCH_SRC = PKG / "cli" / "benchmark.hip"
 BENCH_BIN = PKG / "lib" / "visual-simulation-of-smoke-benchmark"
+SCENE_SRC = PKG / "cli" / "scene_plume.hip"
+SCENE_BIN = PKG / "lib" / "visual-simulation-of-smoke-scene"
 
 ARCH = os.environ.get("SMOKE_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -48,9 +52,10 @@ def build(verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    if BENCH_SRC.exists():
-        cmd = [HIPCC, *FLAGS, str(BENCH_SRC), "-o", str(BENCH_BIN), f"-L{LIB.parent}", "-lvisual-simulation-of-smoke",
-               "-Wl,-rpath,$ORIGIN"]
+    for src, out in ((BENCH_SRC, BENCH_BIN), (SCENE_SRC, SCENE_BIN)):
+        if not src.exists():
+            continue
+        cmd = [HIPCC, *FLAGS, str(src), "-o", str(out), f"-L{LIB.parent}", "-lvisual-simulation-of-smoke", "-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
