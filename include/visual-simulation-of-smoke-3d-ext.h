@@ -282,6 +282,22 @@ typedef struct SmokeSimulationTrainingCrops {
 
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops);
 
+/* The same crops of the last solve's right-hand side, kept on the device for
+   training without a file: for crop i, r[i * N + q] = b / scale in FP32 (the
+   crop's unit-RMS residual, scale as above, x fastest) and flags[i * N + q]
+   the operator's flag byte, N = size[0] * size[1] * size[2].  `origins` is a
+   host array of count x (x, y, z); r and flags are device buffers of
+   count * N elements.  Ordered on the context's stream (asynchronous). */
+typedef struct SmokeSimulationDeviceCrops {
+    const int32_t* origins;
+    uint32_t count;
+    uint32_t size[3];
+    float* r;
+    uint8_t* flags;
+} SmokeSimulationDeviceCrops;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_copy_training_crops(SmokeSimulationContext context, const SmokeSimulationDeviceCrops* crops);
+
 /* Attaches (or, with NULL, detaches) a learned preconditioner created by
    smoke_nn_preconditioner_create (an opaque SmokeNnPreconditioner*); the
    context does not own it: keep it alive while attached.  Used when the

@@ -75,6 +75,21 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_data(const Smo
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_conv_backward_weights(const SmokeNnConvDesc* desc, const uint16_t* x, const uint16_t* dy, float* dw,
                                                                           float* db, void* stream);
 
+/* Kernel choice for the convolutions above, process-wide.  AUTO (the
+   default) runs the LDS-staged kernels on volumes with many output tiles and
+   the gather kernels (split-K for small volumes) elsewhere; GATHER forces
+   the gather kernels; STAGED runs the staged kernels wherever their shape
+   table covers (mode, cin, cout), at any volume size.  Forward and data
+   gradients give the same bits with either choice; the weight gradient's
+   atomic sums do not have a fixed order.  For tests and measurements. */
+typedef enum SmokeNnConvKernels {
+    SMOKE_NN_CONV_KERNELS_AUTO = 0,
+    SMOKE_NN_CONV_KERNELS_GATHER = 1,
+    SMOKE_NN_CONV_KERNELS_STAGED = 2,
+} SmokeNnConvKernels;
+
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_nn_set_conv_kernels(int32_t choice);
+
 /* The learned preconditioner: a 3-level convolutional encoder-decoder with
    3x3x3 kernels and 16-32-64 channels (stride-2 down, transposed stride-2 up,
    ReLU, skip additions), mapping a residual r to z ~ A^-1 r.

@@ -129,12 +129,17 @@ def apply_A(v, flags):
     return np.where(ident, 0.0, out)
 
 
-def unet_loss_and_gradients(flat, r, flags, rounding=True, loss="residual"):
+def unet_loss_and_gradients(flat, r, flags, rounding=True, loss="residual", abs_sums=None):
     """Loss mean_b ||A z - r||^2 / ||r||^2 (loss="residual") or
     mean_b (z^T A z / 2 - z^T r) / ||r||^2 over the fluid rows (loss="energy")
     and the parameter gradients of the preconditioner network, rounding every
     stored tensor to bfloat16 where the GPU does (rounding=False: exact
-    float64 throughout, for finite-difference checks).  r, flags: (B, D, H, W)."""
+    float64 throughout, for finite-difference checks).  r, flags: (B, D, H, W).
+    With a list `abs_sums`, appends per layer (in layer order) the sum over
+    voxels of |dy| per output channel: the scale of the rounding error of any
+    summation of that layer's bias gradient (which cancels where the terms
+    nearly sum to zero, as the output layer's does on a large crop: its terms
+    sum to (A 1)^T q, non-zero only next to the crop's faces and solids)."""
     bf16_ = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     f32 = (lambda a: np.asarray(a, np.float32).astype(np.float64)) if rounding else (lambda a: np.asarray(a, np.float64))
     P = split_parameters(flat)
@@ -185,9 +190,12 @@ def unet_loss_and_gradients(flat, r, flags, rounding=True, loss="residual"):
 
     grads = [None] * len(LAYERS)
 
+    sums = [None] * len(LAYERS)
+
     def wgrad(li, x, dy):
         mode = LAYERS[li][0]
         grads[li] = backward_weights(x, dy, mode)
+        sums[li] = np.abs(dy).reshape(-1, dy.shape[-1]).sum(0)
 
     def dgrad(li, dy, mask, add=None):
         mode = LAYERS[li][0]
@@ -218,4 +226,6 @@ def unet_loss_and_gradients(flat, r, flags, rounding=True, loss="residual"):
     g0b = dgrad(1, g0a, T["a0"])
     wgrad(0, T["in"], g0b)
     flat_grads = np.concatenate([np.concatenate([dw.ravel(), db.ravel()]) for dw, db in grads])
+    if abs_sums is not None:
+        abs_sums.extend(sums)
     return loss, flat_grads

@@ -178,3 +178,79 @@ def test_energy_loss_training_step_matches_float64_network(lib):
         for p in ptrs:
             hip.free(p)
         L.smoke_nn_preconditioner_destroy(net)
+
+
+@pytest.mark.timeout(300)
+def test_training_on_64_cubed_crops_of_real_solves(lib):
+    """64^3 crops of the 128^3 plume's Jacobi PCG solves (the training data
+    of the 64^3-crop network, kept on the device): one crop's loss and every
+    layer's gradients against the float64 network at the shape the network
+    trains on, then Adam on a batch of four crops lowers the loss."""
+    import train_preconditioner as tp
+    L, hip = lib
+    n, crop, count = 128, 64, 4
+    cells = crop ** 3
+    d_r, d_f = hip.malloc(4 * cells * count), hip.malloc(cells * count)
+    net = C.c_void_p()
+    assert L.smoke_nn_preconditioner_create(11, C.byref(net)) == 0
+    try:
+        cfg, src = tp.plume_config(n)
+        with abi.SmokeContext(cfg, hip=hip) as ctx:
+            ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI, relative_tolerance=1e-6,
+                                   max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1, cfl=0.5, dt_max=0.01,
+                                   check_interval=32)
+            for k in range(6):
+                ctx.plume_sources(src, k)
+                ctx.step()
+            # crops around the plume (the source sits at x = z = 64, y = 16)
+            origins = [(32, 0, 32), (0, 0, 0), (64, 10, 32), (16, 40, 48)]
+            ctx.copy_training_crops(origins, (crop, crop, crop), d_r, d_f)
+            ctx.synchronize()
+        r = hip.download(d_r, (count, crop, crop, crop))
+        flags = hip.download(d_f, (count, crop, crop, crop), np.uint8)
+        assert np.isfinite(r).all() and np.abs(r[0]).max() > 0
+
+        npar = L.smoke_nn_preconditioner_parameter_count(net)
+        params = np.empty(npar, np.float32)
+        assert L.smoke_nn_preconditioner_get_parameters(net, params.ctypes.data) == 0
+        cd = (C.c_int32 * 3)(crop, crop, crop)
+        loss = C.c_double()
+        grads = np.empty(npar, np.float32)
+        print("64^3 crop: GPU training step", flush=True)
+        assert L.smoke_nn_preconditioner_train_step(net, d_r, d_f, 1, cd, 0.0, C.byref(loss), grads.ctypes.data, None) == 0
+        print("64^3 crop: float64 network", flush=True)
+        abs_sums = []
+        ref_loss, ref_grads = ref.unet_loss_and_gradients(params, r[:1], flags[:1], abs_sums=abs_sums)
+        rel_loss = abs(loss.value - ref_loss) / ref_loss
+        worst, worst_bias, off = 0.0, 0.0, 0
+        for li, (_, cin, cout, *_) in enumerate(ref.LAYERS):
+            # weights: relative L2 (FP32 against float64 sums, rare bfloat16 flips)
+            size = 27 * cin * cout
+            g, rg = grads[off:off + size], ref_grads[off:off + size]
+            off += size
+            e = np.linalg.norm(g - rg) / max(np.linalg.norm(rg), 1e-30)
+            worst = max(worst, e)
+            assert e <= 3e-2, (li, "weights", e)
+            # biases: each channel a sum over the layer's voxels, relative L2
+            # as for the weights plus the rounding scale of those sums (3e-3 of
+            # the vector of sum |dy|), which dominates where a sum cancels (the
+            # output layer's: its terms add up to the crop-face terms only)
+            g, rg = grads[off:off + cout], ref_grads[off:off + cout]
+            off += cout
+            err = np.linalg.norm(g - rg)
+            bound = 3e-2 * np.linalg.norm(rg) + 3e-3 * np.linalg.norm(abs_sums[li])
+            worst_bias = max(worst_bias, float(err / bound))
+            assert err <= bound, (li, "bias", err, np.linalg.norm(rg), np.linalg.norm(abs_sums[li]))
+        print(f"PARITY train64 loss_rel={rel_loss:.3e} worst_layer_weight_grad_rel={worst:.3e} "
+              f"worst_bias_error_over_bound={worst_bias:.3e}", flush=True)
+        assert rel_loss <= 1e-3
+        first = None
+        for _ in range(20):
+            assert L.smoke_nn_preconditioner_train_step(net, d_r, d_f, count, cd, 1e-3, C.byref(loss), None, None) == 0
+            first = loss.value if first is None else first
+        print(f"64^3 crops: loss {first:.4f} -> {loss.value:.4f} over 20 Adam steps", flush=True)
+        assert loss.value < 0.7 * first, (first, loss.value)
+    finally:
+        hip.free(d_r)
+        hip.free(d_f)
+        L.smoke_nn_preconditioner_destroy(net)

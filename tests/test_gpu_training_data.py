@@ -2,7 +2,9 @@
 training_data.read and checked bitwise against the solve they came from:
 scale = RMS of b over the crop, r = bf16(b / scale), z = bf16(x / scale)
 with x the FP64 solution, and the operator flags (identity rows for solids
-and the anchor, fluid-neighbour bits otherwise)."""
+and the anchor, fluid-neighbour bits otherwise).  The device-resident copy
+of the same crops (smoke_simulation_ext_copy_training_crops) holds FP32
+b / scale and the same flags."""
 import numpy as np
 import pytest
 
@@ -43,14 +45,26 @@ def test_training_crops_roundtrip(tmp_path):
     origins = [(0, 0, 0), (24, 28, 24), (10, 5, 7)]
     size = (16, 20, 12)
     path = tmp_path / "pairs.bin"
-    with abi.SmokeContext(cfg) as ctx:
-        load_gpu(ctx, st)
-        tolerance_options(ctx)
-        ctx.step(2)
-        ctx.write_training_crops(path, origins[:2], size, step=2)
-        ctx.write_training_crops(path, origins[2:], size, step=2)   # appends
-        b = ctx.view("FLOW_PRESSURE_RHS")
-        x = ctx.export_pressure_f64()
+    hip = abi.Hip()
+    cells = size[0] * size[1] * size[2]
+    d_r, d_f = hip.malloc(4 * cells * 3), hip.malloc(cells * 3)
+    try:
+        with abi.SmokeContext(cfg, hip=hip) as ctx:
+            load_gpu(ctx, st)
+            tolerance_options(ctx)
+            ctx.step(2)
+            ctx.write_training_crops(path, origins[:2], size, step=2)
+            ctx.write_training_crops(path, origins[2:], size, step=2)   # appends
+            # the same crops kept on the device (FP32 r)
+            ctx.copy_training_crops(origins, size, d_r, d_f)
+            ctx.synchronize()
+            b = ctx.view("FLOW_PRESSURE_RHS")
+            x = ctx.export_pressure_f64()
+        dev_r = hip.download(d_r, (3, size[2], size[1], size[0]))
+        dev_f = hip.download(d_f, (3, size[2], size[1], size[0]), np.uint8)
+    finally:
+        hip.free(d_r)
+        hip.free(d_f)
     d = td.read(path)
     man = td.read_manifest(path)
     assert d["crop"] == size and d["grid"] == (40, 48, 36) and len(d["scale"]) == 3
@@ -67,3 +81,6 @@ def test_training_crops_roundtrip(tmp_path):
         np.testing.assert_array_equal(d["r_bits"][k], td.float32_to_bf16((bc / s).astype(np.float32)))
         np.testing.assert_array_equal(d["z_bits"][k], td.float32_to_bf16((x[sl] / s).astype(np.float32)))
         np.testing.assert_array_equal(d["flags"][k], flags[sl])
+        # device crops: FP32 b / scale (the file's r is its bfloat16) and the same flags
+        np.testing.assert_array_equal(dev_r[k], (bc / s).astype(np.float32))
+        np.testing.assert_array_equal(dev_f[k], flags[sl])

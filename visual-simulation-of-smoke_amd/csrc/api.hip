@@ -873,6 +873,11 @@ SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationC
     return smoke::guarded_call(context, [&](Context& c) { smoke::write_training_crops(c, *crops); });
 }
 
+SmokeSimulationResult smoke_simulation_ext_copy_training_crops(SmokeSimulationContext context, const SmokeSimulationDeviceCrops* crops) {
+    if (!crops) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
+    return smoke::guarded_call(context, [&](Context& c) { smoke::copy_training_crops(c, *crops); });
+}
+
 SmokeSimulationResult smoke_simulation_ext_set_learned_preconditioner(SmokeSimulationContext context, void* network) {
     return smoke::guarded_call(context, [&](Context& c) {
         if (network) smoke::ensure_pcg_z(c);

@@ -251,6 +251,7 @@ void multilevel_drain_timing(Context& c);                  // fold pending sweep
 
 // Training-pair crops of the last pressure solve (training_data.hip).
 void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& crops);
+void copy_training_crops(Context& c, const SmokeSimulationDeviceCrops& crops);
 void describe_advect_block(int index, SmokeSimulationLaunchShape& out);
 int find_advect_block(const SmokeSimulationLaunchShape& shape);   // candidate index; throws when none matches
 int tune_advect_blocks(Context& c, SmokeSimulationLaunchReport& report);

@@ -101,6 +101,18 @@ __global__ __launch_bounds__(256) void crop_pack_kernel(CropArgs a, const int* _
     f_out[o] = flags[i];
 }
 
+__global__ __launch_bounds__(256) void crop_copy_kernel(CropArgs a, const int* __restrict__ origins, const float* __restrict__ b,
+                                                        const uint8_t* __restrict__ flags, const double* __restrict__ scales,
+                                                        float* __restrict__ r_out, uint8_t* __restrict__ f_out) {
+    const int crop = blockIdx.y;
+    const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (q >= a.cells) return;
+    const int64_t i = crop_cell(a, origins, crop, q);
+    const int64_t o = int64_t(crop) * a.cells + q;
+    r_out[o] = float(double(b[i]) / scales[crop]);
+    f_out[o] = flags[i];
+}
+
 // Device buffer freed on scope exit (this call is offline logging, so it may
 // allocate).
 template <typename T>
@@ -124,20 +136,47 @@ struct File {
 
 }  // namespace
 
-void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& d) {
-    if (!d.path || !d.origins || d.count == 0) throw HipError("training crops: missing path or origins");
+namespace {
+
+CropArgs crop_args(const Context& c, const int32_t* origins, uint32_t count, const uint32_t size[3]) {
+    if (!origins || count == 0) throw HipError("training crops: missing origins");
     CropArgs a{};
     for (int k = 0; k < 3; ++k) {
         a.n[k] = c.grid.n[k];
-        a.c[k] = int(d.size[k]);
-        if (d.size[k] == 0 || d.size[k] > uint32_t(c.grid.n[k])) throw HipError("training crops: bad crop size");
+        a.c[k] = int(size[k]);
+        if (size[k] == 0 || size[k] > uint32_t(c.grid.n[k])) throw HipError("training crops: bad crop size");
     }
-    for (uint32_t i = 0; i < d.count; ++i)
+    for (uint32_t i = 0; i < count; ++i)
         for (int k = 0; k < 3; ++k) {
-            const int o = d.origins[3 * i + k];
+            const int o = origins[3 * i + k];
             if (o < 0 || int64_t(o) + a.c[k] > a.n[k]) throw HipError("training crops: crop outside the grid");
         }
     a.cells = int64_t(a.c[0]) * a.c[1] * a.c[2];
+    return a;
+}
+
+}  // namespace
+
+void copy_training_crops(Context& c, const SmokeSimulationDeviceCrops& d) {
+    if (!d.r || !d.flags) throw HipError("training crops: missing output buffers");
+    const CropArgs a = crop_args(c, d.origins, d.count, d.size);
+    const int rr_blocks = int((a.cells + kChunk - 1) / kChunk);
+    DeviceBuffer<int> origins(3 * size_t(d.count));
+    DeviceBuffer<double> partials(size_t(rr_blocks) * d.count);
+    DeviceBuffer<double> scales(d.count);
+    hip_check(hipMemcpyAsync(origins.p, d.origins, 3 * size_t(d.count) * sizeof(int), hipMemcpyHostToDevice, c.stream), "crop origins");
+    crop_rr_kernel<<<dim3(unsigned(rr_blocks), d.count), 256, 0, c.stream>>>(a, origins.p, c.pressure_rhs, partials.p);
+    crop_scale_kernel<<<d.count, 64, 0, c.stream>>>(partials.p, rr_blocks, a.cells, scales.p);
+    crop_copy_kernel<<<dim3(unsigned((a.cells + 255) / 256), d.count), 256, 0, c.stream>>>(a, origins.p, c.pressure_rhs, c.stencil_flags,
+                                                                                          scales.p, d.r, d.flags);
+    hip_check(hipGetLastError(), "training crop kernels");
+    // the scratch buffers are freed on return: wait for the kernels that read them
+    hip_check(hipStreamSynchronize(c.stream), "training crops sync");
+}
+
+void write_training_crops(Context& c, const SmokeSimulationTrainingCrops& d) {
+    if (!d.path) throw HipError("training crops: missing path");
+    const CropArgs a = crop_args(c, d.origins, d.count, d.size);
     const size_t n = size_t(a.cells) * d.count;
     const int rr_blocks = int((a.cells + kChunk - 1) / kChunk);
 

@@ -129,6 +129,10 @@ class TrainingCrops(C.Structure):
     _fields_ = [("path", C.c_char_p), ("origins", C.c_void_p), ("count", C.c_uint32), ("size", C.c_uint32 * 3), ("step", C.c_uint32)]
 
 
+class DeviceCrops(C.Structure):
+    _fields_ = [("origins", C.c_void_p), ("count", C.c_uint32), ("size", C.c_uint32 * 3), ("r", C.c_void_p), ("flags", C.c_void_p)]
+
+
 class MultilevelOptions(C.Structure):
     _fields_ = [("levels", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("omega", C.c_float),
                 ("coarse_solver", C.c_uint32), ("coarse_sweeps", C.c_int32), ("reserved", C.c_int32 * 2)]
@@ -254,6 +258,10 @@ class Hip:
         self.check(self.lib.hipMemcpy(C.c_void_p(p), C.c_void_p(arr.ctypes.data), arr.nbytes, self.H2D), "hipMemcpy H2D")
         return p
 
+    def copy(self, dst: int, src: int, nbytes: int):
+        """Device-to-device copy (ordered on the null stream)."""
+        self.check(self.lib.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), int(nbytes), self.D2D), "hipMemcpy D2D")
+
     def download(self, p: int, shape, dtype=np.float32) -> np.ndarray:
         out = np.empty(shape, dtype=dtype)
         self.check(self.lib.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(p), out.nbytes, self.D2H), "hipMemcpy D2H")
@@ -310,6 +318,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_set_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
+        "smoke_simulation_ext_copy_training_crops": [C.c_void_p, C.POINTER(DeviceCrops)],
         "smoke_simulation_ext_set_learned_preconditioner": [C.c_void_p, C.c_void_p],
         "smoke_simulation_ext_default_multilevel_options": [C.POINTER(MultilevelOptions)],
         "smoke_simulation_ext_set_multilevel_options": [C.c_void_p, C.POINTER(MultilevelOptions)],
@@ -499,6 +508,14 @@ class SmokeContext:
         o = np.ascontiguousarray(np.asarray(origins, np.int32).reshape(-1, 3))
         d = TrainingCrops(str(path).encode(), o.ctypes.data, o.shape[0], (C.c_uint32 * 3)(*size), step)
         check(self.lib.smoke_simulation_ext_write_training_crops(self.handle, C.byref(d)), "write_training_crops")
+
+    def copy_training_crops(self, origins, size, d_r: int, d_flags: int):
+        """Crops (lower corners `origins`, extent `size` = (sx, sy, sz)) of the
+        last solve's unit-RMS right-hand side (FP32) and flags into device
+        buffers d_r / d_flags (count * N elements each)."""
+        o = np.ascontiguousarray(np.asarray(origins, np.int32).reshape(-1, 3))
+        d = DeviceCrops(o.ctypes.data, o.shape[0], (C.c_uint32 * 3)(*size), d_r, d_flags)
+        check(self.lib.smoke_simulation_ext_copy_training_crops(self.handle, C.byref(d)), "copy_training_crops")
 
     def set_learned_preconditioner(self, network):
         """Attaches a learned preconditioner (a LearnedPreconditioner or None)."""

@@ -12,6 +12,15 @@ FP32 Adam) on one GPU.
    printing the loss ||A z - r||^2 / ||r||^2 as it goes;
 3. saves the weights (SMOKENN1) to `--out`.
 
+With `--device-crops` the crops never leave the GPU (the configuration the
+north star's training run is specified on: the 256^3 plume's solves cut into
+64^3 crops, batch 16, 30 epochs over `--samples` crops, Adam lr 1e-3): every
+solve's crops are copied into one device array
+(smoke_simulation_ext_copy_training_crops), each batch is gathered from it
+by device-to-device copies in a per-epoch shuffled order, the last
+`--validation` crops (from the latest solves) are held out and scored after
+every epoch, and a JSON summary line ends the run.
+
 With `--coarse` the network is trained as the coarse-grid solver of the
 multilevel preconditioner instead: the plume (and, with `--spheres`, the
 obstacle scene) runs with flexible PCG preconditioned by the V-cycle whose
@@ -127,6 +136,86 @@ def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plu
     return {"r": np.concatenate(rs), "flags": np.ascontiguousarray(np.concatenate(fs)), "crop": dims}
 
 
+def generate_device_crops(hip: abi.Hip, n: int, samples: int, crop: int, per_step: int, seed: int = 0, log=print):
+    """Runs the n^3 plume with Jacobi PCG to a true residual of 1e-6 and keeps
+    `per_step` random crop^3 crops of every solve's unit-RMS right-hand side
+    and flags on the device until `samples` crops exist (the first two solves
+    are skipped: zero or tiny right-hand sides).  Returns (d_r, d_flags,
+    steps run); the caller frees the two buffers."""
+    cfg, src = plume_config(n)
+    cells = crop ** 3
+    d_r = hip.malloc(4 * cells * samples)
+    d_f = hip.malloc(cells * samples)
+    rng = np.random.default_rng(seed)
+    have, k = 0, 0
+    t0 = time.time()
+    with abi.SmokeContext(cfg, hip=hip) as ctx:
+        ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_JACOBI, relative_tolerance=1e-6,
+                               max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1, cfl=0.5, dt_max=0.01,
+                               check_interval=32)
+        while have < samples:
+            ctx.plume_sources(src, k)
+            ctx.step()
+            if k >= 2:
+                m = min(per_step, samples - have)
+                origins = rng.integers(0, n - crop + 1, size=(m, 3))
+                ctx.copy_training_crops(origins, (crop, crop, crop), d_r + 4 * cells * have, d_f + cells * have)
+                have += m
+            if k % 20 == 0:
+                log(f"data: step {k}, {ctx.stats().iterations} PCG iterations, {have} crops ({time.time() - t0:.0f} s)", flush=True)
+            k += 1
+        ctx.synchronize()
+    return d_r, d_f, k
+
+
+def train_device(net: abi.LearnedPreconditioner, hip: abi.Hip, d_r: int, d_f: int, count: int, validation: int, dims, batch: int, lr: float,
+                 epochs: int, time_limit: float, seed: int = 0, log=print) -> dict:
+    """Adam on the first count - validation device crops in batches of
+    `batch` (a fresh shuffle per epoch, gathered by device copies); after each
+    epoch the mean loss over the held-out crops (lr 0: no update).  Returns
+    the run's record."""
+    cells = int(np.prod(dims))
+    n_train = count - validation
+    d_rb = hip.malloc(4 * cells * batch)
+    d_fb = hip.malloc(cells * batch)
+    rng = np.random.default_rng(seed)
+    rec = {"train_crops": n_train, "validation_crops": validation, "batch": batch, "lr": lr, "epochs_planned": epochs, "epochs": [],
+           "steps": 0}
+    t0 = time.time()
+    last = t0
+    try:
+        for epoch in range(epochs):
+            order = rng.permutation(n_train)
+            losses = []
+            for b0 in range(0, n_train - batch + 1, batch):
+                for j, idx in enumerate(order[b0:b0 + batch]):
+                    hip.copy(d_rb + 4 * cells * j, d_r + 4 * cells * int(idx), 4 * cells)
+                    hip.copy(d_fb + cells * j, d_f + cells * int(idx), cells)
+                losses.append(net.train_step(d_rb, d_fb, batch, dims, lr))
+                rec["steps"] += 1
+                now = time.time()
+                if now - last > 10.0:
+                    log(f"train: epoch {epoch} step {len(losses)} loss {np.mean(losses[-50:]):.4f} ({now - t0:.0f} s)", flush=True)
+                    last = now
+                if now - t0 > time_limit:
+                    break
+            val = []
+            for v0 in range(n_train, count - batch + 1, batch):
+                val.append(net.train_step(d_r + 4 * cells * v0, d_f + cells * v0, batch, dims, 0.0))
+            rec["epochs"].append({"epoch": epoch, "steps": len(losses), "train_loss": float(np.mean(losses)) if losses else None,
+                                  "validation_loss": float(np.mean(val)) if val else None, "seconds": round(time.time() - t0, 1)})
+            log(f"epoch {epoch}: train loss {rec['epochs'][-1]['train_loss']:.4f}, validation loss "
+                f"{rec['epochs'][-1]['validation_loss']:.4f} ({time.time() - t0:.0f} s)", flush=True)
+            if time.time() - t0 > time_limit:
+                break
+    finally:
+        hip.free(d_rb)
+        hip.free(d_fb)
+    rec["seconds"] = round(time.time() - t0, 1)
+    rec["ms_per_step"] = round(1e3 * rec["seconds"] / max(rec["steps"], 1), 2)
+    return rec
+
+
 def train(net: abi.LearnedPreconditioner, data: dict, batch: int, lr: float, epochs: int, time_limit: float, seed: int = 0, log=print):
     """Adam on the crops (r scaled back from bfloat16, unit RMS); returns the loss history."""
     hip = abi.Hip()
@@ -183,7 +272,30 @@ def main():
     ap.add_argument("--capacity", type=int, default=1500, help="coarse volumes captured per scene")
     ap.add_argument("--spheres", action="store_true", help="also capture the obstacle scene's solves")
     ap.add_argument("--sweeps", type=int, default=3, help="pre- and post-sweeps of the V-cycle that produces the coarse data")
+    ap.add_argument("--device-crops", action="store_true", help="keep the crops on the GPU (the 20k-crop, 30-epoch run)")
+    ap.add_argument("--samples", type=int, default=20000, help="training crops with --device-crops")
+    ap.add_argument("--validation", type=int, default=512, help="held-out crops with --device-crops")
     args = ap.parse_args()
+    if args.device_crops:
+        import json
+        hip = abi.Hip()
+        total = args.samples + args.validation
+        t0 = time.time()
+        d_r, d_f, steps = generate_device_crops(hip, args.grid, total, args.crop, args.crops_per_step, args.seed)
+        data_s = time.time() - t0
+        try:
+            with abi.LearnedPreconditioner(seed=args.seed) as net:
+                rec = train_device(net, hip, d_r, d_f, total, args.validation, (args.crop,) * 3, args.batch, args.lr, args.epochs,
+                                   args.time_limit, args.seed)
+                net.save(args.out)
+        finally:
+            hip.free(d_r)
+            hip.free(d_f)
+        done = rec["steps"] * args.batch / (args.samples * args.epochs)
+        rec.update({"grid": args.grid, "crop": args.crop, "solves": steps, "data_seconds": round(data_s, 1),
+                    "fraction_of_samples_x_epochs": round(done, 4), "weights": str(args.out)})
+        print(json.dumps(rec), flush=True)
+        return
     if args.coarse:
         scenes = ("plume", "spheres") if args.spheres else ("plume",)
         data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes, sweeps=args.sweeps)
