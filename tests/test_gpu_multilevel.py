@@ -134,15 +134,27 @@ def test_multilevel_solve_reaches_tolerance_in_few_iterations():
 
 
 def test_coarse_network_gains_over_the_cycle_without_it():
+    """The network at the 16^3 coarsest level against (a) the same cycle with
+    that level only smoothed and (b) the best cycle without a network: the
+    hierarchy continued down to 4^3 with V(4,4) on every level and 8 sweeps
+    there (the bench's classical solver).  The network must save at least 30 %
+    of (a)'s iterations.  It does not beat (b): on this 64^3 scene the
+    classical cycle needs fewer iterations (measured 44 against 56 over the 8
+    steps; at 256^3 both need 4.0 per step), so (b) is a regression bound --
+    the network within 1.5x of the classical cycle's iterations."""
     if not COARSE_WEIGHTS.exists():
         pytest.skip("coarse-grid weights not trained yet")
-    base = dict(levels=3, pre_sweeps=2, post_sweeps=2, omega=2.0 / 3.0)   # 16^3 coarsest level, as trained
+    base = dict(pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0)
     with abi.LearnedPreconditioner(seed=0, path=COARSE_WEIGHTS) as net:
-        net_its, net_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, net=net, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=0, **base)
-    smooth_its, smooth_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=4, **base)
-    print("network coarse", net_its, "smoothed coarse", smooth_its, flush=True)
-    assert max(net_res) <= 1e-6 and max(smooth_res) <= 1e-6
+        net_its, net_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, net=net, levels=3, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=0,
+                                            **base)
+    smooth_its, smooth_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, levels=3, coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=16, **base)
+    deep_its, deep_res = _obstacle_solves(abi.PRECONDITIONER_MULTILEVEL, levels=5, coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=8, **base)
+    print("network coarse", net_its, "smoothed coarse", smooth_its, "classical to 4^3", deep_its, flush=True)
+    print(f"PARITY coarse_network_iterations network={sum(net_its)} smoothed={sum(smooth_its)} classical={sum(deep_its)}", flush=True)
+    assert max(net_res) <= 1e-6 and max(smooth_res) <= 1e-6 and max(deep_res) <= 1e-6
     assert sum(net_its) < 0.7 * sum(smooth_its), (net_its, smooth_its)
+    assert sum(net_its) <= 1.5 * sum(deep_its), (net_its, deep_its)
 
 
 def test_network_coarse_solver_requires_a_network():
@@ -180,3 +192,52 @@ def test_multilevel_solve_on_vector_row_grid():
             assert st.learned_fallback == 0, k
             assert st.relative_residual <= 1e-6, (k, st.relative_residual)
             assert st.iterations < 40, (k, st.iterations)
+
+
+def test_coarse_network_solve_in_few_launches_matches_the_general_path():
+    """The coarse-grid network solve runs as two single-block kernels around
+    the convolutions (deflation, normalisation, input; output and deflated
+    correction); with a capture buffer attached the V-cycle takes the general
+    path (separate deflation, sum and input kernels).  Both apply the same
+    operator: the sums run in other orders, so the network's bfloat16 input
+    may round differently in a few cells, and the V-cycles agree to that."""
+    if not COARSE_WEIGHTS.exists():
+        pytest.skip("coarse-grid weights not trained yet")
+    import train_preconditioner as tp
+    n = 64
+    cfg, src = tp.plume_config(n)
+    occ = _spheres((n, n, n), 6, 7)
+    occ[: n // 5] = 0
+    hip = abi.Hip()
+    with abi.LearnedPreconditioner(seed=0, path=COARSE_WEIGHTS) as net, abi.SmokeContext(cfg, hip=hip) as ctx:
+        ctx.set_occupancy(occ)
+        ctx.set_multilevel_options(levels=3, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=0)
+        ctx.set_learned_preconditioner(net)
+        _tolerance(ctx, abi.PRECONDITIONER_MULTILEVEL)
+        for k in range(4):
+            ctx.plume_sources(src, k)
+            ctx.step()
+        r = ctx.view("FLOW_PRESSURE_RHS").astype(np.float32)
+        d_r, d_z = hip.upload(r), hip.malloc(r.nbytes)
+        cap = hip.malloc(4 * 16 ** 3 * 4)
+        try:
+            ctx.apply_multilevel(d_r, d_z)
+            ctx.synchronize()
+            z_few = hip.download(d_z, r.shape)
+            ctx.apply_multilevel(d_r, d_z)
+            ctx.synchronize()
+            z_again = hip.download(d_z, r.shape)
+            ctx.set_multilevel_capture(cap, 4)
+            ctx.apply_multilevel(d_r, d_z)
+            ctx.synchronize()
+            z_general = hip.download(d_z, r.shape)
+            ctx.set_multilevel_capture(None, 0)
+        finally:
+            hip.free(d_r)
+            hip.free(d_z)
+            hip.free(cap)
+        ctx.set_learned_preconditioner(None)
+    np.testing.assert_array_equal(z_few, z_again)             # deterministic
+    e = float(np.linalg.norm(z_few.astype(np.float64) - z_general) / np.linalg.norm(z_general))
+    print(f"PARITY coarse_network_few_launches_vs_general rel={e:.3e}", flush=True)
+    assert np.isfinite(z_few).all() and e < 1e-3

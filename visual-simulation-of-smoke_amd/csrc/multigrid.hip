@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void mg_deflate_rhs_kernel(int cells, const ui
     if (blockIdx.x == 0 && threadIdx.x == 0) defl[1] = c;
     const float cf = float(c);
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < cells) rd[i] = (flags[i] & kIdentity) ? 0.0f : r[i] - cf * av[i];
+    if (i < cells) rd[i] = (flags[i] & kIdentity) ? 0.0f : fmaf(-cf, av[i], r[i]);
 }
 
 // z = z' + (c - Av.z' / v.Av) v
@@ -662,6 +662,13 @@ void vcycle(Context& c, int l) {
     const Coarse none{nullptr, 0, 0};
     const auto& o = m.options;
     if (l == m.built - 1) {
+        const bool capturing = m.capture && m.captured < m.capacity;
+        if (o.coarse_solver == SMOKE_SIMULATION_COARSE_NETWORK && o.coarse_sweeps == 0 && !capturing && L.cells <= kCoarseSolveMaxCells) {
+            // deflation, network and deflated correction in few launches
+            if (!c.learned) throw HipError("multilevel preconditioner: no network attached");
+            preconditioner_coarse_solve(*c.learned, L.r, L.av, L.flags, L.defl, L.n, L.rd, L.z, c.stream);
+            return;
+        }
         // deflated right-hand side (what the coarse solver sees, and what the
         // capture stores as training data)
         const int cells = int(L.cells);

@@ -116,7 +116,30 @@ __global__ __launch_bounds__(SPLIT == 1 ? 256 : 64 * SPLIT) void conv_kernel(Con
     for (int nt = 0; nt < NT; ++nt) acc[nt] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
     const int ksteps = a.kpad / kStep;
     const int first = SPLIT == 1 ? 0 : wib;
-    if (first < ksteps) {
+    if constexpr (SPLIT > 1) {
+        // a wave's few K steps (up to 4 per round): every load issued before
+        // the round's matrix-core work, which keeps the wave's K order
+        constexpr int ROUND = NT >= 3 ? 2 : 4;   // within the 128 VGPRs of a 1024-thread block
+        for (int s0 = first; s0 < ksteps; s0 += ROUND * SPLIT) {
+            bf16x8 ar[ROUND], br[ROUND][NT];
+#pragma unroll
+            for (int j = 0; j < ROUND; ++j) {
+                const int s = s0 + j * SPLIT;
+                ar[j] = zero8();
+                if (s < ksteps) {
+                    ar[j] = load_a(s);
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) br[j][nt] = load8(w + int64_t(nt * 16 + m) * a.kpad + s * kStep + g * 8);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < ROUND; ++j) {
+                if (s0 + j * SPLIT >= ksteps) break;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[j], br[j][nt], acc[nt], 0, 0, 0);
+            }
+        }
+    } else if (first < ksteps) {
         bf16x8 av = load_a(first);
         bf16x8 bv[NT];
 #pragma unroll

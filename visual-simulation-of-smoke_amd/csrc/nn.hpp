@@ -27,4 +27,13 @@ void backward_weights(const SmokeNnConvDesc& d, const uint16_t* x, const uint16_
 struct SmokeNnPreconditioner;
 namespace smoke {
 void preconditioner_apply(SmokeNnPreconditioner& net, const float* r, const uint8_t* flags, const int dims[3], float* z, hipStream_t s);
+
+// The multilevel preconditioner's coarse-grid solve with the network: the
+// deflation of v = 1 (defl[0] = v^T A v on entry; writes defl[1] = c and
+// the deflated right-hand side rd), z' = M(rd) and z = z' + (c - Av.z' /
+// v.Av) v, in two single-block launches around the network's convolutions.
+// For volumes of at most kCoarseSolveMaxCells cells.
+constexpr int64_t kCoarseSolveMaxCells = 8192;
+void preconditioner_coarse_solve(SmokeNnPreconditioner& net, const float* r, const float* av, const uint8_t* flags, double* defl, const int dims[3],
+                                 float* rd, float* z, hipStream_t s);
 }  // namespace smoke

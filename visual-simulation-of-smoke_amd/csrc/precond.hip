@@ -291,6 +291,108 @@ __global__ __launch_bounds__(256) void output_kernel(const float* __restrict__ o
         z[i] = (flags[i] & kIdentity) ? 0.0f : s * out[i];
 }
 
+// ---- the multilevel preconditioner's coarse-grid solve in few launches ----
+//
+// The coarsest level is small (16^3 at 256^3 with five levels), where every
+// launch costs more than its work: the deflation of v = 1 (c = v.r / v.Av,
+// rd = r - c A v), the network input (rd / s with s the RMS of rd) and, after
+// the network, z = s out + (c - Av.z / v.Av) v run as one single-block
+// kernel before the convolutions and one after.  Sums run in a fixed order
+// (each thread's cells in order, wave shuffles, then the waves in order).
+constexpr int kCoarseThreads = 1024;
+
+__device__ double block_sum_fixed(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();   // the previous sum's reads of red are done
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < kCoarseThreads / 64; ++k) t += red[k];
+    return t;
+}
+
+// Each thread owns cells t, t + 1024, ... (at most kCoarsePerThread of them),
+// loaded once into registers: one memory latency per kernel.
+constexpr int kCoarsePerThread = 8;
+
+// defl[0] = v.Av (set up with the hierarchy); writes defl[1] = c, rd, the
+// network input in8 and sums[0] = ||rd||^2 (the output's scale)
+__global__ __launch_bounds__(kCoarseThreads) void coarse_prep_kernel(const float* __restrict__ r, const float* __restrict__ av,
+                                                                     const uint8_t* __restrict__ flags, int cells, double* __restrict__ defl,
+                                                                     float* __restrict__ rd, double* __restrict__ sums,
+                                                                     uint16_t* __restrict__ in8) {
+    __shared__ double red[kCoarseThreads / 64];
+    const int t = int(threadIdx.x);
+    float rv[kCoarsePerThread], avv[kCoarsePerThread];
+    bool fluid[kCoarsePerThread];
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        const int i = t + k * kCoarseThreads;
+        const bool in = i < cells;
+        rv[k] = in ? r[i] : 0.0f;
+        avv[k] = in ? av[i] : 0.0f;
+        fluid[k] = in && !(flags[i] & kIdentity);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) s += fluid[k] ? double(rv[k]) : 0.0;
+    const double vr = block_sum_fixed(s, red);
+    const double vav = defl[0];
+    const double c = vav > 0.0 ? vr / vav : 0.0;
+    if (t == 0) defl[1] = c;
+    const float cf = float(c);
+    float v[kCoarsePerThread];
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        v[k] = fluid[k] ? fmaf(-cf, avv[k], rv[k]) : 0.0f;
+        ss += double(v[k]) * double(v[k]);
+    }
+    const double tot = block_sum_fixed(ss, red);
+    if (t == 0) sums[0] = tot;
+    const double sc = tot > 0.0 ? sqrt(tot / double(cells)) : 1.0;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        const int i = t + k * kCoarseThreads;
+        if (i >= cells) break;
+        rd[i] = v[k];
+        // channels 0-7: r / s (0 on identity rows), the identity mask, zeros
+        const uint32_t c0 = fluid[k] ? uint32_t(to_bf16(float(double(v[k]) / sc))) : 0u;
+        const uint32_t c1 = fluid[k] ? 0u : uint32_t(to_bf16(1.0f));
+        *reinterpret_cast<uint4*>(in8 + int64_t(i) * 8) = make_uint4(c0 | (c1 << 16), 0u, 0u, 0u);
+    }
+}
+
+// z = s out on fluid rows (0 on identity rows), then z += (c - Av.z / v.Av) on fluid rows
+__global__ __launch_bounds__(kCoarseThreads) void coarse_finish_kernel(const float* __restrict__ out, const uint8_t* __restrict__ flags,
+                                                                       const float* __restrict__ av, int cells, const double* __restrict__ sums,
+                                                                       const double* __restrict__ defl, float* __restrict__ z) {
+    __shared__ double red[kCoarseThreads / 64];
+    const int t = int(threadIdx.x);
+    const double tot = sums[0];
+    const float sc = float(tot > 0.0 ? sqrt(tot / double(cells)) : 1.0);
+    float zv[kCoarsePerThread];
+    bool fluid[kCoarsePerThread];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        const int i = t + k * kCoarseThreads;
+        const bool in = i < cells;
+        fluid[k] = in && !(flags[i] & kIdentity);
+        zv[k] = fluid[k] ? sc * out[i] : 0.0f;
+        s += in ? double(av[i]) * double(zv[k]) : 0.0;
+    }
+    const double az = block_sum_fixed(s, red);
+    const double vav = defl[0];
+    const float coef = vav > 0.0 ? float(defl[1] - az / vav) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        const int i = t + k * kCoarseThreads;
+        if (i >= cells) break;
+        z[i] = fluid[k] ? zv[k] + coef : 0.0f;
+    }
+}
+
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
                             float bc1, float bc2) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -458,6 +560,20 @@ void preconditioner_apply(SmokeNnPreconditioner& net, const float* r, const uint
     forward(net, 1, d, s);
     output_kernel<<<grid_for(cells), 256, 0, s>>>(net.z, flags, cells, net.sums, z);
     hip_check(hipGetLastError(), "preconditioner output");
+}
+
+void preconditioner_coarse_solve(SmokeNnPreconditioner& net, const float* r, const float* av, const uint8_t* flags, double* defl, const int dims[3],
+                                 float* rd, float* z, hipStream_t s) {
+    const int64_t cells = int64_t(dims[0]) * dims[1] * dims[2];
+    if (cells > kCoarseSolveMaxCells) throw HipError("coarse solve: volume too large for the single-block kernels");
+    ensure_workspace(net, 1, dims, false);
+    pack_all(net, s);
+    const Dims d = level_dims(dims);
+    coarse_prep_kernel<<<1, kCoarseThreads, 0, s>>>(r, av, flags, int(cells), defl, rd, net.sums, net.T[kIn]);
+    hip_check(hipGetLastError(), "coarse_prep_kernel");
+    forward(net, 1, d, s);
+    coarse_finish_kernel<<<1, kCoarseThreads, 0, s>>>(net.z, flags, av, int(cells), net.sums, defl, z);
+    hip_check(hipGetLastError(), "coarse_finish_kernel");
 }
 
 // Device address of `scratch` on the current device, looked up once per
