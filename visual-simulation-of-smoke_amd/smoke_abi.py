@@ -644,6 +644,16 @@ class LearnedPreconditioner:
         return loss.value
 
 
+CONV_KERNELS = {"auto": 0, "gather": 1, "staged": 2}
+
+
+def set_conv_kernels(choice: str):
+    """Process-wide convolution kernel choice (smoke_nn_set_conv_kernels)."""
+    L = load_library()
+    L.smoke_nn_set_conv_kernels.argtypes = [C.c_int32]
+    check(L.smoke_nn_set_conv_kernels(CONV_KERNELS[choice]), "set_conv_kernels")
+
+
 def inspect_checkpoint(path) -> tuple[int, CheckpointInfo]:
     """Host-only validation of a SMOKECK1 file: (result code, info)."""
     info = CheckpointInfo()

@@ -275,7 +275,10 @@ def main():
     ap.add_argument("--device-crops", action="store_true", help="keep the crops on the GPU (the 20k-crop, 30-epoch run)")
     ap.add_argument("--samples", type=int, default=20000, help="training crops with --device-crops")
     ap.add_argument("--validation", type=int, default=512, help="held-out crops with --device-crops")
+    ap.add_argument("--conv-kernels", choices=("auto", "gather", "staged"), default="auto",
+                    help="convolution kernels (smoke_nn_set_conv_kernels; for measurements)")
     args = ap.parse_args()
+    abi.set_conv_kernels(args.conv_kernels)
     if args.device_crops:
         import json
         hip = abi.Hip()
