@@ -39,8 +39,8 @@ CLASSES = {
 }
 # gather-form blocks (one output per thread), then the LDS-staged form
 ADVECT_BLOCKS = {(64, 4, 1, 1), (32, 8, 1, 1), (16, 16, 1, 1), (32, 4, 2, 1), (64, 2, 2, 1), (128, 2, 1, 1), (8, 8, 4, 1), (8, 8, 8, 1),
-                 (16, 4, 4, 1), (8, 16, 2, 1), (32, 8, 2, 1)}
-ADVECT_TABLE = [(16, 4, 4, 1)] + [(32, 8, 2, 1)] * 3
+                 (16, 4, 4, 1), (8, 16, 2, 1), (32, 8, 2, 2)}
+ADVECT_TABLE = [(16, 4, 4, 1)] + [(32, 8, 2, 2)] * 3
 
 
 def _context(cfg, st, tolerance=True):
@@ -112,7 +112,7 @@ def test_set_launches_applies_and_rejects_shapes():
     st = make_state(cfg, seed=9, solids=2)
     with _context(cfg, st, tolerance=False) as a, _context(cfg, st, tolerance=False) as plain:
         r = a.launches()
-        chosen = [(64, 4, 1, 1), (16, 16, 1, 1), (8, 8, 4, 1), (32, 8, 2, 1)]
+        chosen = [(64, 4, 1, 1), (16, 16, 1, 1), (8, 8, 4, 1), (32, 8, 2, 2)]
         for k, shape in enumerate(chosen):
             r.advect[k].threads_x, r.advect[k].threads_y, r.advect[k].threads_z, r.advect[k].cells_per_thread = shape
         r.pcg_curvature.threads_x, r.pcg_curvature.threads_y, r.pcg_curvature.threads_z, r.pcg_curvature.cells_per_thread = (64, 4, 4, 4)

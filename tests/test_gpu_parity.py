@@ -86,7 +86,7 @@ def test_stage_velocity_advection(name, cubic):
     assert max(errs.values()) < 1.5e-5, errs    # measured <= 1.25e-6 (mixed, linear)
 
 
-STAGED = (32, 8, 2, 1)      # the LDS-staged form: 32 x 8 x 2 threads, one output each (tiles of 32 x 8 x 2)
+STAGED = (32, 8, 2, 2)      # the LDS-staged form: 32 x 8 x 2 threads, two outputs each (tiles of 32 x 8 x 4)
 GATHER = (128, 2, 1, 1)     # a gather-form block: one output per thread, every tap from global memory
 
 

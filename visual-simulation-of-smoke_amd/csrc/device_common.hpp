@@ -319,12 +319,15 @@ struct Stencil {
     float t[3];
 };
 
+// WRAP = false compiles the periodic wrap out (for callers that know no axis
+// of `periodic` is set).
+template <bool WRAP = true>
 __device__ __forceinline__ Stencil locate(const Grid& g, const int periodic[3], int comp, float x, float y, float z) {
     float p[3] = {x, y, z};
     Stencil s;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (periodic[a]) {
+        if (WRAP && periodic[a]) {
             p[a] = wrap_position(p[a], float(g.n[a]));
         }
         // Beyond two cells outside the domain every tap is the same ghost

@@ -23,9 +23,10 @@ const dim3 kCellBlock(64, 4, 1);
 // the block shape changes only the speed), then the LDS-staged form
 // (kAdvectStaged).  Measured on MI355X at 256^3 (12-sphere plume state after
 // 30 steps, cubic advection): the gather form's 16 x 4 x 4 block takes
-// 0.50 ms for u (16 x 16 x 1: 0.55 ms), the staged form 0.62, 0.62 and
-// 0.96 ms for v, w and the scalars (the best gather blocks about twice
-// that).  The table takes the faster of the two.
+// 0.50 ms for u (the staged form 0.49), the staged form 0.51, 0.50 and
+// 0.89 ms for v, w and the scalars (the best gather blocks about twice
+// that).  The table takes the gather form for u and the staged form for
+// the others.
 const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
                               dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
@@ -641,7 +642,7 @@ void launch_boundaries(Context& c, float* const f[3]) {
 
 
 // ---- LDS-staged scalar advection (the same results as advect_scalars_kernel).
-// A block owns a 32 x 8 x 2 tile of cells.  It stages, with the ghost
+// A block owns a 32 x 8 x 4 tile of cells (two per thread).  It stages, with the ghost
 // rules applied once per staged index, the temperature and density sources
 // over the tile plus a halo of 2 (every cubic tap of a backtrace shorter than
 // one cell), the three face components over the boxes the two linear samples
@@ -652,14 +653,23 @@ void launch_boundaries(Context& c, float* const f[3]) {
 // elsewhere.  Any other cell (a trace longer than a cell, a periodic wrap, a
 // hit cell outside the box) is appended to a list that a second kernel
 // advects with the global path, so the common path carries none of the
-// ghost-rule code and keeps its register budget.
-constexpr int kSaX = 32, kSaY = 8, kSaZ = 2;
+// ghost-rule code and keeps its register budget.  Tiles whose boxes lie inside
+// the arrays (all but the domain's edge tiles) stage one box row per
+// wavefront pass (stage_rows); grids with no periodic axis run the kernels
+// compiled without the wrap code (staged_wrap).  Measured at 256^3 against
+// 32 x 8 x 2 tiles with one output per thread and against 32 x 8 x 4 tiles
+// of 1024 threads, two outputs per thread was the fastest (the scalars 0.87
+// against 0.95 ms, v 0.52 against 0.68 and 0.76 ms).
+constexpr int kSaX = 32, kSaY = 8, kSaZ = 4;
 // threads of a staged block: each owns an (x, y) column of the tile and the
 // z planes tz0, tz0 + kSaZStep, ... of it
 constexpr int kSaThreads = 512;
+constexpr int kSaWaves = kSaThreads / 64;
 constexpr int kSaPlane = kSaX * kSaY;
 constexpr int kSaZStep = kSaThreads / kSaPlane;
 static_assert(kSaThreads % kSaPlane == 0 && kSaZ % kSaZStep == 0, "staged tile and block");
+// every staged box is at most one wavefront wide (row-wise staging)
+static_assert(kSaX + 4 <= 64, "staged box width");
 
 struct LdsBox {
     int o[3];   // origin (global index of the first staged entry)
@@ -696,6 +706,38 @@ __device__ __forceinline__ void stage_inside(float* lds, const LdsBox& bx, const
     for (int q = tid; q < n; q += nthreads) {
         const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
         lds[q] = f[(k * d1 + j) * d0 + i];
+    }
+}
+
+// Staging of a box that lies inside an array (row pitch d0, plane pitch
+// d0 * d1, 32-bit indices), one box row per wavefront pass: lane x copies
+// entry x of the row.  The row's indices are the same for the whole
+// wavefront (scalar arithmetic), each row is one contiguous load, and the
+// per-entry cost is one address add and one LDS store.  `wave` must be
+// wavefront-uniform; rows are at most 64 entries wide.
+template <class T>
+__device__ __forceinline__ void stage_rows(T* lds, const LdsBox& bx, const T* __restrict__ f, int d0, int d1, int wave, int lane) {
+    const int rows = bx.d[1] * bx.d[2];
+    const bool active = lane < bx.d[0];
+    for (int r = wave; r < rows; r += kSaWaves) {
+        const int j = bx.o[1] + r % bx.d[1], k = bx.o[2] + r / bx.d[1];
+        const int src = (k * d1 + j) * d0 + bx.o[0];
+        if (active) lds[r * bx.d[0] + lane] = f[src + lane];
+    }
+}
+
+// two fields of one shape over the same box
+__device__ __forceinline__ void stage_rows2(float* la, float* lb, const LdsBox& bx, const float* __restrict__ fa, const float* __restrict__ fb, int d0,
+                                            int d1, int wave, int lane) {
+    const int rows = bx.d[1] * bx.d[2];
+    const bool active = lane < bx.d[0];
+    for (int r = wave; r < rows; r += kSaWaves) {
+        const int j = bx.o[1] + r % bx.d[1], k = bx.o[2] + r / bx.d[1];
+        const int src = (k * d1 + j) * d0 + bx.o[0];
+        if (active) {
+            la[r * bx.d[0] + lane] = fa[src + lane];
+            lb[r * bx.d[0] + lane] = fb[src + lane];
+        }
     }
 }
 
@@ -739,11 +781,12 @@ __device__ __forceinline__ void staged_cubic2(const float* la, const float* lb, 
 
 // hits_solid (B:785-794) against the staged occupancy: 1 hit, 0 free,
 // -1 when the hit cell lies outside the staged box (the caller falls back)
+template <bool WRAP>
 __device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, const LdsBox& bx, float qx, float qy, float qz) {
     float q[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]));
+        if (WRAP && g.flow_periodic[a]) q[a] = wrap_position(q[a], float(g.n[a]));
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -767,7 +810,8 @@ __device__ __forceinline__ int hit_staged(const Grid& g, const uint8_t* locc, co
 // components and the staged hit test: false (nothing written) when a sample
 // stencil or a hit-test cell lies outside its staged box.  The start samples
 // read the taps next to the start (start_velocity), which every box holds.
-template <int C>
+// WRAP = false: no periodic flow axis (the wrap code compiled out).
+template <int C, bool WRAP>
 __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, const LdsBox& bu, const float* lv, const LdsBox& bv, const float* lw,
                                              const LdsBox& bw, const uint8_t* locc, const LdsBox& bo, float s, int i, int j, int k, float& px,
                                              float& py, float& pz) {
@@ -782,22 +826,22 @@ __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, con
     const float sz = C == 2 ? float(k) : float(k) + 0.5f;
     const float hs = -0.5f * s;
     const float mx = fmaf(hs, v0.x, sx), my = fmaf(hs, v0.y, sy), mz = fmaf(hs, v0.z, sz);
-    const Stencil m0 = locate(g, g.flow_periodic, 0, mx, my, mz);
-    const Stencil m1 = locate(g, g.flow_periodic, 1, mx, my, mz);
-    const Stencil m2 = locate(g, g.flow_periodic, 2, mx, my, mz);
+    const Stencil m0 = locate<WRAP>(g, g.flow_periodic, 0, mx, my, mz);
+    const Stencil m1 = locate<WRAP>(g, g.flow_periodic, 1, mx, my, mz);
+    const Stencil m2 = locate<WRAP>(g, g.flow_periodic, 2, mx, my, mz);
     if (!(in_box<0, 1>(bu, m0) && in_box<0, 1>(bv, m1) && in_box<0, 1>(bw, m2))) return false;
     const float v1x = staged_linear(lu, bu, m0), v1y = staged_linear(lv, bv, m1), v1z = staged_linear(lw, bw, m2);
     const float tx = fmaf(-s, v1x, sx), ty = fmaf(-s, v1y, sy), tz = fmaf(-s, v1z, sz);
     px = tx;
     py = ty;
     pz = tz;
-    const int hit = hit_staged(g, locc, bo, tx, ty, tz);
+    const int hit = hit_staged<WRAP>(g, locc, bo, tx, ty, tz);
     if (hit < 0) return false;
     if (hit > 0) {
         float lo = 0.0f, hi = 1.0f;
         for (int it = 0; it < 10; ++it) {
             const float mt = 0.5f * (lo + hi);
-            const int hm = hit_staged(g, locc, bo, fmaf(tx - sx, mt, sx), fmaf(ty - sy, mt, sy), fmaf(tz - sz, mt, sz));
+            const int hm = hit_staged<WRAP>(g, locc, bo, fmaf(tx - sx, mt, sx), fmaf(ty - sy, mt, sy), fmaf(tz - sz, mt, sz));
             if (hm < 0) return false;
             if (hm > 0) hi = mt;
             else lo = mt;
@@ -827,6 +871,8 @@ __device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* _
     rho[i] = rv;
 }
 
+// WRAP = false: no periodic flow or scalar axis (the wrap code compiled out).
+template <bool WRAP>
 __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                            const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                                            const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
@@ -845,7 +891,9 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     unsigned id;
     if (!xcd_tile(vb, tiles, tile, id)) return;
     const int tid = threadIdx.x;
-    const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int X0 = __builtin_amdgcn_readfirstlane(int(tile.x) * kSaX), Y0 = __builtin_amdgcn_readfirstlane(int(tile.y) * kSaY),
+              Z0 = __builtin_amdgcn_readfirstlane(int(tile.z) * kSaZ);
     const LdsBox bs{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
     const LdsBox bu{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 2, kSaZ + 2}};
     const LdsBox bv{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 3, kSaZ + 2}};
@@ -858,19 +906,11 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     const bool inside = index32(g) && box_inside(bs, n0 - 1, n1 - 1, n2 - 1) && box_inside(bu, n0, n1 - 1, n2 - 1) &&
                         box_inside(bv, n0 - 1, n1, n2 - 1) && box_inside(bw, n0 - 1, n1 - 1, n2);
     if (inside) {
-        for (int q = tid; q < kS; q += kSaThreads) {
-            const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
-            const int c = (k * n1 + j) * n0 + i;
-            sT[q] = Ttmp[c];
-            sR[q] = rtmp[c];
-        }
-        stage_inside(su, bu, u, n0 + 1, n1, tid, kSaThreads);
-        stage_inside(sv, bv, v, n0, n1 + 1, tid, kSaThreads);
-        stage_inside(sw, bw, w, n0, n1, tid, kSaThreads);
-        for (int q = tid; q < kO; q += kSaThreads) {   // bo lies inside bs
-            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
-            so[q] = occ[(k * n1 + j) * n0 + i];
-        }
+        stage_rows2(sT, sR, bs, Ttmp, rtmp, n0, n1, wave, lane);
+        stage_rows(su, bu, u, n0 + 1, n1, wave, lane);
+        stage_rows(sv, bv, v, n0, n1 + 1, wave, lane);
+        stage_rows(sw, bw, w, n0, n1, wave, lane);
+        stage_rows(so, bo, occ, n0, n1, wave, lane);   // bo lies inside bs
     } else {
         for (int q = tid; q < kS; q += kSaThreads) {
             const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
@@ -910,10 +950,10 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
             continue;
         }
         float px, py, pz;
-        bool ok = same_periodic && staged_trace<-1>(g, su, bu, sv, bv, sw, bw, so, bo, cps, x, y, z, px, py, pz);
+        bool ok = same_periodic && staged_trace<-1, WRAP>(g, su, bu, sv, bv, sw, bw, so, bo, cps, x, y, z, px, py, pz);
         Stencil st{};
         if (ok) {
-            st = locate(g, g.temperature.periodic, -1, px, py, pz);
+            st = locate<WRAP>(g, g.temperature.periodic, -1, px, py, pz);
             ok = cubic ? in_box<-1, 2>(bs, st) : in_box<0, 1>(bs, st);
         }
         if (!ok) {
@@ -932,17 +972,19 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     }
 }
 
+template <bool WRAP>
 __global__ __launch_bounds__(kSaThreads) void advect_scalars_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                            const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                                            const float* __restrict__ Ttmp, const float* __restrict__ rtmp,
                                                                            float* __restrict__ T, float* __restrict__ rho,
                                                                            const float* __restrict__ solid_T, const float* __restrict__ dt_ptr,
                                                                            int* __restrict__ fallback, unsigned* __restrict__ fallback_count,
-                                                                           uint3 tiles) { advect_scalars_staged_body(hw_block(), g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt_ptr, fallback, fallback_count, tiles); }
+                                                                           uint3 tiles) { advect_scalars_staged_body<WRAP>(hw_block(), g, u, v, w, occ, Ttmp, rtmp, T, rho, solid_T, dt_ptr, fallback, fallback_count, tiles); }
 
+template <bool WRAP>
 struct advect_scalars_staged_k {
     template <class... A>
-    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_scalars_staged_body(vb, a...); }
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_scalars_staged_body<WRAP>(vb, a...); }
 };
 
 // the cells the staged kernel could not advect from LDS, by the global path
@@ -1002,7 +1044,7 @@ __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx
 }
 
 // ---- LDS-staged velocity advection (the same results as advect_face_kernel).
-// A block owns a 32 x 8 x 2 tile of the faces of component C (face indices
+// A block owns a 32 x 8 x 4 tile of the faces of component C (face indices
 // below n on every axis: face n is always a boundary face).  For a backtrace
 // shorter than one cell it stages component C over the tile plus a halo of 2
 // (the cubic taps of the foot, the linear taps of C's own velocity samples),
@@ -1010,7 +1052,7 @@ __device__ __forceinline__ float staged_cubic(const float* lds, const LdsBox& bx
 // at the start and the midpoint) and the occupancy plus a halo of 1 (the hit
 // test and the solid-face test).  Faces whose trace or foot stencil leaves the
 // staged boxes go to the fallback list.
-template <int C>
+template <int C, bool WRAP>
 __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                         const float* __restrict__ w, float* __restrict__ dst,
                                                                         const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
@@ -1027,7 +1069,9 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     unsigned id;
     if (!xcd_tile(vb, tiles, tile, id)) return;
     const int tid = threadIdx.x;
-    const int X0 = int(tile.x) * kSaX, Y0 = int(tile.y) * kSaY, Z0 = int(tile.z) * kSaZ;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int X0 = __builtin_amdgcn_readfirstlane(int(tile.x) * kSaX), Y0 = __builtin_amdgcn_readfirstlane(int(tile.y) * kSaY),
+              Z0 = __builtin_amdgcn_readfirstlane(int(tile.z) * kSaZ);
     const LdsBox bf{{X0 - 2, Y0 - 2, Z0 - 2}, {kSaX + 4, kSaY + 4, kSaZ + 4}};
     const LdsBox bg{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 3, kSaY + 3, kSaZ + 3}};
     const LdsBox bo{{X0 - 1, Y0 - 1, Z0 - 1}, {kSaX + 2, kSaY + 2, kSaZ + 2}};
@@ -1042,13 +1086,10 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     };
     const bool inside = index32(g) && inside_for(C, bf) && inside_for(D0, bg) && inside_for(D1, bg) && box_inside(bo, n0 - 1, n1 - 1, n2 - 1);
     if (inside) {
-        stage_inside(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), tid, kSaThreads);
-        stage_inside(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), tid, kSaThreads);
-        stage_inside(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), tid, kSaThreads);
-        for (int q = tid; q < kO; q += kSaThreads) {
-            const int i = bo.o[0] + q % bo.d[0], j = bo.o[1] + (q / bo.d[0]) % bo.d[1], k = bo.o[2] + q / (bo.d[0] * bo.d[1]);
-            so[q] = occ[(k * n1 + j) * n0 + i];
-        }
+        stage_rows(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), wave, lane);
+        stage_rows(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), wave, lane);
+        stage_rows(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), wave, lane);
+        stage_rows(so, bo, occ, n0, n1, wave, lane);
     } else {
         auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
             const int n = box_size(bx);
@@ -1100,10 +1141,10 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
             }
         }
         float px, py, pz;
-        bool ok = staged_trace<C>(g, lu, bu, lv, bv, lw, bw, so, bo, cps, i, j, k, px, py, pz);
+        bool ok = staged_trace<C, WRAP>(g, lu, bu, lv, bv, lw, bw, so, bo, cps, i, j, k, px, py, pz);
         Stencil st{};
         if (ok) {
-            st = locate(g, g.flow_periodic, C, px, py, pz);
+            st = locate<WRAP>(g, g.flow_periodic, C, px, py, pz);
             ok = cubic ? in_box<-1, 2>(bf, st) : in_box<0, 1>(bf, st);
         }
         if (!ok) {
@@ -1114,17 +1155,17 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     }
 }
 
-template <int C>
+template <int C, bool WRAP>
 __global__ __launch_bounds__(kSaThreads) void advect_face_staged_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                         const float* __restrict__ w, float* __restrict__ dst,
                                                                         const uint8_t* __restrict__ occ, const float* __restrict__ solid_vel,
                                                                         const float* __restrict__ dt_ptr, int* __restrict__ fallback,
-                                                                        unsigned* __restrict__ fallback_count, uint3 tiles) { advect_face_staged_body<C>(hw_block(), g, u, v, w, dst, occ, solid_vel, dt_ptr, fallback, fallback_count, tiles); }
+                                                                        unsigned* __restrict__ fallback_count, uint3 tiles) { advect_face_staged_body<C, WRAP>(hw_block(), g, u, v, w, dst, occ, solid_vel, dt_ptr, fallback, fallback_count, tiles); }
 
-template <int C>
+template <int C, bool WRAP>
 struct advect_face_staged_k {
     template <class... A>
-    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_staged_body<C>(vb, a...); }
+    __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_staged_body<C, WRAP>(vb, a...); }
 };
 
 // the faces the staged kernel could not advect from LDS, by the global path
@@ -1155,6 +1196,14 @@ struct advect_face_fallback_k {
     __device__ __forceinline__ static void run(const VBlock& vb, A... a) { advect_face_fallback_body<C>(vb, a...); }
 };
 
+// The staged kernels come in two forms: with the periodic wrap, and (for grids
+// with no periodic flow or scalar axis) without it.
+bool staged_wrap(const Grid& g) {
+    for (int a = 0; a < 3; ++a)
+        if (g.flow_periodic[a] || g.temperature.periodic[a] || g.density.periodic[a]) return true;
+    return false;
+}
+
 template <int C>
 void launch_advect(Context& c, int block_index) {
     const Grid& g = c.grid;
@@ -1162,10 +1211,10 @@ void launch_advect(Context& c, int block_index) {
         // tiles over face indices below n (at most `cells` faces reach the list)
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_face_staged_kernel<C><<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C],
-                                                                                      c.occupancy, c.solid_velocity[C], c.dt_device,
-                                                                                      c.advect_fallback, c.advect_fallback_count,
-                                                                                      make_uint3(tiles.x, tiles.y, tiles.z));
+        auto* kernel = staged_wrap(g) ? advect_face_staged_kernel<C, true> : advect_face_staged_kernel<C, false>;
+        kernel<<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C],
+                                                               c.dt_device, c.advect_fallback, c.advect_fallback_count,
+                                                               make_uint3(tiles.x, tiles.y, tiles.z));
         advect_face_fallback_kernel<C><<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.dt_device,
                                                                    c.advect_fallback, c.advect_fallback_count);
         hip_check(hipGetLastError(), "staged velocity advection");
@@ -1183,10 +1232,10 @@ void launch_advect_scalars(Context& c, int block_index, float* T, float* rho) {
     if (block_index == kAdvectStaged) {
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         zero_count_kernel<<<1, 1, 0, c.stream>>>(c.advect_fallback_count);
-        advect_scalars_staged_kernel<<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy,
-                                                                                      c.temperature_tmp, c.density_tmp, T, rho, c.solid_temperature,
-                                                                                      c.dt_device, c.advect_fallback, c.advect_fallback_count,
-                                                                                      make_uint3(tiles.x, tiles.y, tiles.z));
+        auto* kernel = staged_wrap(g) ? advect_scalars_staged_kernel<true> : advect_scalars_staged_kernel<false>;
+        kernel<<<xcd_blocks(tiles), kSaThreads, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, T,
+                                                               rho, c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count,
+                                                               make_uint3(tiles.x, tiles.y, tiles.z));
         advect_scalars_fallback_kernel<<<1024, 256, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp,
                                                                  c.density_tmp, T, rho, c.solid_temperature, c.dt_device, c.advect_fallback,
                                                                  c.advect_fallback_count);
@@ -1334,7 +1383,10 @@ unsigned fallback_blocks(const Context& c) { return unsigned(std::min<int64_t>(1
 template <int C>
 void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
     LaunchBuilder<zero_count_k, 64, unsigned*> Z;
-    LaunchBuilder<advect_face_staged_k<C>, kSaThreads, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
+    LaunchBuilder<advect_face_staged_k<C, true>, kSaThreads, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
+                  const float*, int*, unsigned*, uint3>
+        SW;
+    LaunchBuilder<advect_face_staged_k<C, false>, kSaThreads, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*,
                   const float*, int*, unsigned*, uint3>
         S;
     LaunchBuilder<advect_face_fallback_k<C>, 256, Grid, const float*, const float*, const float*, float*, const uint8_t*, const float*, const int*,
@@ -1346,14 +1398,19 @@ void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vecto
         const Grid& g = c.grid;
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         Z.add(p, dim3(1), c.advect_fallback_count);
-        S.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C], c.dt_device,
-              c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        auto add = [&](auto& L) {
+            L.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.solid_velocity[C], c.dt_device,
+                  c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        };
+        if (staged_wrap(g)) add(SW);
+        else add(S);
         F.add(p, dim3(fallback_blocks(c)), g, c.face[0], c.face[1], c.face[2], c.face_star[C], c.occupancy, c.dt_device, c.advect_fallback,
               c.advect_fallback_count);
     }
-    if (!S.empty()) {
+    if (!Z.empty()) {
         out.push_back(Z.finish(A, dim3(1)));
-        out.push_back(S.finish(A, dim3(kSaThreads)));
+        if (!SW.empty()) out.push_back(SW.finish(A, dim3(kSaThreads)));
+        if (!S.empty()) out.push_back(S.finish(A, dim3(kSaThreads)));
         out.push_back(F.finish(A, dim3(256)));
     }
     for (int b = 0; b < kAdvectBlockCount; ++b) {
@@ -1373,8 +1430,11 @@ void batch_advect_face(const std::vector<Context*>& P, TableArena& A, std::vecto
 
 void batch_advect_scalars(const std::vector<Context*>& P, TableArena& A, std::vector<BatchedLaunch>& out) {
     LaunchBuilder<zero_count_k, 64, unsigned*> Z;
-    LaunchBuilder<advect_scalars_staged_k, kSaThreads, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*,
-                  float*, float*, const float*, const float*, int*, unsigned*, uint3>
+    LaunchBuilder<advect_scalars_staged_k<true>, kSaThreads, Grid, const float*, const float*, const float*, const uint8_t*, const float*,
+                  const float*, float*, float*, const float*, const float*, int*, unsigned*, uint3>
+        SW;
+    LaunchBuilder<advect_scalars_staged_k<false>, kSaThreads, Grid, const float*, const float*, const float*, const uint8_t*, const float*,
+                  const float*, float*, float*, const float*, const float*, int*, unsigned*, uint3>
         S;
     LaunchBuilder<advect_scalars_fallback_k, 256, Grid, const float*, const float*, const float*, const uint8_t*, const float*, const float*, float*,
                   float*, const float*, const float*, const int*, const unsigned*>
@@ -1385,14 +1445,19 @@ void batch_advect_scalars(const std::vector<Context*>& P, TableArena& A, std::ve
         const Grid& g = c.grid;
         const dim3 tiles(unsigned((g.n[0] + kSaX - 1) / kSaX), unsigned((g.n[1] + kSaY - 1) / kSaY), unsigned((g.n[2] + kSaZ - 1) / kSaZ));
         Z.add(p, dim3(1), c.advect_fallback_count);
-        S.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, c.temperature, c.density,
-              c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        auto add = [&](auto& L) {
+            L.add(p, dim3(xcd_blocks(tiles)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, c.temperature,
+                  c.density, c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count, make_uint3(tiles.x, tiles.y, tiles.z));
+        };
+        if (staged_wrap(g)) add(SW);
+        else add(S);
         F.add(p, dim3(fallback_blocks(c)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.temperature_tmp, c.density_tmp, c.temperature, c.density,
               c.solid_temperature, c.dt_device, c.advect_fallback, c.advect_fallback_count);
     }
-    if (!S.empty()) {
+    if (!Z.empty()) {
         out.push_back(Z.finish(A, dim3(1)));
-        out.push_back(S.finish(A, dim3(kSaThreads)));
+        if (!SW.empty()) out.push_back(SW.finish(A, dim3(kSaThreads)));
+        if (!S.empty()) out.push_back(S.finish(A, dim3(kSaThreads)));
         out.push_back(F.finish(A, dim3(256)));
     }
     for (int b = 0; b < kAdvectBlockCount; ++b) {
