@@ -23,10 +23,8 @@ const dim3 kCellBlock(64, 4, 1);
 // the block shape changes only the speed), then the LDS-staged form
 // (kAdvectStaged).  Measured on MI355X at 256^3 (12-sphere plume state after
 // 30 steps, cubic advection): the gather form's 16 x 4 x 4 block takes
-// 0.50 ms for u (the staged form 0.49), the staged form 0.51, 0.50 and
-// 0.89 ms for v, w and the scalars (the best gather blocks about twice
-// that).  The table takes the gather form for u and the staged form for
-// the others.
+// 0.50 ms for u, the staged form 0.45, 0.45 and 0.73 ms for v, w and the
+// scalars (the best gather blocks about twice that).
 const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
                               dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
@@ -654,12 +652,18 @@ void launch_boundaries(Context& c, float* const f[3]) {
 // hit cell outside the box) is appended to a list that a second kernel
 // advects with the global path, so the common path carries none of the
 // ghost-rule code and keeps its register budget.  Tiles whose boxes lie inside
-// the arrays (all but the domain's edge tiles) stage one box row per
-// wavefront pass (stage_rows); grids with no periodic axis run the kernels
-// compiled without the wrap code (staged_wrap).  Measured at 256^3 against
-// 32 x 8 x 2 tiles with one output per thread and against 32 x 8 x 4 tiles
-// of 1024 threads, two outputs per thread was the fastest (the scalars 0.87
-// against 0.95 ms, v 0.52 against 0.68 and 0.76 ms).
+// the arrays (all but the domain's edge tiles) stage their FP32 boxes by LDS
+// DMA, one box row per wavefront pass (dma_rows); grids with no periodic axis
+// run the kernels compiled without the wrap code (staged_wrap).  Measured at
+// 256^3 against 32 x 8 x 2 tiles with one output per thread and against
+// 32 x 8 x 4 tiles of 1024 threads, two outputs per thread was the fastest
+// (with register staging: the scalars 0.87 against 0.95 ms, v 0.52 against
+// 0.68 and 0.76 ms).  The DMA staging took the scalars from 0.87 to 0.73 ms
+// and v and w from 0.52 to 0.45: staged through registers, each row's load
+// waited for its memory round trip before the next was issued.  Edge tiles
+// keep the per-entry ghost-rule staging: the same clamped-DMA-then-patch
+// scheme for them raised the kernels' scalar register spills tenfold and
+// made every tile slower.
 constexpr int kSaX = 32, kSaY = 8, kSaZ = 4;
 // threads of a staged block: each owns an (x, y) column of the tile and the
 // z planes tz0, tz0 + kSaZStep, ... of it
@@ -698,48 +702,50 @@ __device__ __forceinline__ bool box_inside(const LdsBox& bx, int t0, int t1, int
            bx.o[2] + bx.d[2] - 1 <= t2;
 }
 
-// Staging of a box that lies inside an array of row pitch d0 and plane pitch
-// d0 * d1 (32-bit indices: the caller checks that the array has fewer than
-// 2^31 entries).
-__device__ __forceinline__ void stage_inside(float* lds, const LdsBox& bx, const float* __restrict__ f, int d0, int d1, int tid, int nthreads) {
-    const int n = box_size(bx);
-    for (int q = tid; q < n; q += nthreads) {
-        const int i = bx.o[0] + q % bx.d[0], j = bx.o[1] + (q / bx.d[0]) % bx.d[1], k = bx.o[2] + q / (bx.d[0] * bx.d[1]);
-        lds[q] = f[(k * d1 + j) * d0 + i];
+// Staging of a box of D0 x D1 x D2 entries, one box row (j, k) = (r % D1,
+// r / D1) per wavefront pass: wavefront w takes rows w, w + kSaWaves, ...,
+// lane x entry x of the row, so a row's indices are wavefront-uniform (scalar
+// arithmetic) and each row is one contiguous load.  The FP32 rows go by LDS
+// DMA (global_load_lds_dword: lane x's value is written to lds[r * D0 + x]
+// without passing through registers), so a wavefront's row loads are all in
+// flight together; the byte rows (occupancy) go through registers, all loaded
+// before any is stored.  `wave` must be wavefront-uniform.
+//
+// dma_rows: a box inside an array of row pitch d0 and plane pitch d0 * d1,
+// 32-bit indices (the caller checks that the array has fewer than 2^31
+// entries).
+template <int D0, int D1, int D2>
+__device__ __forceinline__ void dma_rows(float* lds, const float* __restrict__ f, const LdsBox& bx, int d0, int d1, int wave, int lane) {
+    static_assert(D0 <= 64, "one row per wavefront pass");
+    for (int r = wave; r < D1 * D2; r += kSaWaves) {
+        const int j = r % D1, k = r / D1;
+        if (lane < D0) __builtin_amdgcn_global_load_lds(f + ((bx.o[2] + k) * d1 + bx.o[1] + j) * d0 + bx.o[0] + lane, lds + r * D0, 4, 0, 0);
     }
 }
 
-// Staging of a box that lies inside an array (row pitch d0, plane pitch
-// d0 * d1, 32-bit indices), one box row per wavefront pass: lane x copies
-// entry x of the row.  The row's indices are the same for the whole
-// wavefront (scalar arithmetic), each row is one contiguous load, and the
-// per-entry cost is one address add and one LDS store.  `wave` must be
-// wavefront-uniform; rows are at most 64 entries wide.
-template <class T>
-__device__ __forceinline__ void stage_rows(T* lds, const LdsBox& bx, const T* __restrict__ f, int d0, int d1, int wave, int lane) {
-    const int rows = bx.d[1] * bx.d[2];
-    const bool active = lane < bx.d[0];
-    for (int r = wave; r < rows; r += kSaWaves) {
-        const int j = bx.o[1] + r % bx.d[1], k = bx.o[2] + r / bx.d[1];
-        const int src = (k * d1 + j) * d0 + bx.o[0];
-        if (active) lds[r * bx.d[0] + lane] = f[src + lane];
-    }
-}
-
-// two fields of one shape over the same box
-__device__ __forceinline__ void stage_rows2(float* la, float* lb, const LdsBox& bx, const float* __restrict__ fa, const float* __restrict__ fb, int d0,
-                                            int d1, int wave, int lane) {
-    const int rows = bx.d[1] * bx.d[2];
-    const bool active = lane < bx.d[0];
-    for (int r = wave; r < rows; r += kSaWaves) {
-        const int j = bx.o[1] + r % bx.d[1], k = bx.o[2] + r / bx.d[1];
-        const int src = (k * d1 + j) * d0 + bx.o[0];
-        if (active) {
-            la[r * bx.d[0] + lane] = fa[src + lane];
-            lb[r * bx.d[0] + lane] = fb[src + lane];
+template <int D0, int D1, int D2>
+struct ByteRows {
+    static_assert(D0 <= 64, "one row per wavefront pass");
+    static constexpr int kRows = D1 * D2;
+    static constexpr int kPasses = (kRows + kSaWaves - 1) / kSaWaves;
+    uint8_t v[kPasses];
+    // value(i, j, k) at the box's entry (i, j, k) (global indices)
+    template <class V>
+    __device__ __forceinline__ void load(const LdsBox& bx, int wave, int lane, V value) {
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) {
+            const int r = wave + p * kSaWaves;
+            if (r < kRows && lane < D0) v[p] = value(bx.o[0] + lane, bx.o[1] + r % D1, bx.o[2] + r / D1);
         }
     }
-}
+    __device__ __forceinline__ void store(uint8_t* lds, int wave, int lane) const {
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) {
+            const int r = wave + p * kSaWaves;
+            if (r < kRows && lane < D0) lds[r * D0 + lane] = v[p];
+        }
+    }
+};
 
 // sample_linear on staged values; the stencil must lie in the box
 __device__ __forceinline__ float staged_linear(const float* lds, const LdsBox& bx, const Stencil& s) {
@@ -906,12 +912,16 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     const bool inside = index32(g) && box_inside(bs, n0 - 1, n1 - 1, n2 - 1) && box_inside(bu, n0, n1 - 1, n2 - 1) &&
                         box_inside(bv, n0 - 1, n1, n2 - 1) && box_inside(bw, n0 - 1, n1 - 1, n2);
     if (inside) {
-        stage_rows2(sT, sR, bs, Ttmp, rtmp, n0, n1, wave, lane);
-        stage_rows(su, bu, u, n0 + 1, n1, wave, lane);
-        stage_rows(sv, bv, v, n0, n1 + 1, wave, lane);
-        stage_rows(sw, bw, w, n0, n1, wave, lane);
-        stage_rows(so, bo, occ, n0, n1, wave, lane);   // bo lies inside bs
+        ByteRows<kSaX + 2, kSaY + 2, kSaZ + 2> ro;
+        ro.load(bo, wave, lane, [&](int i, int j, int k) { return occ[(k * n1 + j) * n0 + i]; });   // bo lies inside bs
+        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sT, Ttmp, bs, n0, n1, wave, lane);
+        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sR, rtmp, bs, n0, n1, wave, lane);
+        dma_rows<kSaX + 3, kSaY + 2, kSaZ + 2>(su, u, bu, n0 + 1, n1, wave, lane);
+        dma_rows<kSaX + 2, kSaY + 3, kSaZ + 2>(sv, v, bv, n0, n1 + 1, wave, lane);
+        dma_rows<kSaX + 2, kSaY + 2, kSaZ + 3>(sw, w, bw, n0, n1, wave, lane);
+        ro.store(so, wave, lane);
     } else {
+        // edge tiles (and grids too large for 32-bit indices): every entry with the ghost rules
         for (int q = tid; q < kS; q += kSaThreads) {
             const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
             sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
@@ -1086,11 +1096,14 @@ __device__ __forceinline__ void advect_face_staged_body(const VBlock& vb, Grid g
     };
     const bool inside = index32(g) && inside_for(C, bf) && inside_for(D0, bg) && inside_for(D1, bg) && box_inside(bo, n0 - 1, n1 - 1, n2 - 1);
     if (inside) {
-        stage_rows(sf, bf, fc, n0 + (C == 0), n1 + (C == 1), wave, lane);
-        stage_rows(sg0, bg, f0, n0 + (D0 == 0), n1 + (D0 == 1), wave, lane);
-        stage_rows(sg1, bg, f1, n0 + (D1 == 0), n1 + (D1 == 1), wave, lane);
-        stage_rows(so, bo, occ, n0, n1, wave, lane);
+        ByteRows<kSaX + 2, kSaY + 2, kSaZ + 2> ro;
+        ro.load(bo, wave, lane, [&](int i, int j, int k) { return occ[(k * n1 + j) * n0 + i]; });
+        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sf, fc, bf, n0 + (C == 0), n1 + (C == 1), wave, lane);
+        dma_rows<kSaX + 3, kSaY + 3, kSaZ + 3>(sg0, f0, bg, n0 + (D0 == 0), n1 + (D0 == 1), wave, lane);
+        dma_rows<kSaX + 3, kSaY + 3, kSaZ + 3>(sg1, f1, bg, n0 + (D1 == 0), n1 + (D1 == 1), wave, lane);
+        ro.store(so, wave, lane);
     } else {
+        // edge tiles (and grids too large for 32-bit indices): every entry with the ghost rules
         auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
             const int n = box_size(bx);
             for (int q = tid; q < n; q += kSaThreads) {
