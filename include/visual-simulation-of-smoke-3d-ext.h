@@ -222,9 +222,9 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_measure_copy_ban
 
    Advection has two forms with bitwise-equal results.  The gather form runs
    one thread per output and reads every tap from global memory (cells_per_thread
-   1, any candidate block; the table's choice for u).  The LDS-staged form
-   (32 x 8 x 2 threads, cells_per_thread 2: tiles of 32 x 8 x 4 outputs, the
-   table's choice for v, w and the scalars) stages
+   1, any candidate block).  The LDS-staged form (32 x 8 x 2 threads,
+   cells_per_thread 2: tiles of 32 x 8 x 4 outputs, the table's choice for all
+   four kernels) stages
    each tile's sources, velocities and occupancy with their ghost values once,
    advects every output whose backtrace (shorter than a cell, no periodic
    wrap) stays inside the staged boxes from LDS, and advects the rest in a

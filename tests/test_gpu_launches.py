@@ -40,7 +40,7 @@ CLASSES = {
 # gather-form blocks (one output per thread), then the LDS-staged form
 ADVECT_BLOCKS = {(64, 4, 1, 1), (32, 8, 1, 1), (16, 16, 1, 1), (32, 4, 2, 1), (64, 2, 2, 1), (128, 2, 1, 1), (8, 8, 4, 1), (8, 8, 8, 1),
                  (16, 4, 4, 1), (8, 16, 2, 1), (32, 8, 2, 2)}
-ADVECT_TABLE = [(16, 4, 4, 1)] + [(32, 8, 2, 2)] * 3
+ADVECT_TABLE = [(32, 8, 2, 2)] * 4
 
 
 def _context(cfg, st, tolerance=True):

@@ -22,15 +22,15 @@ const dim3 kCellBlock(64, 4, 1);
 // (threads along x, y, z; every thread computes one output independently, so
 // the block shape changes only the speed), then the LDS-staged form
 // (kAdvectStaged).  Measured on MI355X at 256^3 (12-sphere plume state after
-// 30 steps, cubic advection): the gather form's 16 x 4 x 4 block takes
-// 0.50 ms for u, the staged form 0.45, 0.45 and 0.73 ms for v, w and the
-// scalars (the best gather blocks about twice that).
+// 30 steps, cubic advection): the staged form takes 0.44, 0.45, 0.45 and
+// 0.75 ms for u, v, w and the scalars; the best gather block 0.50 ms for u
+// (16 x 4 x 4) and about twice the staged time for the others.
 const dim3 kAdvectBlocks[] = {dim3(64, 4, 1), dim3(32, 8, 1), dim3(16, 16, 1), dim3(32, 4, 2), dim3(64, 2, 2), dim3(128, 2, 1),
                               dim3(8, 8, 4), dim3(8, 8, 8), dim3(16, 4, 4), dim3(8, 16, 2)};
 constexpr int kAdvectBlockCount = int(sizeof(kAdvectBlocks) / sizeof(kAdvectBlocks[0]));
 constexpr int kAdvectStaged = kAdvectBlockCount;
 constexpr int kAdvectCandidates = kAdvectBlockCount + 1;
-constexpr int kAdvectTable[4] = {8, kAdvectStaged, kAdvectStaged, kAdvectStaged};   // u: 16 x 4 x 4
+constexpr int kAdvectTable[4] = {kAdvectStaged, kAdvectStaged, kAdvectStaged, kAdvectStaged};
 
 __global__ void fill_kernel(float* __restrict__ p, float value, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -690,8 +690,10 @@ __device__ __forceinline__ bool in_box(const LdsBox& bx, const Stencil& s) {
     return ok;
 }
 
+// Offset of entry (x, y, z) in a box it lies in: non-negative offsets and
+// extents below 2^24, so the products are full-rate 24-bit multiplies.
 __device__ __forceinline__ int box_base(const LdsBox& bx, int x, int y, int z) {
-    return ((z - bx.o[2]) * bx.d[1] + (y - bx.o[1])) * bx.d[0] + (x - bx.o[0]);
+    return int(__umul24(__umul24(unsigned(z - bx.o[2]), unsigned(bx.d[1])) + unsigned(y - bx.o[1]), unsigned(bx.d[0]))) + (x - bx.o[0]);
 }
 
 // True when every index of the box lies in [0, top[a]] on each axis a (n - 1
@@ -845,6 +847,7 @@ __device__ __forceinline__ bool staged_trace(const Grid& g, const float* lu, con
     if (hit < 0) return false;
     if (hit > 0) {
         float lo = 0.0f, hi = 1.0f;
+#pragma nounroll
         for (int it = 0; it < 10; ++it) {
             const float mt = 0.5f * (lo + hi);
             const int hm = hit_staged<WRAP>(g, locc, bo, fmaf(tx - sx, mt, sx), fmaf(ty - sy, mt, sy), fmaf(tz - sz, mt, sz));
