@@ -116,16 +116,35 @@ __device__ __forceinline__ int cell_stride(const Grid& g) {
     return C == 0 ? 1 : (C == 1 ? g.n[0] : g.n[0] * g.n[1]);
 }
 
+// Tiles of kCellBlock (64 x 4 x 1 cells) over the grid, taken in the
+// XCD-aware order (xcd_tile): a launch of xcd_blocks(cell_tiles(g)) blocks.
+// Consecutive tiles run on one XCD, so a tile's z +- 1 neighbour planes were
+// mostly brought into that XCD's L2 by the tiles before it.
+inline uint3 cell_tiles(const Grid& g) { return make_uint3(unsigned((g.n[0] + 63) / 64), unsigned((g.n[1] + 3) / 4), unsigned(g.n[2])); }
+inline unsigned cell_tile_blocks(const Grid& g) {
+    const uint3 t = cell_tiles(g);
+    return xcd_blocks(dim3(t.x, t.y, t.z));
+}
+
+__device__ __forceinline__ bool xcd_cell(const VBlock& vb, const uint3& tiles, int& x, int& y, int& z) {
+    uint3 t;
+    unsigned id;
+    if (!xcd_tile(vb, tiles, t, id)) return false;
+    x = int(t.x) * 64 + int(threadIdx.x);
+    y = int(t.y) * 4 + int(threadIdx.y);
+    z = int(t.z);
+    return true;
+}
+
 // ------------------------------------------------------------ forces (stage 0)
 // B:2001-2003: solid temperature, then omega and |omega| from the
 // cell-centred velocity with the flow ghost rules (B:870-900).
 __device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                  const uint8_t* __restrict__ occ, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
-                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature) {
-    const int x = vb.idx.x * blockDim.x + threadIdx.x;
-    const int y = vb.idx.y * blockDim.y + threadIdx.y;
-    const int z = vb.idx.z * blockDim.z + threadIdx.z;
-    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature, uint3 tiles) {
+    int x, y, z;
+    if (!xcd_cell(vb, tiles, x, y, z)) return;
+    if (x >= g.n[0] || y >= g.n[1]) return;
     const int64_t i = cidx(g, x, y, z);
     if (occ[i] != 0) {
         if (temperature) temperature[i] = solid_temperature[i];
@@ -133,12 +152,27 @@ __device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const f
         return;
     }
     const float h = g.h;
-    const float dvz_dy = 0.5f * (load_center_component(g, w, 2, x, y + 1, z) - load_center_component(g, w, 2, x, y - 1, z)) / h;
-    const float dvy_dz = 0.5f * (load_center_component(g, v, 1, x, y, z + 1) - load_center_component(g, v, 1, x, y, z - 1)) / h;
-    const float dvx_dz = 0.5f * (load_center_component(g, u, 0, x, y, z + 1) - load_center_component(g, u, 0, x, y, z - 1)) / h;
-    const float dvz_dx = 0.5f * (load_center_component(g, w, 2, x + 1, y, z) - load_center_component(g, w, 2, x - 1, y, z)) / h;
-    const float dvy_dx = 0.5f * (load_center_component(g, v, 1, x + 1, y, z) - load_center_component(g, v, 1, x - 1, y, z)) / h;
-    const float dvx_dy = 0.5f * (load_center_component(g, u, 0, x, y + 1, z) - load_center_component(g, u, 0, x, y - 1, z)) / h;
+    // centre value of component comp at cell (xx, yy, zz): a domain cell's is
+    // the face average of center_velocity (32-bit indices), the ghost rules
+    // only for the neighbours outside the domain (edge cells' waves take the
+    // slow path for those neighbours only)
+    const bool fast = index32(g);
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    auto cc = [&](const float* __restrict__ f, int comp, int xx, int yy, int zz) {
+        if (fast && xx >= 0 && yy >= 0 && zz >= 0 && xx < n0 && yy < n1 && zz < n2) {
+            const int d0 = n0 + (comp == 0), d1 = n1 + (comp == 1);
+            const int lo = (zz * d1 + yy) * d0 + xx;
+            const int hi = lo + (comp == 0 ? 1 : (comp == 1 ? d0 : d0 * d1));
+            return 0.5f * (f[lo] + f[hi]);
+        }
+        return load_center_component(g, f, comp, xx, yy, zz);
+    };
+    const float dvz_dy = 0.5f * (cc(w, 2, x, y + 1, z) - cc(w, 2, x, y - 1, z)) / h;
+    const float dvy_dz = 0.5f * (cc(v, 1, x, y, z + 1) - cc(v, 1, x, y, z - 1)) / h;
+    const float dvx_dz = 0.5f * (cc(u, 0, x, y, z + 1) - cc(u, 0, x, y, z - 1)) / h;
+    const float dvz_dx = 0.5f * (cc(w, 2, x + 1, y, z) - cc(w, 2, x - 1, y, z)) / h;
+    const float dvy_dx = 0.5f * (cc(v, 1, x + 1, y, z) - cc(v, 1, x - 1, y, z)) / h;
+    const float dvx_dy = 0.5f * (cc(u, 0, x, y + 1, z) - cc(u, 0, x, y - 1, z)) / h;
     const float wx = dvz_dy - dvy_dz, wy = dvx_dz - dvz_dx, wz = dvy_dx - dvx_dy;
     ox[i] = wx;
     oy[i] = wy;
@@ -150,7 +184,7 @@ __device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const f
 
 __global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                  const uint8_t* __restrict__ occ, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
-                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature) { vorticity_body(hw_block(), g, u, v, w, occ, ox, oy, oz, om, temperature, solid_temperature); }
+                                 float* __restrict__ om, float* __restrict__ temperature, const float* __restrict__ solid_temperature, uint3 tiles) { vorticity_body(hw_block(), g, u, v, w, occ, ox, oy, oz, om, temperature, solid_temperature, tiles); }
 
 struct vorticity_k {
     template <class... A>
@@ -162,19 +196,25 @@ __device__ __forceinline__ void cell_force_body(const VBlock& vb, Grid g, const 
                                   const float* __restrict__ rho, const float* __restrict__ T, const uint8_t* __restrict__ occ,
                                   const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
                                   const float* __restrict__ om, float a, float b, float tamb, float eps, float* __restrict__ fx,
-                                  float* __restrict__ fy, float* __restrict__ fz) {
-    const int x = vb.idx.x * blockDim.x + threadIdx.x;
-    const int y = vb.idx.y * blockDim.y + threadIdx.y;
-    const int z = vb.idx.z * blockDim.z + threadIdx.z;
-    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) return;
+                                  float* __restrict__ fy, float* __restrict__ fz, uint3 tiles) {
+    int x, y, z;
+    if (!xcd_cell(vb, tiles, x, y, z)) return;
+    if (x >= g.n[0] || y >= g.n[1]) return;
     const int64_t i = cidx(g, x, y, z);
     float sx = fux[i], sy = fuy[i], sz = fuz[i];
     if (occ[i] == 0) {
         sy += -a * rho[i] + b * (T[i] - tamb);
         const float h = g.h;
-        const float gx = 0.5f * (load_flow_cell(g, om, x + 1, y, z) - load_flow_cell(g, om, x - 1, y, z)) / h;
-        const float gy = 0.5f * (load_flow_cell(g, om, x, y + 1, z) - load_flow_cell(g, om, x, y - 1, z)) / h;
-        const float gz = 0.5f * (load_flow_cell(g, om, x, y, z + 1) - load_flow_cell(g, om, x, y, z - 1)) / h;
+        // |omega| at a neighbour: a plain load for domain cells (32-bit
+        // indices), the ghost rule only for the neighbours outside the domain
+        const bool fast = index32(g);
+        auto om_at = [&](int xx, int yy, int zz) {
+            if (fast && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.n[0] && yy < g.n[1] && zz < g.n[2]) return om[(zz * g.n[1] + yy) * g.n[0] + xx];
+            return load_flow_cell(g, om, xx, yy, zz);
+        };
+        const float gx = 0.5f * (om_at(x + 1, y, z) - om_at(x - 1, y, z)) / h;
+        const float gy = 0.5f * (om_at(x, y + 1, z) - om_at(x, y - 1, z)) / h;
+        const float gz = 0.5f * (om_at(x, y, z + 1) - om_at(x, y, z - 1)) / h;
         const float gm = sqrtf(gx * gx + gy * gy + gz * gz);
         if (!(gm < 1.0e-6f)) {
             const float inv = 1.0f / gm;
@@ -194,7 +234,7 @@ __global__ void cell_force_kernel(Grid g, const float* __restrict__ fux, const f
                                   const float* __restrict__ rho, const float* __restrict__ T, const uint8_t* __restrict__ occ,
                                   const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
                                   const float* __restrict__ om, float a, float b, float tamb, float eps, float* __restrict__ fx,
-                                  float* __restrict__ fy, float* __restrict__ fz) { cell_force_body(hw_block(), g, fux, fuy, fuz, rho, T, occ, ox, oy, oz, om, a, b, tamb, eps, fx, fy, fz); }
+                                  float* __restrict__ fy, float* __restrict__ fz, uint3 tiles) { cell_force_body(hw_block(), g, fux, fuy, fuz, rho, T, occ, ox, oy, oz, om, a, b, tamb, eps, fx, fy, fz, tiles); }
 
 struct cell_force_k {
     template <class... A>
@@ -1313,12 +1353,14 @@ void enqueue_time_step(Context& c) {
 
 void enqueue_forces(Context& c) {
     const Grid& g = c.grid;
-    vorticity_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1],
-                                                                 c.vorticity[2], c.vorticity_magnitude, c.temperature, c.solid_temperature);
-    cell_force_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(
+    const uint3 tiles = cell_tiles(g);
+    vorticity_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0],
+                                                                         c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.temperature,
+                                                                         c.solid_temperature, tiles);
+    cell_force_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(
         g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
         c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor, c.config.ambient_temperature,
-        c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2]);
+        c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2], tiles);
     launch_face_force<0>(c);
     launch_face_force<1>(c);
     launch_face_force<2>(c);
@@ -1355,8 +1397,9 @@ void enqueue_scalars(Context& c) {
 
 void enqueue_diagnostics(Context& c) {
     const Grid& g = c.grid;
-    vorticity_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1],
-                                                                 c.vorticity[2], c.vorticity_magnitude, nullptr, nullptr);
+    vorticity_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0],
+                                                                                 c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, nullptr,
+                                                                                 nullptr, cell_tiles(g));
     diagnostics_kernel<<<cell_grid(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.center_velocity[0],
                                                                    c.center_velocity[1], c.center_velocity[2], c.velocity_magnitude, c.divergence);
     hip_check(hipGetLastError(), "diagnostics");
@@ -1541,19 +1584,21 @@ void build_batched_stages(const std::vector<Context*>& P, TableArena& A, Batched
     // forces
     {
         LaunchBuilder<vorticity_k, kCellThreads, Grid, const float*, const float*, const float*, const uint8_t*, float*, float*, float*, float*, float*,
-                      const float*>
+                      const float*, uint3>
             V;
         LaunchBuilder<cell_force_k, kCellThreads, Grid, const float*, const float*, const float*, const float*, const float*, const uint8_t*,
-                      const float*, const float*, const float*, const float*, float, float, float, float, float*, float*, float*>
+                      const float*, const float*, const float*, const float*, float, float, float, float, float*, float*, float*, uint3>
             F;
         for (unsigned p = 0; p < P.size(); ++p) {
             Context& c = *P[p];
             const Grid& g = c.grid;
-            V.add(p, cell_grid(g), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
-                  c.vorticity_magnitude, c.temperature, c.solid_temperature);
-            F.add(p, cell_grid(g), g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0],
-                  c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor,
-                  c.config.ambient_temperature, c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2]);
+            const uint3 tiles = cell_tiles(g);
+            V.add(p, dim3(cell_tile_blocks(g)), g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
+                  c.vorticity_magnitude, c.temperature, c.solid_temperature, tiles);
+            F.add(p, dim3(cell_tile_blocks(g)), g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy,
+                  c.vorticity[0], c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.config.buoyancy_density_factor,
+                  c.config.buoyancy_temperature_factor, c.config.ambient_temperature, c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1],
+                  c.cell_force[2], tiles);
         }
         out.forces.push_back(V.finish(A, kCellBlock));
         out.forces.push_back(F.finish(A, kCellBlock));
