@@ -45,6 +45,9 @@ import smoke_abi as abi  # noqa: E402
 METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_coarse16.bin"
+# damped-Jacobi sweeps after the network at the coarsest level of the learned solver (150 steps at
+# 256^3: 0, 2, 4, 8, 16 sweeps gave 100.5, 101.2, 103.3, 99.3, 101.8 steps/s on the 12-sphere scene)
+LEARNED_COARSE_SWEEPS = 4
 # algorithmic HBM bytes per cell
 SWEEP_BYTES_PER_CELL = 13       # damped-Jacobi sweep: reads r, x (4+4), flags (1); writes x' (4)
 RESTRICT_BYTES_PER_CELL = 9     # reads r, x, flags (+ 1/8 of a cell written)
@@ -93,8 +96,9 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
     if solver == "jacobi":
         solver_options(ctx, check_interval, warm_start)
     elif solver == "learned":
+        # the network's coarse solve followed by 4 damped-Jacobi sweeps at 16^3
         ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
-                                   coarse_sweeps=0)
+                                   coarse_sweeps=LEARNED_COARSE_SWEEPS)
         ctx.set_learned_preconditioner(net)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "smoothed":
@@ -372,7 +376,8 @@ def main():
                                 f"adaptive dt; value = {head}"),
                    "grid": [n, n, n], "scene": args.scene, "relative_tolerance": 1e-6,
                    "warm_start": ["zero", "previous", "extrapolated"][args.warm_start],
-                   "solvers": {"learned": "FPCG + 5-level V(4,4)-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + ")",
+                   "solvers": {"learned": ("FPCG + 5-level V(4,4)-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + f") + "
+                                           f"{LEARNED_COARSE_SWEEPS} Jacobi sweeps"),
                                "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(4,4)-cycle, Jacobi coarse solver (no network)",
                                "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,

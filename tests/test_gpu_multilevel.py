@@ -194,13 +194,15 @@ def test_multilevel_solve_on_vector_row_grid():
             assert st.iterations < 40, (k, st.iterations)
 
 
-def test_coarse_network_solve_in_few_launches_matches_the_general_path():
+@pytest.mark.parametrize("sweeps", [0, 4])
+def test_coarse_network_solve_in_few_launches_matches_the_general_path(sweeps):
     """The coarse-grid network solve runs as two single-block kernels around
-    the convolutions (deflation, normalisation, input; output and deflated
-    correction); with a capture buffer attached the V-cycle takes the general
-    path (separate deflation, sum and input kernels).  Both apply the same
-    operator: the sums run in other orders, so the network's bfloat16 input
-    may round differently in a few cells, and the V-cycles agree to that."""
+    the convolutions (deflation, normalisation, input; output, the Jacobi
+    sweeps after the network and the deflated correction); with a capture
+    buffer attached the V-cycle takes the general path (separate deflation,
+    sum, input and sweep kernels).  Both apply the same operator: the sums run
+    in other orders, so the network's bfloat16 input may round differently in
+    a few cells, and the V-cycles agree to that."""
     if not COARSE_WEIGHTS.exists():
         pytest.skip("coarse-grid weights not trained yet")
     import train_preconditioner as tp
@@ -211,7 +213,7 @@ def test_coarse_network_solve_in_few_launches_matches_the_general_path():
     hip = abi.Hip()
     with abi.LearnedPreconditioner(seed=0, path=COARSE_WEIGHTS) as net, abi.SmokeContext(cfg, hip=hip) as ctx:
         ctx.set_occupancy(occ)
-        ctx.set_multilevel_options(levels=3, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=0)
+        ctx.set_multilevel_options(levels=3, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=sweeps)
         ctx.set_learned_preconditioner(net)
         _tolerance(ctx, abi.PRECONDITIONER_MULTILEVEL)
         for k in range(4):
@@ -239,5 +241,5 @@ def test_coarse_network_solve_in_few_launches_matches_the_general_path():
         ctx.set_learned_preconditioner(None)
     np.testing.assert_array_equal(z_few, z_again)             # deterministic
     e = float(np.linalg.norm(z_few.astype(np.float64) - z_general) / np.linalg.norm(z_general))
-    print(f"PARITY coarse_network_few_launches_vs_general rel={e:.3e}", flush=True)
+    print(f"PARITY coarse_network_few_launches_vs_general sweeps={sweeps} rel={e:.3e}", flush=True)
     assert np.isfinite(z_few).all() and e < 1e-3

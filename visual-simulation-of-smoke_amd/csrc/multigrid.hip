@@ -663,10 +663,16 @@ void vcycle(Context& c, int l) {
     const auto& o = m.options;
     if (l == m.built - 1) {
         const bool capturing = m.capture && m.captured < m.capacity;
-        if (o.coarse_solver == SMOKE_SIMULATION_COARSE_NETWORK && o.coarse_sweeps == 0 && !capturing && L.cells <= kCoarseSolveMaxCells) {
-            // deflation, network and deflated correction in few launches
+        if (o.coarse_solver == SMOKE_SIMULATION_COARSE_NETWORK && o.coarse_sweeps <= kCoarseSolveMaxSweeps && !capturing &&
+            L.cells <= kCoarseSolveMaxCells) {
+            // deflation, network, sweeps and deflated correction in few launches
             if (!c.learned) throw HipError("multilevel preconditioner: no network attached");
-            preconditioner_coarse_solve(*c.learned, L.r, L.av, L.flags, L.defl, L.n, L.rd, L.z, c.stream);
+            const LevelArgs a = level_args(c, L);
+            CoarseSweeps cs{};
+            for (int f = 0; f < 6; ++f) cs.outflow[f] = a.outflow[f];
+            cs.omega = a.omega;
+            cs.count = o.coarse_sweeps;
+            preconditioner_coarse_solve(*c.learned, L.r, L.av, L.flags, L.defl, L.n, cs, L.rd, L.z, c.stream);
             return;
         }
         // deflated right-hand side (what the coarse solver sees, and what the

@@ -30,10 +30,21 @@ void preconditioner_apply(SmokeNnPreconditioner& net, const float* r, const uint
 
 // The multilevel preconditioner's coarse-grid solve with the network: the
 // deflation of v = 1 (defl[0] = v^T A v on entry; writes defl[1] = c and
-// the deflated right-hand side rd), z' = M(rd) and z = z' + (c - Av.z' /
-// v.Av) v, in two single-block launches around the network's convolutions.
-// For volumes of at most kCoarseSolveMaxCells cells.
+// the deflated right-hand side rd), z' = M(rd), `sweeps` damped-Jacobi
+// sweeps on A z' = rd from z', and z = z' + (c - Av.z' / v.Av) v, in two
+// single-block launches around the network's convolutions.  For volumes of
+// at most kCoarseSolveMaxCells cells.
 constexpr int64_t kCoarseSolveMaxCells = 8192;
+constexpr int kCoarseSolveMaxSweeps = 64;
+// the coarse level's operator as the sweeps see it: the flag byte's
+// neighbour bits (a set bit at the domain edge is a periodic neighbour) plus
+// the OUTFLOW domain faces (x-, x+, y-, y+, z-, z+) on the diagonal; the
+// damping and the sweep count
+struct CoarseSweeps {
+    int outflow[6];
+    float omega;
+    int count;
+};
 void preconditioner_coarse_solve(SmokeNnPreconditioner& net, const float* r, const float* av, const uint8_t* flags, double* defl, const int dims[3],
-                                 float* rd, float* z, hipStream_t s);
+                                 const CoarseSweeps& sweeps, float* rd, float* z, hipStream_t s);
 }  // namespace smoke

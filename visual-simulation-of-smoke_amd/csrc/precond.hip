@@ -363,24 +363,82 @@ __global__ __launch_bounds__(kCoarseThreads) void coarse_prep_kernel(const float
     }
 }
 
-// z = s out on fluid rows (0 on identity rows), then z += (c - Av.z / v.Av) on fluid rows
+// z = s out on fluid rows (0 on identity rows); then `sweeps.count` damped
+// Jacobi sweeps z <- z + omega D^-1 (rd - A z) on the fluid rows (the
+// multilevel cycle's sweep: neighbours from the flag byte, periodic images
+// at the domain edge, OUTFLOW faces on the diagonal, taps summed in the
+// order x-, x+, y-, y+, z-, z+), z kept in LDS; then z += (c - Av.z / v.Av)
+// on fluid rows
 __global__ __launch_bounds__(kCoarseThreads) void coarse_finish_kernel(const float* __restrict__ out, const uint8_t* __restrict__ flags,
-                                                                       const float* __restrict__ av, int cells, const double* __restrict__ sums,
+                                                                       const float* __restrict__ av, const float* __restrict__ rd, int n0, int n1,
+                                                                       int n2, CoarseSweeps sweeps, const double* __restrict__ sums,
                                                                        const double* __restrict__ defl, float* __restrict__ z) {
     __shared__ double red[kCoarseThreads / 64];
+    __shared__ float zs[kCoarseSolveMaxCells];
     const int t = int(threadIdx.x);
+    const int cells = n0 * n1 * n2;
     const double tot = sums[0];
     const float sc = float(tot > 0.0 ? sqrt(tot / double(cells)) : 1.0);
     float zv[kCoarsePerThread];
     bool fluid[kCoarsePerThread];
-    double s = 0.0;
 #pragma unroll
     for (int k = 0; k < kCoarsePerThread; ++k) {
         const int i = t + k * kCoarseThreads;
         const bool in = i < cells;
         fluid[k] = in && !(flags[i] & kIdentity);
         zv[k] = fluid[k] ? sc * out[i] : 0.0f;
-        s += in ? double(av[i]) * double(zv[k]) : 0.0;
+    }
+    if (sweeps.count > 0) {
+        unsigned fl[kCoarsePerThread];
+        float rv[kCoarsePerThread], inv[kCoarsePerThread], dg[kCoarsePerThread];
+#pragma unroll
+        for (int k = 0; k < kCoarsePerThread; ++k) {
+            const int i = t + k * kCoarseThreads;
+            const bool in = i < cells;
+            fl[k] = in ? unsigned(flags[i]) : 0u;
+            rv[k] = in ? rd[i] : 0.0f;
+            if (in) zs[i] = zv[k];
+            const int x = i % n0, y = (i / n0) % n1, zz = i / (n0 * n1);
+            const int of = (x == 0 && sweeps.outflow[0]) + (x == n0 - 1 && sweeps.outflow[1]) + (y == 0 && sweeps.outflow[2]) +
+                           (y == n1 - 1 && sweeps.outflow[3]) + (zz == 0 && sweeps.outflow[4]) + (zz == n2 - 1 && sweeps.outflow[5]);
+            const int d = __popc(fl[k] & 0x3fu) + of;
+            dg[k] = float(d > 1 ? d : 1);
+            inv[k] = __builtin_amdgcn_rcpf(dg[k]);
+        }
+        const int sy = n0, sz = n0 * n1;
+        for (int sw = 0; sw < sweeps.count; ++sw) {
+            __syncthreads();   // zs holds the previous iterate
+#pragma unroll
+            for (int k = 0; k < kCoarsePerThread; ++k) {
+                const int i = t + k * kCoarseThreads;
+                if (!fluid[k]) continue;
+                const unsigned f = fl[k];
+                const int x = i % n0, y = (i / n0) % n1, zz = i / (n0 * n1);
+                // neighbour taps (set flag bits only: the neighbour exists, at the edge its periodic image)
+                const float t0 = (f & 1u) ? zs[x > 0 ? i - 1 : i + n0 - 1] : 0.0f;
+                const float t1 = (f & 2u) ? zs[x < n0 - 1 ? i + 1 : i - (n0 - 1)] : 0.0f;
+                const float t2 = (f & 4u) ? zs[y > 0 ? i - sy : i + (n1 - 1) * sy] : 0.0f;
+                const float t3 = (f & 8u) ? zs[y < n1 - 1 ? i + sy : i - (n1 - 1) * sy] : 0.0f;
+                const float t4 = (f & 16u) ? zs[zz > 0 ? i - sz : i + (n2 - 1) * sz] : 0.0f;
+                const float t5 = (f & 32u) ? zs[zz < n2 - 1 ? i + sz : i - (n2 - 1) * sz] : 0.0f;
+                const float s = t0 + t1 + t2 + t3 + t4 + t5;
+                const float xi = zv[k];
+                const float res = rv[k] - (dg[k] * xi - s);
+                zv[k] = xi + sweeps.omega * res * inv[k];
+            }
+            __syncthreads();   // every read of the previous iterate is done
+#pragma unroll
+            for (int k = 0; k < kCoarsePerThread; ++k) {
+                const int i = t + k * kCoarseThreads;
+                if (fluid[k]) zs[i] = zv[k];
+            }
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCoarsePerThread; ++k) {
+        const int i = t + k * kCoarseThreads;
+        s += i < cells ? double(av[i]) * double(zv[k]) : 0.0;
     }
     const double az = block_sum_fixed(s, red);
     const double vav = defl[0];
@@ -563,16 +621,17 @@ void preconditioner_apply(SmokeNnPreconditioner& net, const float* r, const uint
 }
 
 void preconditioner_coarse_solve(SmokeNnPreconditioner& net, const float* r, const float* av, const uint8_t* flags, double* defl, const int dims[3],
-                                 float* rd, float* z, hipStream_t s) {
+                                 const CoarseSweeps& sweeps, float* rd, float* z, hipStream_t s) {
     const int64_t cells = int64_t(dims[0]) * dims[1] * dims[2];
     if (cells > kCoarseSolveMaxCells) throw HipError("coarse solve: volume too large for the single-block kernels");
+    if (sweeps.count < 0 || sweeps.count > kCoarseSolveMaxSweeps) throw HipError("coarse solve: sweep count out of range");
     ensure_workspace(net, 1, dims, false);
     pack_all(net, s);
     const Dims d = level_dims(dims);
     coarse_prep_kernel<<<1, kCoarseThreads, 0, s>>>(r, av, flags, int(cells), defl, rd, net.sums, net.T[kIn]);
     hip_check(hipGetLastError(), "coarse_prep_kernel");
     forward(net, 1, d, s);
-    coarse_finish_kernel<<<1, kCoarseThreads, 0, s>>>(net.z, flags, av, int(cells), net.sums, defl, z);
+    coarse_finish_kernel<<<1, kCoarseThreads, 0, s>>>(net.z, flags, av, rd, dims[0], dims[1], dims[2], sweeps, net.sums, defl, z);
     hip_check(hipGetLastError(), "coarse_finish_kernel");
 }
 
