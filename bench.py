@@ -45,6 +45,8 @@ import smoke_abi as abi  # noqa: E402
 METRIC = "steps/s @256³ to rel-resid 1e-6: learned-FPCG vs plain PCG; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_coarse16.bin"
+# the grid-level network (64^3-crop training run), for the "network" solver
+GRID_WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_plume256_crop64.bin"
 # damped-Jacobi sweeps after the network at the coarsest level of the learned solver (150 steps at
 # 256^3: 0, 2, 4, 8, 16 sweeps gave 100.5, 101.2, 103.3, 99.3, 101.8 steps/s on the 12-sphere scene)
 LEARNED_COARSE_SWEEPS = 4
@@ -106,6 +108,11 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
         ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
                                    coarse_sweeps=16)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
+    elif solver == "network":
+        # the network as the preconditioner of the 256^3 grid itself (flexible PCG,
+        # Jacobi-PCG fallback when the residual drops < 10 % over 5 iterations)
+        ctx.set_learned_preconditioner(net)
+        solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_LEARNED)
     elif solver == "multilevel":
         ctx.set_multilevel_options(levels=7, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
                                    coarse_sweeps=8)
@@ -291,22 +298,26 @@ def main():
     copy_gbps, _ = abi.measure_copy_bandwidth(1 << 30, 20)
     solvers = [s.strip() for s in args.solvers.split(",") if s.strip()]
     extra_scenes = [s.strip() for s in args.extra_scenes.split(",") if s.strip() and s.strip() != args.scene]
-    net = abi.LearnedPreconditioner(seed=0, path=args.weights) if "learned" in solvers else None
+    nets = {}
+    if "learned" in solvers:
+        nets["learned"] = abi.LearnedPreconditioner(seed=0, path=args.weights)
+    if "network" in solvers:
+        nets["network"] = abi.LearnedPreconditioner(seed=0, path=GRID_WEIGHTS)
     results = {}
     for solver in solvers:
-        results[solver] = run_solver(solver, args, hip, net, not args.no_timing, args.scene)
+        results[solver] = run_solver(solver, args, hip, nets.get(solver), not args.no_timing, args.scene)
         print(f"bench {args.scene} {solver}: {results[solver]['steps_per_s']} steps/s", file=sys.stderr, flush=True)
     scenes = {}
     for scene in extra_scenes:
         scenes[scene] = {}
         for solver in solvers:
-            scenes[scene][solver] = run_solver(solver, args, hip, net, False, scene)
+            scenes[scene][solver] = run_solver(solver, args, hip, nets.get(solver), False, scene)
             print(f"bench {scene} {solver}: {scenes[scene][solver]['steps_per_s']} steps/s", file=sys.stderr, flush=True)
         if "learned" in scenes[scene] and "jacobi" in scenes[scene]:
             scenes[scene]["learned_vs_plain_pcg_speedup"] = round(
                 scenes[scene]["learned"]["steps_per_s"] / scenes[scene]["jacobi"]["steps_per_s"], 3)
-    if net is not None:
-        net.close()
+    for network in nets.values():
+        network.close()
     head = solvers[0]
     value = results[head]["steps_per_s"]
 
@@ -379,7 +390,8 @@ def main():
                    "solvers": {"learned": ("FPCG + 5-level V(4,4)-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + f") + "
                                            f"{LEARNED_COARSE_SWEEPS} Jacobi sweeps"),
                                "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(4,4)-cycle, Jacobi coarse solver (no network)",
-                               "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps"}},
+                               "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps",
+                               "network": "FPCG preconditioned by the network on the 256^3 grid itself (" + GRID_WEIGHTS.name + ")"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,
         "scenes": scenes,
         "roofline": roofline, "cpu_baseline": None,
