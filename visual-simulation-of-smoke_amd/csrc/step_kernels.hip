@@ -921,6 +921,11 @@ __device__ __forceinline__ void advect_scalar_cell(const Grid& g, const float* _
 }
 
 // WRAP = false: no periodic flow or scalar axis (the wrap code compiled out).
+// Two phases through one LDS pool: the velocity and occupancy boxes serve
+// the traces (each lane keeps its cells' sample stencils in registers),
+// then the same LDS takes the T and rho boxes for the samples.  The pool is
+// the larger of the two (27 KB, not the 56 KB of all five boxes at once),
+// so three blocks fit a CU instead of two.
 template <bool WRAP>
 __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v,
                                                                            const float* __restrict__ w, const uint8_t* __restrict__ occ,
@@ -934,8 +939,15 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     constexpr int kV = (kSaX + 2) * (kSaY + 3) * (kSaZ + 2);
     constexpr int kW = (kSaX + 2) * (kSaY + 2) * (kSaZ + 3);
     constexpr int kO = (kSaX + 2) * (kSaY + 2) * (kSaZ + 2);
-    __shared__ float sT[kS], sR[kS], su[kU], sv[kV], sw[kW];
+    constexpr int kPool = kU + kV + kW > 2 * kS ? kU + kV + kW : 2 * kS;
+    constexpr int kCells = kSaZ / kSaZStep;   // cells per lane
+    __shared__ float pool[kPool];
     __shared__ uint8_t so[kO];
+    float* const su = pool;
+    float* const sv = pool + kU;
+    float* const sw = pool + kU + kV;
+    float* const sT = pool;
+    float* const sR = pool + kS;
     uint3 tile;
     unsigned id;
     if (!xcd_tile(vb, tiles, tile, id)) return;
@@ -957,19 +969,12 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
     if (inside) {
         ByteRows<kSaX + 2, kSaY + 2, kSaZ + 2> ro;
         ro.load(bo, wave, lane, [&](int i, int j, int k) { return occ[(k * n1 + j) * n0 + i]; });   // bo lies inside bs
-        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sT, Ttmp, bs, n0, n1, wave, lane);
-        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sR, rtmp, bs, n0, n1, wave, lane);
         dma_rows<kSaX + 3, kSaY + 2, kSaZ + 2>(su, u, bu, n0 + 1, n1, wave, lane);
         dma_rows<kSaX + 2, kSaY + 3, kSaZ + 2>(sv, v, bv, n0, n1 + 1, wave, lane);
         dma_rows<kSaX + 2, kSaY + 2, kSaZ + 3>(sw, w, bw, n0, n1, wave, lane);
         ro.store(so, wave, lane);
     } else {
         // edge tiles (and grids too large for 32-bit indices): every entry with the ghost rules
-        for (int q = tid; q < kS; q += kSaThreads) {
-            const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
-            sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
-            sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
-        }
         auto stage_face = [&](int comp, const float* __restrict__ f, float* lds, const LdsBox& bx) {
             const int n = box_size(bx);
             for (int q = tid; q < n; q += kSaThreads) {
@@ -987,15 +992,21 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
         }
     }
     __syncthreads();
+    // phase 1: the traces; solid cells and fallbacks are finished here
     const int x = X0 + tid % kSaX, y = Y0 + (tid / kSaX) % kSaY;
-    if (x >= g.n[0] || y >= g.n[1]) return;
+    const bool column = x < g.n[0] && y < g.n[1];
     const float cps = *dt_ptr / g.h;   // cells moved per unit velocity and time step
     const bool cubic = g.cubic != 0;
     const bool same_periodic = g.temperature.periodic[0] == g.density.periodic[0] && g.temperature.periodic[1] == g.density.periodic[1] &&
                                g.temperature.periodic[2] == g.density.periodic[2];
-    for (int tz = tid / kSaPlane; tz < kSaZ; tz += kSaZStep) {
-        const int z = Z0 + tz;
-        if (z >= g.n[2]) break;
+    const int tz0 = tid / kSaPlane;
+    Stencil st[kCells];
+    bool sample[kCells];
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        sample[c] = false;
+        const int z = Z0 + tz0 + c * kSaZStep;
+        if (!column || z >= g.n[2]) continue;
         const int64_t i = cidx(g, x, y, z);
         if (so[box_base(bo, x, y, z)] != 0) {
             T[i] = solid_T[i];
@@ -1004,21 +1015,39 @@ __device__ __forceinline__ void advect_scalars_staged_body(const VBlock& vb, Gri
         }
         float px, py, pz;
         bool ok = same_periodic && staged_trace<-1, WRAP>(g, su, bu, sv, bv, sw, bw, so, bo, cps, x, y, z, px, py, pz);
-        Stencil st{};
         if (ok) {
-            st = locate<WRAP>(g, g.temperature.periodic, -1, px, py, pz);
-            ok = cubic ? in_box<-1, 2>(bs, st) : in_box<0, 1>(bs, st);
+            st[c] = locate<WRAP>(g, g.temperature.periodic, -1, px, py, pz);
+            ok = cubic ? in_box<-1, 2>(bs, st[c]) : in_box<0, 1>(bs, st[c]);
         }
         if (!ok) {
             fallback[atomicAdd(fallback_count, 1u)] = int(i);
             continue;
         }
+        sample[c] = true;
+    }
+    __syncthreads();   // the velocity boxes are read: the pool takes the scalars
+    if (inside) {
+        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sT, Ttmp, bs, n0, n1, wave, lane);
+        dma_rows<kSaX + 4, kSaY + 4, kSaZ + 4>(sR, rtmp, bs, n0, n1, wave, lane);
+    } else {
+        for (int q = tid; q < kS; q += kSaThreads) {
+            const int i = bs.o[0] + q % bs.d[0], j = bs.o[1] + (q / bs.d[0]) % bs.d[1], k = bs.o[2] + q / (bs.d[0] * bs.d[1]);
+            sT[q] = load_scalar(g, g.temperature, Ttmp, i, j, k);
+            sR[q] = load_scalar(g, g.density, rtmp, i, j, k);
+        }
+    }
+    __syncthreads();
+    // phase 2: the samples
+#pragma unroll
+    for (int c = 0; c < kCells; ++c) {
+        if (!sample[c]) continue;
+        const int64_t i = cidx(g, x, y, Z0 + tz0 + c * kSaZStep);
         float tv, rv;
         if (cubic) {
-            staged_cubic2(sT, sR, bs, st, tv, rv);
+            staged_cubic2(sT, sR, bs, st[c], tv, rv);
         } else {
-            tv = staged_linear(sT, bs, st);
-            rv = staged_linear(sR, bs, st);
+            tv = staged_linear(sT, bs, st[c]);
+            rv = staged_linear(sR, bs, st[c]);
         }
         T[i] = tv;
         rho[i] = rv;
