@@ -56,11 +56,13 @@ def _tolerance(ctx, precond, **kw):
 
 
 @pytest.mark.parametrize("pre,post,levels,shape", [(2, 2, 3, (21, 18, 24)), (1, 1, 4, (21, 18, 24)), (3, 2, 3, (21, 18, 24)),
-                                                    (2, 2, 4, (97, 96, 128)), (1, 3, 3, (96, 97, 128))])
+                                                    (2, 2, 4, (97, 96, 128)), (1, 3, 3, (96, 97, 128)), (4, 1, 3, (97, 96, 128)),
+                                                    (2, 2, 3, (97, 96, 124))])
 def test_vcycle_matches_restatement(pre, post, levels, shape):
-    """(nz, ny, nx) shapes with x periodic and odd z; the larger one (over 2^20
-    cells) runs the fine level through the element-wise first-sweep and
-    prolongation passes."""
+    """(nz, ny, nx) shapes with x periodic and odd z.  The larger ones (over
+    2^20 cells) run the fine level through the element-wise first-sweep and
+    prolongation passes, in float4 rows when nx % 8 == 0 (nx = 124: the
+    scalar kernels)."""
     flow = [P, P, W, W, W, W]
     cfg = abi.make_config(shape[2], shape[1], shape[0], flow=flow)
     occ = _spheres(shape, 5, 3)
