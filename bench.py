@@ -12,11 +12,12 @@ step).  Every pressure solve reaches a true relative residual
 Solvers, each timed on its own context over the same steps:
   learned     flexible PCG preconditioned by a 5-level V(4,4)-cycle whose coarsest
               (16^3) level is solved by the trained network
-              (weights/preconditioner_coarse16.bin) inside an exact deflation
-              -- the headline `value`;
+              (weights/preconditioner_coarse16.bin) inside an exact deflation,
+              then 8 Jacobi sweeps -- the headline `value`;
   jacobi      plain Jacobi PCG (the hand-written baseline solver);
-  multilevel  the same flexible PCG with a 7-level V(4,4)-cycle and Jacobi sweeps
-              on its 4^3 coarsest level (no network), reported beside them;
+  multilevel  the same flexible PCG with a 7-level V(3,3)-cycle and Jacobi sweeps
+              on its 4^3 coarsest level (no network; the fastest classical
+              cycle measured), reported beside them;
   smoothed    (on request) the learned solver's 5-level cycle with the
               network replaced by 16 Jacobi sweeps: what the network gains.
 
@@ -47,9 +48,18 @@ HBM_PEAK_GBPS = 8000.0
 WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_coarse16.bin"
 # the grid-level network (64^3-crop training run), for the "network" solver
 GRID_WEIGHTS = REPO / "visual-simulation-of-smoke_amd" / "weights" / "preconditioner_plume256_crop64.bin"
+# damping of every V-cycle's Jacobi sweeps: 6/7 minimises damped Jacobi's smoothing factor on the 3D
+# 7-point Laplacian (5/7 against 7/9 at 2/3); 150 steps of the 256^3 12-sphere scene, steps/s at
+# 2/3, 0.75, 0.8, 6/7, 0.9, 0.95: learned V(4,4) 102.3, 104.3, 107.2, 112.2, 104.0, 92.0; classical
+# V(3,3) 99.2, 107.7, 113.3, 116.5, 110.9, 97.6
+OMEGA = 6.0 / 7.0
 # damped-Jacobi sweeps after the network at the coarsest level of the learned solver (150 steps at
-# 256^3: 0, 2, 4, 8, 16 sweeps gave 100.5, 101.2, 103.3, 99.3, 101.8 steps/s on the 12-sphere scene)
-LEARNED_COARSE_SWEEPS = 4
+# 256^3 with OMEGA: 0, 4, 8, 16 sweeps gave 111.9, 112.3, 113.9, 106.0 steps/s on the 12-sphere scene)
+LEARNED_COARSE_SWEEPS = 8
+# pre- and post-sweeps: the learned cycle is fastest at V(4,4) (V(3,3): 112.5 against 113.9 steps/s),
+# the classical one at V(3,3) (V(4,4): 110.4 against 116.4)
+LEARNED_SWEEPS = 4
+CLASSICAL_SWEEPS = 3
 # algorithmic HBM bytes per cell
 SWEEP_BYTES_PER_CELL = 13       # damped-Jacobi sweep: reads r, x (4+4), flags (1); writes x' (4)
 RESTRICT_BYTES_PER_CELL = 9     # reads r, x, flags (+ 1/8 of a cell written)
@@ -98,15 +108,15 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
     if solver == "jacobi":
         solver_options(ctx, check_interval, warm_start)
     elif solver == "learned":
-        # the network's coarse solve followed by 4 damped-Jacobi sweeps at 16^3
-        ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_NETWORK,
-                                   coarse_sweeps=LEARNED_COARSE_SWEEPS)
+        # the network's coarse solve followed by damped-Jacobi sweeps at 16^3
+        ctx.set_multilevel_options(levels=5, pre_sweeps=LEARNED_SWEEPS, post_sweeps=LEARNED_SWEEPS, omega=OMEGA,
+                                   coarse_solver=abi.COARSE_NETWORK, coarse_sweeps=LEARNED_COARSE_SWEEPS)
         ctx.set_learned_preconditioner(net)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "smoothed":
         # the learned solver's cycle with its network replaced by Jacobi sweeps (the ablation)
-        ctx.set_multilevel_options(levels=5, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
-                                   coarse_sweeps=16)
+        ctx.set_multilevel_options(levels=5, pre_sweeps=LEARNED_SWEEPS, post_sweeps=LEARNED_SWEEPS, omega=OMEGA,
+                                   coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=16)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     elif solver == "network":
         # the network as the preconditioner of the 256^3 grid itself (flexible PCG,
@@ -114,8 +124,8 @@ def configure(ctx: abi.SmokeContext, solver: str, net, warm_start: int, check_in
         ctx.set_learned_preconditioner(net)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_LEARNED)
     elif solver == "multilevel":
-        ctx.set_multilevel_options(levels=7, pre_sweeps=4, post_sweeps=4, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
-                                   coarse_sweeps=8)
+        ctx.set_multilevel_options(levels=7, pre_sweeps=CLASSICAL_SWEEPS, post_sweeps=CLASSICAL_SWEEPS, omega=OMEGA,
+                                   coarse_solver=abi.COARSE_JACOBI, coarse_sweeps=8)
         solver_options(ctx, check_interval, warm_start, abi.PRECONDITIONER_MULTILEVEL)
     else:
         raise ValueError(solver)
@@ -387,9 +397,10 @@ def main():
                                 f"adaptive dt; value = {head}"),
                    "grid": [n, n, n], "scene": args.scene, "relative_tolerance": 1e-6,
                    "warm_start": ["zero", "previous", "extrapolated"][args.warm_start],
+                   "vcycle_omega": round(OMEGA, 6),
                    "solvers": {"learned": ("FPCG + 5-level V(4,4)-cycle, network coarse solver at 16^3 (" + Path(args.weights).name + f") + "
                                            f"{LEARNED_COARSE_SWEEPS} Jacobi sweeps"),
-                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(4,4)-cycle, Jacobi coarse solver (no network)",
+                               "jacobi": "plain Jacobi PCG", "multilevel": "FPCG + 7-level V(3,3)-cycle, Jacobi coarse solver (no network)",
                                "smoothed": "the learned solver's 5-level cycle, network replaced by 16 Jacobi sweeps",
                                "network": "FPCG preconditioned by the network on the 256^3 grid itself (" + GRID_WEIGHTS.name + ")"}},
         "solvers": results, "learned_vs_plain_pcg_speedup": speedup,

@@ -343,7 +343,8 @@ typedef struct SmokeSimulationMultilevelOptions {
 } SmokeSimulationMultilevelOptions;
 
 /* Defaults: 5 levels (a 16^3 coarsest level for a 256^3 grid), 4 pre- and 4
-   post-sweeps, omega 2/3, network coarse solver with no extra sweeps. */
+   post-sweeps, omega 6/7 (the smallest smoothing factor of damped Jacobi on
+   the 3D 7-point Laplacian), network coarse solver followed by 8 sweeps. */
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_default_multilevel_options(SmokeSimulationMultilevelOptions* out_options);
 /* Validates and stores the options; the hierarchy is (re)built on the next
    multilevel solve.  Fails when a periodic axis does not halve evenly down to

@@ -95,7 +95,8 @@ def spheres_occupancy(n: int, count: int = 12, seed: int = 5) -> np.ndarray:
     return occ
 
 
-def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plume",), coarse_sweeps: int = 16, sweeps: int = 2, log=print):
+def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plume",), coarse_sweeps: int = 16, sweeps: int = 2,
+                    omega: float = 2.0 / 3.0, log=print):
     """Coarsest-level right-hand sides of the V-cycles of `steps` multilevel
     solves per scene; returns {"r": (count, D, H, W) float32, "flags": uint8,
     "crop": (W, H, D)}."""
@@ -108,7 +109,7 @@ def generate_coarse(n: int, levels: int, steps: int, capacity: int, scenes=("plu
         with abi.SmokeContext(cfg, hip=hip) as ctx:
             if scene == "spheres":
                 ctx.set_occupancy(spheres_occupancy(n))
-            ctx.set_multilevel_options(levels=levels, pre_sweeps=sweeps, post_sweeps=sweeps, omega=2.0 / 3.0, coarse_solver=abi.COARSE_JACOBI,
+            ctx.set_multilevel_options(levels=levels, pre_sweeps=sweeps, post_sweeps=sweeps, omega=omega, coarse_solver=abi.COARSE_JACOBI,
                                        coarse_sweeps=coarse_sweeps)
             ctx.set_solver_options(pressure_mode=abi.PRESSURE_TOLERANCE, preconditioner=abi.PRECONDITIONER_MULTILEVEL,
                                    relative_tolerance=1e-6, max_iterations=50000, max_refinements=4, warm_start=2, adaptive_dt=1,
@@ -272,6 +273,7 @@ def main():
     ap.add_argument("--capacity", type=int, default=1500, help="coarse volumes captured per scene")
     ap.add_argument("--spheres", action="store_true", help="also capture the obstacle scene's solves")
     ap.add_argument("--sweeps", type=int, default=3, help="pre- and post-sweeps of the V-cycle that produces the coarse data")
+    ap.add_argument("--omega", type=float, default=2.0 / 3.0, help="damping of that V-cycle's Jacobi sweeps")
     ap.add_argument("--device-crops", action="store_true", help="keep the crops on the GPU (the 20k-crop, 30-epoch run)")
     ap.add_argument("--samples", type=int, default=20000, help="training crops with --device-crops")
     ap.add_argument("--validation", type=int, default=512, help="held-out crops with --device-crops")
@@ -301,7 +303,7 @@ def main():
         return
     if args.coarse:
         scenes = ("plume", "spheres") if args.spheres else ("plume",)
-        data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes, sweeps=args.sweeps)
+        data = generate_coarse(args.grid, args.levels, args.steps, args.capacity, scenes, sweeps=args.sweeps, omega=args.omega)
         print(f"data: {data['r'].shape[0]} coarse volumes of {data['crop']}", flush=True)
         with abi.LearnedPreconditioner(seed=args.seed) as net:
             net.set_loss(abi.NN_LOSS_ENERGY)
