@@ -253,6 +253,15 @@ SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_reset_launches(S
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_get_launches(SmokeSimulationContext context, SmokeSimulationLaunchReport* out_report);
 SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_launches(SmokeSimulationContext context, const SmokeSimulationLaunchReport* shapes);
 
+/* Form of the force pass (stage 0; no counterpart in the reference, whose
+   vorticity and force kernels are B:870-948).  fused != 0 (the default): one
+   kernel stages the cell-centred velocity and omega of a marching 64 x 4
+   column in LDS and writes the cell force, with no omega arrays in HBM (the
+   omega views are recomputed when read).  fused == 0: vorticity_kernel writes
+   omega and |omega|, then cell_force_kernel reads them.  Both give the same
+   bits.  Rebuilds the step graph. */
+SMOKE_SIMULATION_API SmokeSimulationResult smoke_simulation_ext_set_force_pass(SmokeSimulationContext context, int32_t fused);
+
 /* Training pairs of the pressure solve, for a learned preconditioner: crops
    of the last solve's right-hand side r = b, its solution z = A^-1 b (the
    FP64 solution in tolerance mode, the FP32 pressure in fixed mode) and the

@@ -72,6 +72,33 @@ def test_stage_forces(name):
     assert rel_l2(g["temperature"], s["temperature"]) == 0.0
 
 
+@pytest.mark.parametrize("name,n", [(name, (20, 24, 18)) for name in CONFIGS] + [("benchmark", (2, 130, 37)), ("mixed", (131, 9, 70))])
+def test_fused_force_pass_equals_two_pass_form(name, n):
+    """The fused force kernel (omega staged in LDS, periodic images staged
+    across the domain) gives the bits of vorticity_kernel + cell_force_kernel:
+    every face after the force stage, the solid temperature, and the omega
+    views (recomputed lazily after the fused pass).  Grids: the three
+    boundary sets on a ragged 20 x 24 x 18 grid, a periodic x extent of 2
+    (every staged x position an image), and tiles cut in x, y and z."""
+    cfg = abi.make_config(*n, cell_size=1.0 / n[1], dt=0.01, vorticity=2.0, **CONFIGS[name])
+    st = make_state(cfg, 5, solids=2, solid_motion=True)
+    out = []
+    for fused in (False, True):
+        with abi.SmokeContext(cfg) as ctx:
+            load_gpu(ctx, st)
+            ctx.set_force_pass(fused)
+            ctx.run_stages(abi.STAGE_FORCES, abi.STAGE_FORCES)
+            g = ctx.export_state()
+            views = [np.array(v) for v in ctx.view("FLOW_VORTICITY")] + [np.array(ctx.view("FLOW_VORTICITY_MAGNITUDE"))]
+            out.append((g, views))
+    (g0, v0), (g1, v1) = out
+    for k in ("velocity_x", "velocity_y", "velocity_z", "temperature"):
+        np.testing.assert_array_equal(g1[k], g0[k], err_msg=k)
+    for a, b in zip(v1, v0):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(g1["velocity_y"] - st["v"]).max() > 0          # the pass did change the velocity
+
+
 @pytest.mark.parametrize("name", list(CONFIGS))
 @pytest.mark.parametrize("cubic", [False, True])
 def test_stage_velocity_advection(name, cubic):

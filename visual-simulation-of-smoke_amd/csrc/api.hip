@@ -868,6 +868,13 @@ SmokeSimulationResult smoke_simulation_ext_set_launches(SmokeSimulationContext c
     });
 }
 
+SmokeSimulationResult smoke_simulation_ext_set_force_pass(SmokeSimulationContext context, int32_t fused) {
+    return smoke::guarded_call(context, [&](Context& c) {
+        c.fused_force = fused != 0;
+        smoke::rebuild_graph(c);
+    });
+}
+
 SmokeSimulationResult smoke_simulation_ext_write_training_crops(SmokeSimulationContext context, const SmokeSimulationTrainingCrops* crops) {
     if (!crops) return SMOKE_SIMULATION_RESULT_BACKEND_FAILURE;
     return smoke::guarded_call(context, [&](Context& c) { smoke::write_training_crops(c, *crops); });

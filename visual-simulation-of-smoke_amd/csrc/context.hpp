@@ -161,6 +161,9 @@ struct Context {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool diagnostics_dirty = true;
+    // force pass form: the fused kernel (omega staged in LDS) or vorticity_kernel
+    // then cell_force_kernel (omega arrays in HBM); the same bits either way
+    bool fused_force = true;
 
     // launch shapes of the tunable kernels: indices into the candidate tables
     // of pcg.hip (stencil passes 2 and 3) and step_kernels.hip (advection of

@@ -106,9 +106,10 @@ struct time_step_k {
 };
 
 // arrays of the grid (faces included) are indexable in 32 bits
-__device__ __forceinline__ bool index32(const Grid& g) {
+__host__ __device__ __forceinline__ bool index32(const Grid& g) {
     return (int64_t(g.n[0]) + 1) * (int64_t(g.n[1]) + 1) * (int64_t(g.n[2]) + 1) < (int64_t(1) << 31);
 }
+inline bool index32_host(const Grid& g) { return index32(g); }
 
 // distance between neighbouring cells along axis C in the cell arrays
 template <int C>
@@ -137,6 +138,76 @@ __device__ __forceinline__ bool xcd_cell(const VBlock& vb, const uint3& tiles, i
 }
 
 // ------------------------------------------------------------ forces (stage 0)
+// omega from the centred differences of the cell-centred velocity
+// (B:870-900); the arguments are centre components at the six neighbours.
+// Both the two-pass force stage and the fused one evaluate omega, |omega|
+// and the cell force through these helpers, so they produce the same bits.
+struct Omega {
+    float x, y, z;
+};
+
+// half_inv_h = 0.5 / h: the centred difference (a - b) / (2 h) as one
+// multiplication (exactly the quotient when h is a power of two)
+__device__ __forceinline__ Omega omega_from(float w_yp, float w_ym, float v_zp, float v_zm, float u_zp, float u_zm, float w_xp, float w_xm,
+                                            float v_xp, float v_xm, float u_yp, float u_ym, float half_inv_h) {
+    const float dvz_dy = (w_yp - w_ym) * half_inv_h;
+    const float dvy_dz = (v_zp - v_zm) * half_inv_h;
+    const float dvx_dz = (u_zp - u_zm) * half_inv_h;
+    const float dvz_dx = (w_xp - w_xm) * half_inv_h;
+    const float dvy_dx = (v_xp - v_xm) * half_inv_h;
+    const float dvx_dy = (u_yp - u_ym) * half_inv_h;
+    return Omega{dvz_dy - dvy_dz, dvx_dz - dvz_dx, dvy_dx - dvx_dy};
+}
+
+// |omega| with its fused multiply-adds written out, so that the rounding
+// does not depend on how the compiler contracts the sum of squares
+__device__ __forceinline__ float omega_norm(const Omega& o) { return sqrtf(fmaf(o.z, o.z, fmaf(o.x, o.x, o.y * o.y))); }
+
+// centre value of component comp at cell (xx, yy, zz): a domain cell's is the
+// face average of center_velocity (32-bit indices when `fast`), the ghost
+// rules only for cells outside the domain
+__device__ __forceinline__ float centre_component(const Grid& g, bool fast, const float* __restrict__ f, int comp, int xx, int yy, int zz) {
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    if (fast && xx >= 0 && yy >= 0 && zz >= 0 && xx < n0 && yy < n1 && zz < n2) {
+        const int d0 = n0 + (comp == 0), d1 = n1 + (comp == 1);
+        const int lo = (zz * d1 + yy) * d0 + xx;
+        const int hi = lo + (comp == 0 ? 1 : (comp == 1 ? d0 : d0 * d1));
+        return 0.5f * (f[lo] + f[hi]);
+    }
+    return load_center_component(g, f, comp, xx, yy, zz);
+}
+
+// omega at the fluid domain cell (x, y, z), every input read from the face arrays
+__device__ __forceinline__ Omega omega_at(const Grid& g, bool fast, const float* __restrict__ u, const float* __restrict__ v,
+                                          const float* __restrict__ w, int x, int y, int z) {
+    return omega_from(centre_component(g, fast, w, 2, x, y + 1, z), centre_component(g, fast, w, 2, x, y - 1, z),
+                      centre_component(g, fast, v, 1, x, y, z + 1), centre_component(g, fast, v, 1, x, y, z - 1),
+                      centre_component(g, fast, u, 0, x, y, z + 1), centre_component(g, fast, u, 0, x, y, z - 1),
+                      centre_component(g, fast, w, 2, x + 1, y, z), centre_component(g, fast, w, 2, x - 1, y, z),
+                      centre_component(g, fast, v, 1, x + 1, y, z), centre_component(g, fast, v, 1, x - 1, y, z),
+                      centre_component(g, fast, u, 0, x, y + 1, z), centre_component(g, fast, u, 0, x, y - 1, z), 0.5f / g.h);
+}
+
+// B:902-948 for a fluid cell: buoyancy (+y) and vorticity confinement added
+// to the user force (sx, sy, sz); om_* are |omega| at the six neighbours
+// (ghost rules applied), o is omega at the cell
+__device__ __forceinline__ void fluid_cell_force(float& sx, float& sy, float& sz, float a, float b, float tamb, float eps, float h, float rho, float T,
+                                                 float om_xp, float om_xm, float om_yp, float om_ym, float om_zp, float om_zm, const Omega& o) {
+    sy += -a * rho + b * (T - tamb);
+    const float half_inv_h = 0.5f / h;
+    const float gx = (om_xp - om_xm) * half_inv_h;
+    const float gy = (om_yp - om_ym) * half_inv_h;
+    const float gz = (om_zp - om_zm) * half_inv_h;
+    const float gm = sqrtf(gx * gx + gy * gy + gz * gz);
+    if (!(gm < 1.0e-6f)) {
+        const float inv = 1.0f / gm;
+        const float nx = gx * inv, ny = gy * inv, nz = gz * inv;
+        sx += eps * h * (ny * o.z - nz * o.y);
+        sy += eps * h * (nz * o.x - nx * o.z);
+        sz += eps * h * (nx * o.y - ny * o.x);
+    }
+}
+
 // B:2001-2003: solid temperature, then omega and |omega| from the
 // cell-centred velocity with the flow ghost rules (B:870-900).
 __device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
@@ -151,35 +222,11 @@ __device__ __forceinline__ void vorticity_body(const VBlock& vb, Grid g, const f
         ox[i] = oy[i] = oz[i] = om[i] = 0.0f;
         return;
     }
-    const float h = g.h;
-    // centre value of component comp at cell (xx, yy, zz): a domain cell's is
-    // the face average of center_velocity (32-bit indices), the ghost rules
-    // only for the neighbours outside the domain (edge cells' waves take the
-    // slow path for those neighbours only)
-    const bool fast = index32(g);
-    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    auto cc = [&](const float* __restrict__ f, int comp, int xx, int yy, int zz) {
-        if (fast && xx >= 0 && yy >= 0 && zz >= 0 && xx < n0 && yy < n1 && zz < n2) {
-            const int d0 = n0 + (comp == 0), d1 = n1 + (comp == 1);
-            const int lo = (zz * d1 + yy) * d0 + xx;
-            const int hi = lo + (comp == 0 ? 1 : (comp == 1 ? d0 : d0 * d1));
-            return 0.5f * (f[lo] + f[hi]);
-        }
-        return load_center_component(g, f, comp, xx, yy, zz);
-    };
-    const float dvz_dy = 0.5f * (cc(w, 2, x, y + 1, z) - cc(w, 2, x, y - 1, z)) / h;
-    const float dvy_dz = 0.5f * (cc(v, 1, x, y, z + 1) - cc(v, 1, x, y, z - 1)) / h;
-    const float dvx_dz = 0.5f * (cc(u, 0, x, y, z + 1) - cc(u, 0, x, y, z - 1)) / h;
-    const float dvz_dx = 0.5f * (cc(w, 2, x + 1, y, z) - cc(w, 2, x - 1, y, z)) / h;
-    const float dvy_dx = 0.5f * (cc(v, 1, x + 1, y, z) - cc(v, 1, x - 1, y, z)) / h;
-    const float dvx_dy = 0.5f * (cc(u, 0, x, y + 1, z) - cc(u, 0, x, y - 1, z)) / h;
-    const float wx = dvz_dy - dvy_dz, wy = dvx_dz - dvz_dx, wz = dvy_dx - dvx_dy;
-    ox[i] = wx;
-    oy[i] = wy;
-    oz[i] = wz;
-    // the sum of squares with its fused multiply-adds written out, so that
-    // the rounding does not depend on how the compiler contracts it
-    om[i] = sqrtf(fmaf(wz, wz, fmaf(wx, wx, wy * wy)));
+    const Omega o = omega_at(g, index32(g), u, v, w, x, y, z);
+    ox[i] = o.x;
+    oy[i] = o.y;
+    oz[i] = o.z;
+    om[i] = omega_norm(o);
 }
 
 __global__ void vorticity_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
@@ -203,8 +250,6 @@ __device__ __forceinline__ void cell_force_body(const VBlock& vb, Grid g, const 
     const int64_t i = cidx(g, x, y, z);
     float sx = fux[i], sy = fuy[i], sz = fuz[i];
     if (occ[i] == 0) {
-        sy += -a * rho[i] + b * (T[i] - tamb);
-        const float h = g.h;
         // |omega| at a neighbour: a plain load for domain cells (32-bit
         // indices), the ghost rule only for the neighbours outside the domain
         const bool fast = index32(g);
@@ -212,18 +257,8 @@ __device__ __forceinline__ void cell_force_body(const VBlock& vb, Grid g, const 
             if (fast && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.n[0] && yy < g.n[1] && zz < g.n[2]) return om[(zz * g.n[1] + yy) * g.n[0] + xx];
             return load_flow_cell(g, om, xx, yy, zz);
         };
-        const float gx = 0.5f * (om_at(x + 1, y, z) - om_at(x - 1, y, z)) / h;
-        const float gy = 0.5f * (om_at(x, y + 1, z) - om_at(x, y - 1, z)) / h;
-        const float gz = 0.5f * (om_at(x, y, z + 1) - om_at(x, y, z - 1)) / h;
-        const float gm = sqrtf(gx * gx + gy * gy + gz * gz);
-        if (!(gm < 1.0e-6f)) {
-            const float inv = 1.0f / gm;
-            const float nx = gx * inv, ny = gy * inv, nz = gz * inv;
-            const float wx = ox[i], wy = oy[i], wz = oz[i];
-            sx += eps * h * (ny * wz - nz * wy);
-            sy += eps * h * (nz * wx - nx * wz);
-            sz += eps * h * (nx * wy - ny * wx);
-        }
+        fluid_cell_force(sx, sy, sz, a, b, tamb, eps, g.h, rho[i], T[i], om_at(x + 1, y, z), om_at(x - 1, y, z), om_at(x, y + 1, z),
+                         om_at(x, y - 1, z), om_at(x, y, z + 1), om_at(x, y, z - 1), Omega{ox[i], oy[i], oz[i]});
     }
     fx[i] = sx;
     fy[i] = sy;
@@ -240,6 +275,250 @@ struct cell_force_k {
     template <class... A>
     __device__ __forceinline__ static void run(const VBlock& vb, A... a) { cell_force_body(vb, a...); }
 };
+
+// ---- fused force pass: omega, |omega| and the cell force in one kernel,
+// with no omega arrays in HBM.  A block owns a 64 x 4 column of cells and
+// marches through kFfPlanes z planes of it.  Per plane it stages the
+// cell-centred velocity of the plane two ahead on a halo of 2 (ghost rules
+// for cells outside the domain) in LDS, computes omega and |omega| of the
+// plane one ahead on a halo of 1 from those, and then the cell force of the
+// current plane, whose |omega| neighbours are all in LDS.  Rings of three
+// planes hold both.  Same helpers, so the same bits as vorticity_kernel
+// followed by cell_force_kernel; the omega views are recomputed lazily.
+constexpr int kFfTx = 64, kFfTy = 4, kFfPlanes = 16;
+constexpr int kFfCx = kFfTx + 4, kFfCy = kFfTy + 4, kFfCn = kFfCx * kFfCy;   // centre-velocity plane, halo 2
+constexpr int kFfOx = kFfTx + 2, kFfOy = kFfTy + 2, kFfOn = kFfOx * kFfOy;   // omega plane, halo 1
+
+inline uint3 force_tiles(const Grid& g) {
+    return make_uint3(unsigned((g.n[0] + kFfTx - 1) / kFfTx), unsigned((g.n[1] + kFfTy - 1) / kFfTy), unsigned((g.n[2] + kFfPlanes - 1) / kFfPlanes));
+}
+
+template <bool I32>
+__global__ __launch_bounds__(kFfTx * kFfTy) void force_fused_kernel(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                                                                     const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                                     const float* __restrict__ fux, const float* __restrict__ fuy,
+                                                                     const float* __restrict__ fuz, const float* __restrict__ rho,
+                                                                     float* __restrict__ T, const float* __restrict__ solid_temperature, float a,
+                                                                     float b, float tamb, float eps, float* __restrict__ fx, float* __restrict__ fy,
+                                                                     float* __restrict__ fz, uint3 tiles) {
+    using Index = typename std::conditional<I32, int, int64_t>::type;
+    constexpr int kThreads = kFfTx * kFfTy;
+    constexpr int kPer = (kFfCn + kThreads - 1) / kThreads;   // staged centre cells per thread
+    __shared__ float cv[3][3][kFfCn];   // [plane ring][component][cell]
+    __shared__ float om[3][4][kFfOn];   // [plane ring][wx, wy, wz, |w|][cell]
+    uint3 tile;
+    unsigned id;
+    if (!xcd_tile(tiles, tile, id)) return;   // the whole block
+    const int t = int(threadIdx.x);
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const int x0 = int(tile.x) * kFfTx, y0 = int(tile.y) * kFfTy, z0 = int(tile.z) * kFfPlanes;
+    const int zend = min(z0 + kFfPlanes, n2);
+    const float h = g.h;
+    const Index d0u = n0 + 1, d1v = n1 + 1;
+    const Index su = Index(d0u) * n1, sv = Index(n0) * d1v, sw = Index(n0) * n1;   // z strides of the face arrays
+    auto ring = [](int z) { return (z + 3) % 3; };   // z >= -3
+    // A staged position off a periodic axis stands for its periodic image
+    // (the ghost rule of that axis is the wrap), so the halo values there are
+    // the far side's.
+    auto wrap = [&](int q, int k) { return g.flow_periodic[k] ? wrapi(q, g.n[k]) : q; };
+
+    const float half_inv_h = 0.5f / h;
+
+    // Centre velocity of plane zz on the halo of 2, in two halves.  fetch
+    // issues the two face loads per component of every staged position into
+    // a register set, with the coordinates clamped into the domain (always in
+    // bounds), two planes before they are stored.  store averages them -- the
+    // centre value of a domain position, and the interior value the ghost
+    // rule of a position one cell off a non-periodic side starts from (the
+    // only off-domain positions an omega of a domain cell or of a periodic
+    // image reads; 0 further out).  A thread's staged positions are the same
+    // in every plane: their x, y parts are worked out once.
+    int qx[kPer], qy[kPer], off_u[kPer], off_vw[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = min(t + k * kThreads, kFfCn - 1);
+        qx[k] = wrap(x0 - 2 + i % kFfCx, 0);
+        qy[k] = wrap(y0 - 2 + i / kFfCx, 1);
+        const int xc = clampi(qx[k], 0, n0 - 1), yc = clampi(qy[k], 0, n1 - 1);
+        off_u[k] = yc * (n0 + 1) + xc;
+        off_vw[k] = yc * n0 + xc;
+    }
+    struct Fetch {
+        float lo[3][kPer], hi[3][kPer];
+    };
+    auto fetch_centre = [&](int zz, Fetch& f) {
+        const Index zc = clampi(wrap(zz, 2), 0, n2 - 1);
+        const float* __restrict__ pu = u + zc * su;
+        const float* __restrict__ pv = v + zc * sv;
+        const float* __restrict__ pw = w + zc * sw;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            f.lo[0][k] = pu[off_u[k]];
+            f.hi[0][k] = pu[off_u[k] + 1];
+            f.lo[1][k] = pv[off_vw[k]];
+            f.hi[1][k] = pv[off_vw[k] + n0];
+            f.lo[2][k] = pw[off_vw[k]];
+            f.hi[2][k] = pw[off_vw[k] + sw];
+        }
+    };
+    auto store_centre = [&](int zz, const Fetch& f) {
+        const int s = ring(zz);
+        const int zw = wrap(zz, 2);
+        const bool offz = zw < 0 || zw >= n2;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = t + k * kThreads;
+            if (i >= kFfCn) break;
+            float cc[3];
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) cc[comp] = 0.5f * (f.lo[comp][k] + f.hi[comp][k]);
+            const bool offx = qx[k] < 0 || qx[k] >= n0, offy = qy[k] < 0 || qy[k] >= n1;
+            if (offx || offy || offz) {
+                // load_center_component's rule at its first off-domain axis
+                // (the others clamped, as fetched)
+                const bool near = qx[k] >= -1 && qx[k] <= n0 && qy[k] >= -1 && qy[k] <= n1 && zw >= -1 && zw <= n2;
+                const int ax = offx ? 0 : (offy ? 1 : 2);
+                const int face = 2 * ax + ((offx ? qx[k] : (offy ? qy[k] : zw)) < 0 ? 0 : 1);
+                // the face's kind and wall velocity read with constant indices
+                // (a computed index into the kernel's Grid argument would
+                // put the whole struct in scratch memory)
+#pragma unroll
+                for (int comp = 0; comp < 3; ++comp) {
+                    float vv = cc[comp];
+#pragma unroll
+                    for (int f6 = 0; f6 < 6; ++f6) {
+                        if (face != f6) continue;
+                        const uint32_t type = g.flow_type[f6];
+                        if (!(type == kOutflow || (type == kFreeSlip && comp != (f6 >> 1)))) vv = 2.0f * g.flow_velocity[f6][comp] - vv;
+                    }
+                    cc[comp] = near ? vv : 0.0f;
+                }
+            }
+            cv[s][0][i] = cc[0];
+            cv[s][1][i] = cc[1];
+            cv[s][2][i] = cc[2];
+        }
+    };
+    // this thread's omega positions: LDS index of the position in a centre
+    // plane and the xy offset of its cell (-1 off the domain)
+    constexpr int kOmPer = (kFfOn + kThreads - 1) / kThreads;
+    int om_c[kOmPer], om_off[kOmPer];
+#pragma unroll
+    for (int k = 0; k < kOmPer; ++k) {
+        const int i = t + k * kThreads;
+        const int ox = i % kFfOx, oy = i / kFfOx;
+        const int xw = wrap(x0 - 1 + ox, 0), yw = wrap(y0 - 1 + oy, 1);
+        om_c[k] = (oy + 1) * kFfCx + ox + 1;
+        om_off[k] = (i < kFfOn && xw >= 0 && xw < n0 && yw >= 0 && yw < n1) ? yw * n0 + xw : -1;
+    }
+    // occupancy of this thread's omega positions of plane zz (1 off the
+    // domain or past the plane), fetched an iteration before stage_omega
+    auto fetch_occ = [&](int zz, uint8_t (&o)[kOmPer]) {
+        const int zw = wrap(zz, 2);
+        const bool inz = zw >= 0 && zw < n2;
+        const uint8_t* __restrict__ po = occ + Index(inz ? zw : 0) * sw;
+#pragma unroll
+        for (int k = 0; k < kOmPer; ++k) o[k] = (inz && om_off[k] >= 0) ? po[om_off[k]] : uint8_t(1);
+    };
+    // omega and |omega| of plane zz on the halo of 1, at the fluid domain
+    // cells and periodic images (0 elsewhere, as vorticity_kernel writes in
+    // solid cells); centre planes zz - 1, zz, zz + 1 staged
+    auto stage_omega = [&](int zz, const uint8_t (&o)[kOmPer]) {
+        const int s = ring(zz), sm = ring(zz - 1), sp = ring(zz + 1);
+#pragma unroll
+        for (int k = 0; k < kOmPer; ++k) {
+            const int i = t + k * kThreads;
+            if (i >= kFfOn) break;
+            Omega w3{0.0f, 0.0f, 0.0f};
+            float m = 0.0f;
+            if (o[k] == 0) {
+                const int c = om_c[k];
+                w3 = omega_from(cv[s][2][c + kFfCx], cv[s][2][c - kFfCx], cv[sp][1][c], cv[sm][1][c], cv[sp][0][c], cv[sm][0][c], cv[s][2][c + 1],
+                                cv[s][2][c - 1], cv[s][1][c + 1], cv[s][1][c - 1], cv[s][0][c + kFfCx], cv[s][0][c - kFfCx], half_inv_h);
+                m = omega_norm(w3);
+            }
+            om[s][0][i] = w3.x;
+            om[s][1][i] = w3.y;
+            om[s][2][i] = w3.z;
+            om[s][3][i] = m;
+        }
+    };
+
+    // prologue: centre planes z0 - 2 .. z0 + 2, omega planes z0 - 1, z0;
+    // plane z0 + 3 fetched, the occupancy of omega plane z0 + 1 fetched
+    Fetch fa, fb;
+    uint8_t occ_a[kOmPer], occ_b[kOmPer];
+    fetch_centre(z0 - 2, fa);
+    fetch_centre(z0 - 1, fb);
+    fetch_occ(z0 - 1, occ_a);
+    fetch_occ(z0, occ_b);
+    store_centre(z0 - 2, fa);
+    store_centre(z0 - 1, fb);
+    fetch_centre(z0, fa);
+    fetch_centre(z0 + 1, fb);
+    store_centre(z0, fa);
+    __syncthreads();
+    stage_omega(z0 - 1, occ_a);
+    __syncthreads();
+    store_centre(z0 + 1, fb);   // over plane z0 - 2, last read above
+    fetch_centre(z0 + 2, fa);
+    fetch_occ(z0 + 1, occ_a);
+    __syncthreads();
+    stage_omega(z0, occ_b);
+    __syncthreads();
+    store_centre(z0 + 2, fa);   // over plane z0 - 1, last read above
+    fetch_centre(z0 + 3, fa);
+
+    const int tx = t % kFfTx, ty = t / kFfTx;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool mine = x < n0 && y < n1;
+    const int c = (ty + 1) * kFfOx + tx + 1;   // this thread's cell in an omega plane
+    // neighbours off a non-periodic side of the domain
+    const bool off_xm = !g.flow_periodic[0] && x == 0, off_xp = !g.flow_periodic[0] && x == n0 - 1;
+    const bool off_ym = !g.flow_periodic[1] && y == 0, off_yp = !g.flow_periodic[1] && y == n1 - 1;
+    // one output plane: centre planes z, z + 1, z + 2 and omega planes z - 1,
+    // z staged; `cur` holds centre plane z + 3, `ocur` the occupancy of
+    // omega plane z + 1; the next ones are fetched into `next` and `onext`
+    auto plane = [&](int z, Fetch& cur, Fetch& next, uint8_t (&ocur)[kOmPer], uint8_t (&onext)[kOmPer]) {
+        fetch_centre(z + 4, next);
+        fetch_occ(z + 2, onext);
+        const Index i = (Index(z) * n1 + (mine ? y : 0)) * n0 + (mine ? x : 0);
+        const float ux = fux[i], uy = fuy[i], uz = fuz[i], ri = rho[i], Ti = T[i];
+        const uint8_t oi = occ[i];
+        __syncthreads();
+        stage_omega(z + 1, ocur);   // over omega(z - 2), last read by plane z - 1
+        __syncthreads();
+        if (!mine) {
+            store_centre(z + 3, cur);
+            return;
+        }
+        {
+            float sx = ux, sy = uy, sz = uz;
+            if (oi == 0) {
+                const int s = ring(z);
+                // |omega| of a neighbour (load_flow_cell's rule): staged in
+                // LDS for domain cells and periodic images; off a
+                // non-periodic side the clamp gives the cell itself
+                const float self = om[s][3][c];
+                const bool off_zm = !g.flow_periodic[2] && z == 0, off_zp = !g.flow_periodic[2] && z == n2 - 1;
+                fluid_cell_force(sx, sy, sz, a, b, tamb, eps, h, ri, Ti, off_xp ? self : om[s][3][c + 1], off_xm ? self : om[s][3][c - 1],
+                                 off_yp ? self : om[s][3][c + kFfOx], off_ym ? self : om[s][3][c - kFfOx], off_zp ? self : om[ring(z + 1)][3][c],
+                                 off_zm ? self : om[ring(z - 1)][3][c], Omega{om[s][0][c], om[s][1][c], om[s][2][c]});
+            } else {
+                T[i] = solid_temperature[i];
+            }
+            fx[i] = sx;
+            fy[i] = sy;
+            fz[i] = sz;
+        }
+        store_centre(z + 3, cur);   // over centre plane z, last read by stage_omega(z + 1)
+    };
+    // two planes per trip, so the register sets swap roles without copies
+    for (int z = z0; z < zend; z += 2) {
+        plane(z, fa, fb, occ_a, occ_b);
+        if (z + 1 < zend) plane(z + 1, fb, fa, occ_b, occ_a);
+    }
+}
 
 // B:950-1005 + solid part of B:1014-1168: u_face += dt * mean of the
 // in-domain neighbour cell forces (no wrap), then the solid override.
@@ -1382,14 +1661,24 @@ void enqueue_time_step(Context& c) {
 
 void enqueue_forces(Context& c) {
     const Grid& g = c.grid;
-    const uint3 tiles = cell_tiles(g);
-    vorticity_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0],
-                                                                         c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.temperature,
-                                                                         c.solid_temperature, tiles);
-    cell_force_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(
-        g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0], c.vorticity[1], c.vorticity[2],
-        c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor, c.config.ambient_temperature,
-        c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2], tiles);
+    if (c.fused_force) {
+        const uint3 tiles = force_tiles(g);
+        auto kernel = index32_host(g) ? force_fused_kernel<true> : force_fused_kernel<false>;
+        kernel<<<xcd_blocks(dim3(tiles.x, tiles.y, tiles.z)), kFfTx * kFfTy, 0, c.stream>>>(
+            g, c.face[0], c.face[1], c.face[2], c.occupancy, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature,
+            c.solid_temperature, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor, c.config.ambient_temperature,
+            c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2], tiles);
+    } else {
+        const uint3 tiles = cell_tiles(g);
+        vorticity_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(g, c.face[0], c.face[1], c.face[2], c.occupancy, c.vorticity[0],
+                                                                             c.vorticity[1], c.vorticity[2], c.vorticity_magnitude, c.temperature,
+                                                                             c.solid_temperature, tiles);
+        cell_force_kernel<<<cell_tile_blocks(g), kCellBlock, 0, c.stream>>>(
+            g, c.force_user[0], c.force_user[1], c.force_user[2], c.density, c.temperature, c.occupancy, c.vorticity[0], c.vorticity[1],
+            c.vorticity[2], c.vorticity_magnitude, c.config.buoyancy_density_factor, c.config.buoyancy_temperature_factor,
+            c.config.ambient_temperature, c.config.vorticity_confinement, c.cell_force[0], c.cell_force[1], c.cell_force[2], tiles);
+    }
+    c.diagnostics_dirty = true;   // the velocity changes; the views are recomputed when read
     launch_face_force<0>(c);
     launch_face_force<1>(c);
     launch_face_force<2>(c);

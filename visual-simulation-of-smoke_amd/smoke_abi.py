@@ -317,6 +317,7 @@ def load_library(path: Path | str | None = None):
         "smoke_simulation_ext_reset_launches": [C.c_void_p],
         "smoke_simulation_ext_get_launches": [C.c_void_p, C.POINTER(LaunchReport)],
         "smoke_simulation_ext_set_launches": [C.c_void_p, C.POINTER(LaunchReport)],
+        "smoke_simulation_ext_set_force_pass": [C.c_void_p, C.c_int32],
         "smoke_simulation_ext_write_training_crops": [C.c_void_p, C.POINTER(TrainingCrops)],
         "smoke_simulation_ext_copy_training_crops": [C.c_void_p, C.POINTER(DeviceCrops)],
         "smoke_simulation_ext_set_learned_preconditioner": [C.c_void_p, C.c_void_p],
@@ -531,6 +532,11 @@ class SmokeContext:
     def set_launches(self, report: LaunchReport):
         """Applies the launch shapes of `report` (see launches())."""
         check(self.lib.smoke_simulation_ext_set_launches(self.handle, C.byref(report)), "set_launches")
+
+    def set_force_pass(self, fused: bool):
+        """Force pass as the fused LDS kernel (default) or as the vorticity and
+        cell-force kernels with omega arrays in HBM; the same bits."""
+        check(self.lib.smoke_simulation_ext_set_force_pass(self.handle, 1 if fused else 0), "set_force_pass")
 
     def set_multilevel_options(self, **kw):
         o = MultilevelOptions()
