@@ -1270,36 +1270,67 @@ void run_tolerance_pcg(Context& c) {
     // by less than 10 % over 5 iterations or turned non-finite.
     // Up to two iterations stay in flight: iteration k's scalars are copied
     // to a pinned slot and signalled by an event, and the host judges
-    // iteration k while iteration k + 1 runs.  An iteration enqueued after
-    // the solve finished changes nothing (its update kernels skip on the done
-    // flag and its preconditioner application only rewrites z) but costs a
-    // whole iteration of device time, so when the last residual ratio
-    // predicts that the iteration in flight ends the solve, the next one is
-    // held back until that iteration has been judged.  A wrong guess costs
-    // one host round trip with the device idle.
+    // iteration k while iteration k + 1 runs.  Each iteration is enqueued
+    // with the preconditioner application that prepares the next one (z =
+    // M(r), r.z, beta), which costs most of an iteration's device time and
+    // is wasted after the iteration that ends the solve.  So an iteration
+    // predicted to end the solve (r.r falling by the last ratio again for
+    // each iteration ahead comes within a factor 4 of the stop threshold, or
+    // the iteration cap) is enqueued without its application and with
+    // nothing after it; if the judgement then finds it did not end the
+    // solve, its application is enqueued at that point (one host round trip
+    // with the device idle).  Every iteration's device work is unchanged
+    // (same iterations, same bits); only applications whose z is never used
+    // are left out.
     auto learned_inner = [&]() -> bool {
         learned_apply(c);
         learned_rz(c, true);
         std::vector<double> rr{host.rr};
-        int pending[2] = {0, 0}, in_flight = 0, next_slot = 0;
-        auto enqueue = [&]() {
+        struct Pending {
+            int slot;
+            bool applied;
+        };
+        Pending pending[2] = {};
+        int in_flight = 0, next_slot = 0;
+        auto enqueue = [&](bool apply) {
             const int s = next_slot;
             next_slot ^= 1;
             learned_iteration(c);
             hip_check(hipMemcpyAsync(&c.host_scalars[s], c.scalars, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream), "scalars copy");
             hip_check(hipEventRecord(c.status_event[s], c.stream), "status event");
-            learned_apply(c);
-            learned_rz(c, false);
-            pending[in_flight++] = s;
+            if (apply) {
+                learned_apply(c);
+                learned_rz(c, false);
+            }
+            pending[in_flight++] = Pending{s, apply};
         };
-        enqueue();
-        enqueue();
+        // is the iteration `ahead` iterations after the last judged one
+        // predicted to reach the stop test (or the iteration cap)?
+        auto predicted_end = [&](int ahead) {
+            if (host.iteration + ahead >= host.max_iterations) return true;
+            const size_t k = rr.size() - 1;
+            if (k < 1 || !(rr[k - 1] > 0.0)) return false;
+            const double ratio = rr[k] / rr[k - 1];
+            if (!(ratio < 1.0)) return false;
+            double next = rr[k];
+            for (int q = 0; q < ahead; ++q) next *= ratio;
+            return next <= 4.0 * host.stop2;
+        };
+        // up to two in flight; nothing after an iteration enqueued without
+        // its application or predicted to end the solve
+        auto fill = [&]() {
+            while (in_flight < 2) {
+                if (in_flight > 0 && (!pending[in_flight - 1].applied || predicted_end(in_flight))) break;
+                enqueue(!predicted_end(in_flight + 1));
+            }
+        };
+        fill();
         for (;;) {
-            const int slot = pending[0];
+            const Pending judged = pending[0];
             pending[0] = pending[1];
             --in_flight;
-            hip_check(hipEventSynchronize(c.status_event[slot]), "status wait");
-            host = c.host_scalars[slot];
+            hip_check(hipEventSynchronize(c.status_event[judged.slot]), "status wait");
+            host = c.host_scalars[judged.slot];
             ++learned_iterations;
             if (!std::isfinite(host.rr) || !std::isfinite(host.rz)) return false;
             rr.push_back(host.rr);
@@ -1318,14 +1349,12 @@ void run_tolerance_pcg(Context& c) {
                 const double before = *std::min_element(rr.begin(), rr.end() - 10);
                 if (recent > 0.81 * before) return false;
             }
-            // iterations to have in flight after iteration k: just k + 1 when
-            // it is predicted to reach the stop test (r.r falling by the last
-            // ratio again; ratios vary, so within a factor 4 of the
-            // threshold) or the iteration cap
-            const double ratio = rr[k - 1] > 0.0 ? rr[k] / rr[k - 1] : 1.0;
-            const bool next_ends = (ratio < 1.0 && rr[k] * ratio <= 4.0 * host.stop2) || host.iteration + 1 >= host.max_iterations;
-            const int target = next_ends ? 1 : 2;
-            while (in_flight < target) enqueue();
+            if (!judged.applied) {
+                // predicted to end the solve, and it did not
+                learned_apply(c);
+                learned_rz(c, false);
+            }
+            fill();
         }
     };
     while (!host.done) {
