@@ -108,6 +108,9 @@ void destroy_resources(Context& c) {
     c.host_status = nullptr;
     if (c.host_scalars) hipHostFree(c.host_scalars);
     c.host_scalars = nullptr;
+    if (c.host_step) hipHostFree(c.host_step);
+    c.host_step = nullptr;
+    if (c.step_event) hipEventDestroy(c.step_event), c.step_event = nullptr;
     for (hipEvent_t& e : c.status_event)
         if (e) hipEventDestroy(e), e = nullptr;
     for (auto& row : c.timing_events)
@@ -445,6 +448,10 @@ SmokeSimulationResult smoke_simulation_create_context_cuda(const SmokeSimulation
         hip_check(hipHostMalloc(&pinned, 2 * sizeof(smoke::PcgScalars), hipHostMallocDefault), "hipHostMalloc");
         c.host_scalars = static_cast<smoke::PcgScalars*>(pinned);
         for (hipEvent_t& e : c.status_event) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        pinned = nullptr;
+        hip_check(hipHostMalloc(&pinned, 2 * sizeof(float), hipHostMallocDefault), "hipHostMalloc");
+        c.host_step = static_cast<float*>(pinned);
+        hip_check(hipEventCreateWithFlags(&c.step_event, hipEventDisableTiming), "event");
         for (auto& row : c.timing_events)
             for (hipEvent_t& e : row) hip_check(hipEventCreate(&e), "event");
 
@@ -650,11 +657,20 @@ SmokeSimulationResult smoke_simulation_ext_get_solver_stats(SmokeSimulationConte
             c.stats.refinements = 0;
         }
         float dt = 0.f, speed = 0.f;
-        int anchor = 0;
-        hip_check(hipMemcpyAsync(&dt, c.dt_device, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read dt");
-        hip_check(hipMemcpyAsync(&speed, c.max_speed, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read speed");
-        hip_check(hipMemcpyAsync(&anchor, c.anchor, sizeof(int), hipMemcpyDeviceToHost, c.stream), "read anchor");
-        hip_check(hipStreamSynchronize(c.stream), "sync");
+        int anchor = c.anchor_host;
+        if (c.step_info_pending && anchor >= 0) {
+            // the copies made right after the last step's time-step kernel:
+            // no wait for the rest of the step
+            hip_check(hipEventSynchronize(c.step_event), "step info");
+            dt = c.host_step[0];
+            speed = c.host_step[1];
+        } else {
+            hip_check(hipMemcpyAsync(&dt, c.dt_device, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read dt");
+            hip_check(hipMemcpyAsync(&speed, c.max_speed, sizeof(float), hipMemcpyDeviceToHost, c.stream), "read speed");
+            hip_check(hipMemcpyAsync(&anchor, c.anchor, sizeof(int), hipMemcpyDeviceToHost, c.stream), "read anchor");
+            hip_check(hipStreamSynchronize(c.stream), "sync");
+            c.anchor_host = anchor;
+        }
         c.stats.dt = dt;
         c.stats.max_face_speed = speed;
         c.stats.anchor = anchor;

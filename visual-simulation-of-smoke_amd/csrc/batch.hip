@@ -464,6 +464,7 @@ void fused_step(SmokeSimulationBatch_t& b, const SmokeSimulationPlumeSource* sou
     for (Context* c : F.ctx) {
         c->stats.steps += 1;
         c->diagnostics_dirty = true;
+        c->step_info_pending = false;   // this step's dt was not copied out: the stats read it from the device
     }
     hip_check(hipEventRecord(b.solved, s), "batch event");
     for (hipStream_t lane : b.lanes) hip_check(hipStreamWaitEvent(lane, b.solved, 0), "lane wait");

@@ -150,6 +150,13 @@ struct Context {
     PcgStatus* host_status = nullptr;   // pinned, two slots
     PcgScalars* host_scalars = nullptr; // pinned, two slots (flexible PCG: whole scalar block per iteration)
     hipEvent_t status_event[2] = {};
+    // tolerance mode: dt and max |u_face| of the step, copied to pinned memory
+    // behind step_event right after the time-step kernel, so that the solver
+    // stats need not drain the stream; the anchor read once per occupancy
+    float* host_step = nullptr;         // pinned: dt, max |u_face|
+    hipEvent_t step_event = nullptr;
+    bool step_info_pending = false;
+    int anchor_host = -1;               // -1: not read since the occupancy last changed
 
     // time step
     float* dt_device = nullptr;          // dt used by the current step

@@ -1634,6 +1634,7 @@ void enqueue_fill_u8(Context& c, uint8_t* p, uint8_t value, int64_t n) {
 }
 
 void enqueue_occupancy_derived(Context& c) {
+    c.anchor_host = -1;                           // the anchor is recomputed below
     c.solves_done = std::min(c.solves_done, 1);   // the warm-start history predates this occupancy
     c.mg.built = 0;                               // the multilevel hierarchy is rebuilt from the new occupancy
     const Grid& g = c.grid;
@@ -1657,6 +1658,16 @@ void enqueue_time_step(Context& c) {
     time_step_kernel<<<1, 1, 0, c.stream>>>(c.max_speed_bits, c.max_speed, c.dt_device, c.options.adaptive_dt ? 1 : 0, c.options.cfl, g.h,
                                             c.options.dt_max, c.config.dt);
     hip_check(hipGetLastError(), "time_step_kernel");
+    if (c.options.pressure_mode != SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS && c.host_step) {
+        // tolerance mode (never captured into a graph): dt and max |u_face|
+        // for the solver stats, behind their own event
+        hip_check(hipMemcpyAsync(&c.host_step[0], c.dt_device, sizeof(float), hipMemcpyDeviceToHost, c.stream), "step dt");
+        hip_check(hipMemcpyAsync(&c.host_step[1], c.max_speed, sizeof(float), hipMemcpyDeviceToHost, c.stream), "step speed");
+        hip_check(hipEventRecord(c.step_event, c.stream), "step event");
+        c.step_info_pending = true;
+    } else {
+        c.step_info_pending = false;
+    }
 }
 
 void enqueue_forces(Context& c) {
