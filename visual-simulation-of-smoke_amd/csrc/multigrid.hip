@@ -252,14 +252,18 @@ __device__ __forceinline__ void row4_residual(const LevelArgs& a, const float* _
 
 // DOT: also the block's partial sum of r . out (FP64), written to
 // partials[block] -- the r.z of flexible PCG when `out` is its z
-template <bool DOT>
+//
+// LPR lanes per row: a block of 256 threads covers 4 LPR x cells of 256 / LPR
+// rows (64: 256 x 4; 32 for levels at most 128 cells wide, whose rows would
+// leave half of 64 lanes idle: 128 x 8).
+template <bool DOT, int LPR>
 __global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, uint3 tiles, const float* __restrict__ r, const float* __restrict__ xb,
                                                         float* __restrict__ out, double* __restrict__ partials) {
     uint3 tile;
     unsigned id;
     if (!xcd_tile(tiles, tile, id)) return;
-    const int x0 = (int(tile.x) * 64 + (threadIdx.x & 63)) * 4;
-    const int y = int(tile.y) * 4 + (threadIdx.x >> 6);
+    const int x0 = (int(tile.x) * LPR + int(threadIdx.x % LPR)) * 4;
+    const int y = int(tile.y) * (256 / LPR) + int(threadIdx.x / LPR);
     const int z = int(tile.z);
     const bool inside = x0 < a.n[0] && y < a.n[1];
     double dot = 0.0;
@@ -285,9 +289,10 @@ __global__ __launch_bounds__(256) void mg_sweep4_kernel(LevelArgs a, uint3 tiles
     }
 }
 
+template <int LPR>
 __global__ __launch_bounds__(256) void mg_scale4_kernel(LevelArgs a, const float* __restrict__ r, float* __restrict__ out) {
-    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int x0 = (int(blockIdx.x) * LPR + int(threadIdx.x % LPR)) * 4;
+    const int y = int(blockIdx.y) * (256 / LPR) + int(threadIdx.x / LPR);
     const int z = blockIdx.z;
     if (x0 >= a.n[0] || y >= a.n[1]) return;
     const Row4 q = row4_at(a, x0, y, z);
@@ -301,9 +306,10 @@ __global__ __launch_bounds__(256) void mg_scale4_kernel(LevelArgs a, const float
 
 // x += correction of the parent cell on fluid rows; the four cells have the
 // parents X = x0 / 2 and X + 1 (x0 % 4 == 0, so the pair is one aligned float2)
+template <int LPR>
 __global__ __launch_bounds__(256) void mg_prolong4_kernel(LevelArgs a, Coarse cc, float* __restrict__ xb) {
-    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int x0 = (int(blockIdx.x) * LPR + int(threadIdx.x % LPR)) * 4;
+    const int y = int(blockIdx.y) * (256 / LPR) + int(threadIdx.x / LPR);
     const int z = blockIdx.z;
     if (x0 >= a.n[0] || y >= a.n[1]) return;
     const Row4 q = row4_at(a, x0, y, z);
@@ -316,18 +322,19 @@ __global__ __launch_bounds__(256) void mg_prolong4_kernel(LevelArgs a, Coarse cc
     *reinterpret_cast<float4*>(xb + q.i0) = v;
 }
 
-// restriction with float4 rows: a 64 x 4 block covers 256 x cells, 4 rows and
-// two z planes; each thread sums its 2 x 4 cells into two coarse x cells, the
-// y pair is summed through LDS
+// restriction with float4 rows: a block covers 4 LPR x cells, 256 / LPR rows
+// and two z planes; each thread sums its 2 x 4 cells into two coarse x cells,
+// the y pair is summed through LDS
+template <int LPR>
 __global__ __launch_bounds__(256) void mg_restrict4_kernel(LevelArgs a, uint3 tiles, const float* __restrict__ r, const float* __restrict__ xb, int nc0,
                                                            int nc1, const uint8_t* __restrict__ cflags, float* __restrict__ rc) {
-    __shared__ float2 rows[4][64];
+    __shared__ float2 rows[256 / LPR][LPR];
     uint3 tile;
     unsigned id;
     if (!xcd_tile(tiles, tile, id)) return;
-    const int lane = threadIdx.x & 63, rowi = threadIdx.x >> 6;
-    const int x0 = (int(tile.x) * 64 + lane) * 4;
-    const int y = int(tile.y) * 4 + rowi;
+    const int lane = int(threadIdx.x % LPR), rowi = int(threadIdx.x / LPR);
+    const int x0 = (int(tile.x) * LPR + lane) * 4;
+    const int y = int(tile.y) * (256 / LPR) + rowi;
     const int Z = int(tile.z);
     float s0 = 0.0f, s1 = 0.0f;
     if (x0 < a.n[0] && y < a.n[1]) {
@@ -596,6 +603,56 @@ void free_level(T*& p) {
     p = nullptr;
 }
 
+// Launches of the float4-row kernels (levels over 2^20 cells with nx % 8 ==
+// 0): 32 lanes per row when a row has at most 128 cells, else 64.
+struct Vec4Grid {
+    int lpr;
+    dim3 g;    // one z plane per block
+    dim3 gc;   // two z planes per block (restriction)
+};
+
+Vec4Grid vec4_grid(const MgLevel& L) {
+    Vec4Grid v;
+    v.lpr = L.n[0] / 4 <= 32 ? 32 : 64;
+    const unsigned gx = unsigned((L.n[0] / 4 + v.lpr - 1) / v.lpr), gy = unsigned((L.n[1] + 256 / v.lpr - 1) / (256 / v.lpr));
+    v.g = dim3(gx, gy, unsigned(L.n[2]));
+    v.gc = dim3(gx, gy, unsigned((L.n[2] + 1) / 2));
+    return v;
+}
+
+void sweep4(Context& c, const Vec4Grid& v, const LevelArgs& a, const float* r, const float* xb, float* out, double* partials) {
+    const uint3 t = make_uint3(v.g.x, v.g.y, v.g.z);
+    const unsigned blocks = xcd_blocks(v.g);
+    if (v.lpr == 32) {
+        if (partials) mg_sweep4_kernel<true, 32><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, out, partials);
+        else mg_sweep4_kernel<false, 32><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, out, nullptr);
+    } else {
+        if (partials) mg_sweep4_kernel<true, 64><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, out, partials);
+        else mg_sweep4_kernel<false, 64><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, out, nullptr);
+    }
+    hip_check(hipGetLastError(), "mg_sweep4_kernel");
+}
+
+void scale4(Context& c, const Vec4Grid& v, const LevelArgs& a, const float* r, float* out) {
+    if (v.lpr == 32) mg_scale4_kernel<32><<<v.g, 256, 0, c.stream>>>(a, r, out);
+    else mg_scale4_kernel<64><<<v.g, 256, 0, c.stream>>>(a, r, out);
+    hip_check(hipGetLastError(), "mg_scale4_kernel");
+}
+
+void prolong4(Context& c, const Vec4Grid& v, const LevelArgs& a, Coarse cc, float* xb) {
+    if (v.lpr == 32) mg_prolong4_kernel<32><<<v.g, 256, 0, c.stream>>>(a, cc, xb);
+    else mg_prolong4_kernel<64><<<v.g, 256, 0, c.stream>>>(a, cc, xb);
+    hip_check(hipGetLastError(), "mg_prolong4_kernel");
+}
+
+void restrict4(Context& c, const Vec4Grid& v, const LevelArgs& a, const float* r, const float* xb, const MgLevel& C) {
+    const uint3 t = make_uint3(v.gc.x, v.gc.y, v.gc.z);
+    const unsigned blocks = xcd_blocks(v.gc);
+    if (v.lpr == 32) mg_restrict4_kernel<32><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, C.n[0], C.n[1], C.flags, C.r);
+    else mg_restrict4_kernel<64><<<blocks, 256, 0, c.stream>>>(a, t, r, xb, C.n[0], C.n[1], C.flags, C.r);
+    hip_check(hipGetLastError(), "mg_restrict4_kernel");
+}
+
 void sweep(Context& c, const MgLevel& L, int xin, const float* r, const float* xb, Coarse cc, float* out) {
     const LevelArgs a = level_args(c, L);
     const dim3 g = cell_grid(L.n);
@@ -725,11 +782,10 @@ void vcycle(Context& c, int l) {
     // float4 rows: x extent a multiple of 4 (and of 8 when the coarse x pair of
     // the restriction must be a float2-aligned pair: nc0 even)
     const bool vec = split && L.n[0] % 8 == 0;
-    const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
-    const uint3 t4 = make_uint3(g4.x, g4.y, g4.z);
+    const Vec4Grid v4 = vec4_grid(L);
     float* xb = nullptr;   // x in a buffer; nullptr: x = omega D^-1 r on the fly
     if (split) {
-        if (vec) mg_scale4_kernel<<<g4, 256, 0, c.stream>>>(a, L.r, L.ta);
+        if (vec) scale4(c, v4, a, L.r, L.ta);
         else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, L.r, L.ta);
         hip_check(hipGetLastError(), "mg_scale_kernel");
         xb = L.ta;
@@ -738,7 +794,7 @@ void vcycle(Context& c, int l) {
         if (ev) hip_check(hipEventRecord(ev[0], c.stream), "event");
         for (int k = 2; k <= o.pre_sweeps; ++k) {
             float* dst = xb == L.ta ? L.tb : L.ta;
-            if (vec) mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, xb, dst, nullptr);
+            if (vec) sweep4(c, v4, a, L.r, xb, dst, nullptr);
             else sweep(c, L, kBuffer, L.r, xb, none, dst);
             xb = dst;
         }
@@ -756,9 +812,7 @@ void vcycle(Context& c, int l) {
     }
     // fine-cell threads, two z planes per block
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    const uint3 tc4 = make_uint3(gc4.x, gc4.y, gc4.z);
-    if (vec) mg_restrict4_kernel<<<xcd_blocks(gc4), 256, 0, c.stream>>>(a, tc4, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
+    if (vec) restrict4(c, v4, a, L.r, xb, C);
     else if (xb) mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, L.r, xb, C.n[0], C.n[1], C.flags, C.r);
     else mg_restrict_kernel<kFromR><<<gc, 256, 0, c.stream>>>(a, L.r, nullptr, C.n[0], C.n[1], C.flags, C.r);
     hip_check(hipGetLastError(), "mg_restrict_kernel");
@@ -767,7 +821,7 @@ void vcycle(Context& c, int l) {
     const float* cur;
     int done_sweeps;
     if (split) {
-        if (vec) mg_prolong4_kernel<<<g4, 256, 0, c.stream>>>(a, cc, xb);
+        if (vec) prolong4(c, v4, a, cc, xb);
         else mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, xb);
         hip_check(hipGetLastError(), "mg_prolong_kernel");
         cur = xb;
@@ -786,12 +840,12 @@ void vcycle(Context& c, int l) {
     for (int k = done_sweeps + 1; k <= o.post_sweeps; ++k) {
         float* dst = k == o.post_sweeps ? L.z : (cur == bufs[0] ? bufs[1] : bufs[0]);
         const bool last0 = l == 0 && k == o.post_sweeps;
-        const int64_t blocks4 = int64_t(g4.x) * g4.y * g4.z;
+        const int64_t blocks4 = int64_t(v4.g.x) * v4.g.y * v4.g.z;
         if (vec && last0 && m.dot_partials && blocks4 <= m.dot_capacity) {
-            mg_sweep4_kernel<true><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, cur, dst, m.dot_partials);
+            sweep4(c, v4, a, L.r, cur, dst, m.dot_partials);
             m.dot_blocks = int(blocks4);
         } else if (vec) {
-            mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, L.r, cur, dst, nullptr);
+            sweep4(c, v4, a, L.r, cur, dst, nullptr);
         } else {
             sweep(c, L, kBuffer, L.r, cur, none, dst);
         }
@@ -945,10 +999,8 @@ SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
     const LevelArgs a = level_args(c, L);
     const bool vec = L.cells >= (int64_t(1) << 20) && L.n[0] % 8 == 0;
     const dim3 g = cell_grid(L.n);
-    const dim3 g4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned(L.n[2]));
     const dim3 gc(unsigned((L.n[0] + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    const dim3 gc4(unsigned((L.n[0] / 4 + 63) / 64), unsigned((L.n[1] + 3) / 4), unsigned((L.n[2] + 1) / 2));
-    const uint3 t4 = make_uint3(g4.x, g4.y, g4.z), tc4 = make_uint3(gc4.x, gc4.y, gc4.z);
+    const Vec4Grid v4 = vec4_grid(L);
     const Coarse none{nullptr, 0, 0};
     const Coarse cc{C.z, C.n[0], C.n[0] * C.n[1]};
     const float* r = c.pcg_r;
@@ -970,20 +1022,20 @@ SmokeSimulationMultilevelTiming multilevel_timing(Context& c, int reps) {
     out.vector_rows = vec ? 1 : 0;
     out.repetitions = reps;
     out.scale_us = time([&] {
-        if (vec) mg_scale4_kernel<<<g4, 256, 0, c.stream>>>(a, r, L.ta);
+        if (vec) scale4(c, v4, a, r, L.ta);
         else mg_scale_kernel<<<g, 256, 0, c.stream>>>(a, r, L.ta);
     });
     out.sweep_us = time([&] {
-        if (vec) mg_sweep4_kernel<false><<<xcd_blocks(g4), 256, 0, c.stream>>>(a, t4, r, L.ta, L.tb, nullptr);
+        if (vec) sweep4(c, v4, a, r, L.ta, L.tb, nullptr);
         else mg_sweep_kernel<kBuffer><<<g, 256, 0, c.stream>>>(a, r, L.ta, none, L.tb);
     });
     out.restrict_us = time([&] {
-        if (vec) mg_restrict4_kernel<<<xcd_blocks(gc4), 256, 0, c.stream>>>(a, tc4, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
+        if (vec) restrict4(c, v4, a, r, L.tb, C);
         else mg_restrict_kernel<kBuffer><<<gc, 256, 0, c.stream>>>(a, r, L.tb, C.n[0], C.n[1], C.flags, C.r);
     });
     hip_check(hipMemsetAsync(C.z, 0, size_t(C.cells) * sizeof(float), c.stream), "zero correction");
     out.prolong_us = time([&] {
-        if (vec) mg_prolong4_kernel<<<g4, 256, 0, c.stream>>>(a, cc, L.tb);
+        if (vec) prolong4(c, v4, a, cc, L.tb);
         else mg_prolong_kernel<<<g, 256, 0, c.stream>>>(a, cc, L.tb);
     });
     float* saved = m.capture;
