@@ -150,6 +150,10 @@ struct Context {
     PcgStatus* host_status = nullptr;   // pinned, two slots
     PcgScalars* host_scalars = nullptr; // pinned, two slots (flexible PCG: whole scalar block per iteration)
     hipEvent_t status_event[2] = {};
+    // flexible PCG: r.r reduction per iteration of the last inner solve
+    // (geometric mean; 0 until one converged), which predicts the end of the
+    // next solve before it has a ratio of its own -- scheduling only
+    double fpcg_ratio = 0.0;
     // tolerance mode: dt and max |u_face| of the step, copied to pinned memory
     // behind step_event right after the time-step kernel, so that the solver
     // stats need not drain the stream; the anchor read once per occupancy

@@ -1309,9 +1309,10 @@ void run_tolerance_pcg(Context& c) {
         auto predicted_end = [&](int ahead) {
             if (host.iteration + ahead >= host.max_iterations) return true;
             const size_t k = rr.size() - 1;
-            if (k < 1 || !(rr[k - 1] > 0.0)) return false;
-            const double ratio = rr[k] / rr[k - 1];
-            if (!(ratio < 1.0)) return false;
+            // the last ratio of this solve, or the previous solve's mean one
+            double ratio = c.fpcg_ratio;
+            if (k >= 1) ratio = rr[k - 1] > 0.0 ? rr[k] / rr[k - 1] : 1.0;
+            if (!(ratio > 0.0 && ratio < 1.0)) return false;
             double next = rr[k];
             for (int q = 0; q < ahead; ++q) next *= ratio;
             return next <= 4.0 * host.stop2;
@@ -1334,7 +1335,11 @@ void run_tolerance_pcg(Context& c) {
             ++learned_iterations;
             if (!std::isfinite(host.rr) || !std::isfinite(host.rz)) return false;
             rr.push_back(host.rr);
-            if (host.done) return true;
+            if (host.done) {
+                const size_t its = rr.size() - 1;
+                if (its >= 1 && rr[0] > 0.0 && host.rr > 0.0) c.fpcg_ratio = std::pow(host.rr / rr[0], 1.0 / double(its));
+                return true;
+            }
             const size_t k = rr.size() - 1;
             if (!multilevel) {
                 if (k >= 5 && rr[k] > 0.81 * rr[k - 5]) return false;
