@@ -74,14 +74,21 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, sh
 // tile leaves the chip idle and every wave waits out its chain of gathers).
 // The next K step's loads are issued before the current step's matrix-core
 // work.  Epilogue: + bias, + add, relu, * [mask > 0].
+//
+// ngroups > 1 (split-K blocks only): the output channels are dealt over
+// ngroups consecutive blocks of each tile, 16 NT channels each, so a tiny
+// layer (a few tiles) runs on ngroups times as many CUs; every channel's
+// chain of matrix-core instructions is unchanged (same bits).
 template <int NT, int SPLIT>
 __global__ __launch_bounds__(SPLIT == 1 ? 256 : 64 * SPLIT) void conv_kernel(ConvArgs a, const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                    const float* __restrict__ bias, const uint16_t* __restrict__ add,
                                                    const uint16_t* __restrict__ mask, int relu, uint16_t* __restrict__ y,
-                                                   float* __restrict__ yf, int f32_channels) {
+                                                   float* __restrict__ yf, int f32_channels, int ngroups) {
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;   // wave in block
-    const int64_t wave = SPLIT == 1 ? int64_t(blockIdx.x) * 4 + wib : int64_t(blockIdx.x);
+    const int64_t wave = SPLIT == 1 ? int64_t(blockIdx.x) * 4 + wib : int64_t(blockIdx.x) / ngroups;
+    const int cbase = SPLIT == 1 ? 0 : int(blockIdx.x % unsigned(ngroups)) * 16 * NT;   // first output channel
+    w += int64_t(cbase) * a.kpad;
     const int xt = (a.out[0] + 15) / 16;
     const int64_t tiles = int64_t(a.batch) * a.out[2] * a.out[1] * xt;
     if (wave >= tiles) return;   // uniform per block when SPLIT > 1
@@ -177,10 +184,10 @@ __global__ __launch_bounds__(SPLIT == 1 ? 256 : 64 * SPLIT) void conv_kernel(Con
             for (int nt = 0; nt < NT; ++nt) acc[nt] = acc[nt] + part[q][nt][lane];
         }
     }
-    // lane holds D[4 g + i][m]: voxel xtile * 16 + 4 g + i, channel nt * 16 + m
+    // lane holds D[4 g + i][m]: voxel xtile * 16 + 4 g + i, channel cbase + nt * 16 + m
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-        const int n = nt * 16 + m;
+        const int n = cbase + nt * 16 + m;
         const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -723,12 +730,18 @@ template <int SPLIT>
 void launch_conv_split(const ConvArgs& a, unsigned blocks, const uint16_t* x, const uint16_t* w, const float* bias, const uint16_t* add,
                        const uint16_t* mask, int relu, uint16_t* y, float* yf, int f32_channels, hipStream_t s) {
     constexpr unsigned threads = SPLIT == 1 ? 256 : 64 * SPLIT;
+    if (a.cout % 16 != 0 || a.cout < 16 || a.cout > 64) throw HipError("conv: cout must be 16..64");
+    if constexpr (SPLIT == 16) {
+        // the fewest tiles: one 16-channel group per block
+        const int groups = a.cout / 16;
+        conv_kernel<1, SPLIT><<<blocks * unsigned(groups), threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels, groups);
+        return;
+    }
     switch (a.cout / 16) {
-        case 1: conv_kernel<1, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 2: conv_kernel<2, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 3: conv_kernel<3, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        case 4: conv_kernel<4, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels); break;
-        default: throw HipError("conv: cout must be 16..64");
+        case 1: conv_kernel<1, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels, 1); break;
+        case 2: conv_kernel<2, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels, 1); break;
+        case 3: conv_kernel<3, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels, 1); break;
+        default: conv_kernel<4, SPLIT><<<blocks, threads, 0, s>>>(a, x, w, bias, add, mask, relu, y, yf, f32_channels, 1); break;
     }
 }
 
