@@ -355,6 +355,27 @@ def test_adaptive_time_step():
     assert stats.dt == pytest.approx(min(0.01, 0.5 * np.float32(cfg.cell_size) / vmax), rel=1e-6)
 
 
+def test_stats_follow_the_last_step_across_modes():
+    """In tolerance mode the stats take dt and the face speed from the copy
+    made after the step's time-step kernel; after a switch to the fixed mode
+    (the captured step) they must report that step's fixed dt, not the
+    tolerance step's adaptive one."""
+    cfg = cfg_for("closed")
+    st = make_state(cfg, 11)
+    with abi.SmokeContext(cfg) as ctx:
+        load_gpu(ctx, st)
+        tolerance_options(ctx, adaptive=True, cfl=0.5, dt_max=0.004)
+        ctx.step()
+        adaptive_dt = ctx.stats().dt
+        ctx.step()
+        assert ctx.stats().dt > 0.0
+        ctx.set_solver_options(pressure_mode=abi.PRESSURE_FIXED_ITERATIONS, adaptive_dt=0)
+        ctx.step()
+        fixed = ctx.stats()
+    assert adaptive_dt < np.float32(cfg.dt)
+    assert fixed.dt == np.float32(cfg.dt)
+
+
 @pytest.mark.parametrize("warm", [1, 2])
 def test_checkpoint_roundtrip_and_determinism(tmp_path, warm):
     """Restoring a checkpoint continues the run bitwise -- with the

@@ -658,7 +658,8 @@ SmokeSimulationResult smoke_simulation_ext_get_solver_stats(SmokeSimulationConte
         }
         float dt = 0.f, speed = 0.f;
         int anchor = c.anchor_host;
-        if (c.step_info_pending && anchor >= 0) {
+        const bool tolerance = c.options.pressure_mode != SMOKE_SIMULATION_PRESSURE_FIXED_ITERATIONS;
+        if (tolerance && c.step_info_pending && anchor >= 0) {
             // the copies made right after the last step's time-step kernel:
             // no wait for the rest of the step
             hip_check(hipEventSynchronize(c.step_event), "step info");
